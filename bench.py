@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Benchmark of the Mars range x domain x isometry search on one MI355X.
+
+Workload (the quadtree one): a 512x512 synthetic 8-bit plane (seed 2), quadtree
+range sizes 32 -> 16 -> 8 -> 4, domain stride 4, 8 isometries, exhaustive.
+The searched node set of the RD walk is every aligned node of every size
+(independent of lambda), so one step is the whole-plane search table:
+  256 x 12769 + 1024 x 14641 + 4096 x 15625 + 16384 x 16129 ranges x positions,
+  x 8 isometries = 2.77e9 evals.
+A step runs pool build + range prep + search + merge on device-resident data
+(plane in HBM at the start, winners table in HBM at the end), replayed from
+one captured HIP graph.  `value` is evals/s of the split-INT8 MFMA kernel; the
+plain FP32-FMA kernel is timed beside it on the same workload.
+
+Prints ONE JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_INT8_OPS = 5.0e15   # MI355X dense INT8 MFMA, ops/s
+PEAK_HBM = 8.0e12        # bytes/s
+
+W = H = 512
+SEED = 2
+MIN_SIZE, MAX_SIZE, SHIFT = 4, 32, 4
+
+
+def _gpu_leg(steps: int, warmup: int):
+    from mars_amd import _native as N
+    from mars_amd import synth
+    from mars_amd.provider import EncodeParams, HipContext
+
+    img = synth.plane(W, H, SEED)
+    params = EncodeParams(min_size=MIN_SIZE, max_size=MAX_SIZE, shift=SHIFT)
+    res = {}
+    for name, kern in (("mfma_i8", N.KERNEL_MFMA_I8), ("fp32", N.KERNEL_FP32)):
+        ctx = HipContext(0, kern)
+        ctx.prepare(img, params, [SHIFT])
+        if warmup:
+            ctx.run(iters=warmup, use_graph=True)
+        # Per-kernel breakdown: average of several eager steps (HIP events on
+        # the context stream around each launch).
+        br = [ctx.run(iters=1, use_graph=False) for _ in range(5)]
+        t = ctx.run(iters=steps, use_graph=True)
+        by_size = {str(2 << i): float(np.mean([b.ms_by_size[i] for b in br]))
+                   for i in range(6) if br[0].ms_by_size[i] > 0}
+        tabs = ctx.tables()
+        evals_by_size = {str(s): len(rows) * P * 8 for s, sh, rows, cols, c, P in tabs}
+        res[name] = dict(ms_step=t.ms_step, evals=t.evals, ms_pool=float(np.mean([b.ms_pool for b in br])),
+                         ms_search=float(np.mean([b.ms_search for b in br])),
+                         ms_by_size=by_size, evals_by_size=evals_by_size,
+                         survivors=int(br[0].survivors),
+                         winners=[[(c.dom_row, c.dom_col, c.isometry, c.qalfa, c.qbeta)
+                                   if c is not None else None for c in tt[4]] for tt in tabs])
+        ctx.close()
+    return img, res
+
+
+def _cpu_leg(img, budget_s: float = 12.0):
+    """The oracle (C restatement of the reference CPU search) on the host cores,
+    on a bounded sample of the same workload: ranges drawn from every size."""
+    from tests import _oracle as O
+
+    threads = os.cpu_count() or 1
+    nodes = O.nodes(W, H, MIN_SIZE, MAX_SIZE)
+    rng = np.random.default_rng(0)
+    evals = 0
+    t0 = time.perf_counter()
+    used = {}
+    rounds = 0
+    while time.perf_counter() - t0 < budget_s and rounds < 50:
+        for size in sorted(nodes):
+            pick = rng.choice(len(nodes[size]), size=threads, replace=False)
+            rows = [nodes[size][i][0] for i in pick]
+            cols = [nodes[size][i][1] for i in pick]
+            _, ev = O.search_table(img, size, SHIFT, rows, cols, threads=threads)
+            evals += ev
+            used[size] = used.get(size, 0) + len(rows)
+        rounds += 1
+    dt = time.perf_counter() - t0
+    return dict(value=evals / dt, unit="evals/s", cores=threads, kind="port",
+                sample=f"{sum(used.values())} ranges ({', '.join(f'{v} of size {k}' for k, v in sorted(used.items()))}) "
+                       f"of the 512x512 quadtree workload, {evals:.3g} evals in {dt:.1f} s, "
+                       f"oracle/liboracle.so over {threads} threads")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU's share of the search; --gpus {args.gpus} is not "
+              "supported (the search shards by range block with no exchange but a per-range "
+              "winner gather; see the multi-GPU paragraph of DESIGN.md)", file=sys.stderr)
+        return 2
+
+    img, res = _gpu_leg(args.steps, args.warmup)
+    mf, fp = res["mfma_i8"], res["fp32"]
+    evals = mf["evals"]
+    value = evals / (mf["ms_step"] * 1e-3)
+    fp_value = fp["evals"] / (fp["ms_step"] * 1e-3)
+    # Dominant kernel: the search launch with the largest share of the step.
+    dom_size = max(mf["ms_by_size"], key=lambda s: mf["ms_by_size"][s])
+    dom_ms = mf["ms_by_size"][dom_size]
+    s = int(dom_size)
+    ops = mf["evals_by_size"][dom_size] * 2 * s * s   # 2*s^2 integer ops per eval
+    achieved = ops / (dom_ms * 1e-3)
+    diverge = sum(1 for a, b in zip(sum(mf["winners"], []), sum(fp["winners"], [])) if a != b)
+    n_ranges = sum(len(t) for t in mf["winners"])
+
+    cpu = None if args.no_cpu else _cpu_leg(img)
+    line = {
+        "metric": "R×D×iso evals/s, split-INT8 MFMA vs FP32 kernel; batched img/s @512²",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mf["ms_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8 MFMA (i32 acc) + f64 fit",
+        "data": "synthetic (seeded generator, mars_amd/synth.py, seed 2)",
+        "config": {"workload": "quadtree512: 512x512 u8 plane, range sizes 32->4, domain stride 4, "
+                               "8 isometries, exhaustive, whole RD node set per step",
+                   "evals_per_step": evals, "ranges": n_ranges},
+        "images_per_s": 1e3 / mf["ms_step"],
+        "fp32_kernel": {"value": fp_value, "unit": "evals/s", "ms_per_step": fp["ms_step"],
+                        "ranges_differing_from_mfma_winner": diverge},
+        "speedup_vs_fp32": value / fp_value,
+        "breakdown_ms": {"pool_and_ranges": mf["ms_pool"], "search": mf["ms_search"],
+                         "search_by_size": mf["ms_by_size"],
+                         "fp32_search_by_size": fp["ms_by_size"]},
+        "survivor_fraction": mf["survivors"] / evals,
+        "roofline": {"bound": "mfma", "kernel": f"k_search_mfma<{s}>", "achieved": achieved / 1e12,
+                     "peak": PEAK_INT8_OPS / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_INT8_OPS,
+                     "traffic": None, "op": "int8 ops (2*s^2 per eval)",
+                     "avg_launch_ms": dom_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

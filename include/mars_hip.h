@@ -1,0 +1,138 @@
+/*
+ * mars_hip.h -- C ABI of the MI355X (gfx950) range x domain x isometry search
+ * backend for the Mars fractal codec.
+ *
+ * The reference plugs search backends in through the Rust trait
+ *   trait SearchProvider: Sync {
+ *       fn search(&self, &SearchRequest) -> SearchOutcome;
+ *       fn search_candidates(&self, &SearchRequest, max) -> RankedCandidates }
+ * (crates/mars-codec/src/encode.rs:557-579), built once per plane by the
+ * factory passed to encode_image_with_search (encode.rs:1054-1076).  This
+ * library is what a thin FFI provider crate binds: `mars_hip_prepare` is the
+ * factory body (it owns the plane's pool and runs the whole-plane search table
+ * on the GPU), `mars_hip_search` is SearchProvider::search (a read-only,
+ * thread-safe table lookup).  The binding is shown in INTEGRATION.md.
+ *
+ * Conventions: status 0 = ok, < 0 = error (mars_hip_last_error() says why).
+ * Nothing unwinds across the ABI.  No CPU search fallback exists: a caller
+ * that needs one must own it.
+ */
+#ifndef MARS_HIP_H
+#define MARS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MARS_HIP_OK 0
+#define MARS_HIP_E_ARG (-1)         /* bad argument / unsupported geometry */
+#define MARS_HIP_E_DEVICE (-2)      /* HIP runtime error */
+#define MARS_HIP_E_NOT_PREPARED (-3)
+#define MARS_HIP_E_NOT_IN_TABLE (-4) /* (row, col, size, shift) was not prepared */
+#define MARS_HIP_E_UNSUPPORTED (-5)
+
+/* Search kernels (mars_hip_set_kernel). */
+#define MARS_HIP_KERNEL_MFMA_I8 0 /* split-INT8 MFMA + exact certified epilogue (default) */
+#define MARS_HIP_KERNEL_FP32 1    /* plain FP32-FMA correlation + binary32 fit (baseline) */
+
+typedef struct mars_hip_ctx mars_hip_ctx;
+
+/* The EncodeParams fields the search reads (encode.rs:27-60). */
+typedef struct {
+    uint32_t min_size;
+    uint32_t max_size;
+    uint32_t shift;
+    uint32_t bits_alfa;
+    uint32_t bits_beta;
+    uint32_t _reserved;
+    double max_alfa;
+} mars_hip_params;
+
+/* encode.rs:224-244 Candidate with its RawMoments. */
+typedef struct {
+    uint32_t dom_row;
+    uint32_t dom_col;
+    uint32_t isometry;
+    uint32_t qalfa;
+    uint32_t qbeta;
+    uint32_t _pad;
+    double rms;
+    int64_t s0, s1_x4, s2_x16, t0, t1_x4, t2;
+} mars_hip_candidate;
+
+/* Per-run device timings (milliseconds, HIP events on the context stream). */
+typedef struct {
+    double ms_step;     /* whole step: pool build + range prep + search + merge */
+    double ms_pool;     /* contraction + per-size window sums */
+    double ms_search;   /* search kernels, all sizes */
+    double ms_by_size[6]; /* search kernel per size 2,4,8,16,32,64 (index log2(size)-1) */
+    uint64_t evals;     /* range x position x isometry candidates per step */
+    uint64_t survivors; /* candidates that reached the binary64 fit (MFMA kernel) */
+} mars_hip_timing;
+
+/* Open a context on HIP device `device` (0-based).  Creates its own stream. */
+int mars_hip_open(int device, mars_hip_ctx **out);
+void mars_hip_close(mars_hip_ctx *ctx);
+
+/*
+ * Factory body (encode.rs:1054-1076 make_search + the provider's own index):
+ * copy the w x h u8 plane to the device, build the domain pool, and run the
+ * search for every range node a quadtree walk with these params can request
+ * -- every aligned non-forced node from max_size down to min_size, plus the
+ * border nodes down to size 2 (encode.rs:1330-1362, 1865-1919) -- once per
+ * stride in shifts[0..n_shifts).  Single-threaded; copies its inputs.
+ * Supported sizes: 2..32.
+ */
+int mars_hip_prepare(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t h,
+                     const mars_hip_params *p, const uint32_t *shifts, uint32_t n_shifts);
+
+/*
+ * SearchProvider::search (encode.rs:557-602): the winner for the range block
+ * (row, col, size) at domain stride `shift`.  k must be 1 (the default
+ * search_candidates of encode.rs:572-578 returns the winner only).  *n_out is
+ * 0 when no domain position is legal; *evals is the number of fitted
+ * candidates the reference counts (encode.rs:768).  Thread-safe, read-only.
+ */
+int mars_hip_search(const mars_hip_ctx *ctx, uint32_t row, uint32_t col, uint32_t size,
+                    uint32_t shift, uint32_t k, mars_hip_candidate *out, uint32_t *n_out,
+                    uint64_t *evals);
+
+/* Last error message of this context ("" if none). */
+const char *mars_hip_last_error(const mars_hip_ctx *ctx);
+
+/* ---- measurement and diagnostics (no reference counterpart) ---- */
+
+/* Select the search kernel used by later prepare/run calls. */
+int mars_hip_set_kernel(mars_hip_ctx *ctx, int kernel);
+
+/*
+ * Re-run the prepared step `iters` times on device-resident data (no host
+ * copies inside), replayed from one captured HIP graph when use_graph != 0,
+ * then refresh the lookup table.  Fills *timing with per-step averages.
+ */
+int mars_hip_run(mars_hip_ctx *ctx, int iters, int use_graph, mars_hip_timing *timing);
+
+/* Number of prepared (size, shift) tables and the nodes of table i. */
+int mars_hip_table_count(const mars_hip_ctx *ctx, uint32_t *n_tables);
+int mars_hip_table_info(const mars_hip_ctx *ctx, uint32_t i, uint32_t *size, uint32_t *shift,
+                        uint32_t *n_ranges, uint32_t *n_positions);
+/* Copy table i's winners (n_ranges entries, found[j] = 0 for no candidate). */
+int mars_hip_table_read(const mars_hip_ctx *ctx, uint32_t i, uint32_t *rows, uint32_t *cols,
+                        mars_hip_candidate *out, uint8_t *found);
+
+/*
+ * Device self-test of the binary64 fit: evaluates the search's own fit
+ * (encode.rs:247-289) on the GPU for n moment sets (6 int64 each, in
+ * RawMoments order) and writes qalfa, qbeta, rms.
+ */
+int mars_hip_fit_f64(mars_hip_ctx *ctx, const int64_t *moments, uint32_t n, double max_alfa,
+                     uint32_t bits_alfa, uint32_t bits_beta, uint32_t *qalfa, uint32_t *qbeta,
+                     double *rms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,69 @@
+// Shared declarations between the device code (kernels.hip) and the host
+// side of the C ABI (api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "mars_hip.h"
+
+namespace mars {
+
+// Per-(range, position chunk) partial winner.
+struct Partial {
+    double rms;
+    uint32_t key; // (position index * 8 + isometry); 0xffffffff = none
+    uint32_t pad;
+};
+
+struct SearchArgs {
+    const int8_t *rng8;      // [nr_pad][8][kmem] r - 128
+    const float *rngf;       // [nr_pad/4][K][32] r (FP32 baseline)
+    const int32_t *rstat;    // [nr_pad][2] t0, t2
+    uint32_t *gthr;          // [nr_pad] pruning threshold (float bits, SSE units)
+    const int32_t *pos;      // [P][2] S1, S2
+    const float *posdi;      // [P] n*S2 - S1^2
+    const int8_t *lo, *hi;   // split planes, 16 shifted copies each
+    size_t copy_stride;      // bytes between copies
+    uint32_t pitch;          // bytes per contracted row in a copy
+    const float *dplanef;    // contracted plane as f32 (FP32 baseline)
+    uint32_t wc;             // contracted width
+    uint32_t P, npc, shift;  // positions, positions per row, stride
+    uint32_t nr, ngroups, nchunks, tiles_per_chunk;
+    double max_alfa;
+    uint32_t bits_alfa, bits_beta;
+    Partial *partial;        // [nr_pad][nchunks]
+    unsigned long long *counters;
+};
+
+struct MergeArgs {
+    const Partial *partial;
+    uint32_t nchunks, nr, P, npc, shift, size;
+    const uint32_t *rows, *cols;
+    const uint8_t *px;
+    uint32_t w;
+    const int32_t *d;
+    uint32_t wc;
+    const int32_t *pos, *rstat;
+    double max_alfa;
+    uint32_t bits_alfa, bits_beta;
+    mars_hip_candidate *out;
+    uint8_t *found;
+};
+
+int launch_search(int kernel, uint32_t size, const SearchArgs &a, hipStream_t st);
+void launch_contract(const uint8_t *px, uint32_t w, uint32_t wc, uint32_t hc, uint32_t pitch,
+                     size_t copy_stride, int32_t *d, float *df, int8_t *lo, int8_t *hi,
+                     unsigned long long *counters, uint32_t n_counters, hipStream_t st);
+void launch_possums(const int32_t *d, uint32_t wc, uint32_t size, uint32_t shift, uint32_t npc,
+                    uint32_t P, int32_t *pos, float *posdi, hipStream_t st);
+void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const uint32_t *cols,
+                   uint32_t nr, uint32_t nr_pad, uint32_t size, uint32_t kmem, int8_t *rng8,
+                   float *rngf, int32_t *rstat, uint32_t *gthr, hipStream_t st);
+void launch_merge(const MergeArgs &a, hipStream_t st);
+void launch_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
+                         uint32_t bits_beta, uint32_t *qa, uint32_t *qb, double *rms,
+                         hipStream_t st);
+
+} // namespace mars

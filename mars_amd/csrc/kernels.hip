@@ -1,0 +1,838 @@
+// Device code of the Mars search backend for MI355X (gfx950, CDNA4).
+//
+// Data flow for one plane (all device resident):
+//   k_contract   u8 plane -> contracted D (i32, the 2x2 box sums of
+//                encode.rs:473-501) + the split planes D = 256*Dh + Dl, stored
+//                as i8 (Dl-128, Dh) in 16 byte-shifted copies so that any
+//                4/8/16-byte window row segment is one aligned load.
+//   k_possums    per (size, shift) table: per domain position the window sums
+//                S1 = sum D, S2 = sum D^2 (moments.rs:21-44) and
+//                Di = s0*S2 - S1^2.
+//   k_ranges     per table: each range block permuted under the 8 isometries
+//                (encode.rs:833-842) as i8 r-128 (MFMA operand) and as f32
+//                (FP32 baseline operand), plus t0 = sum r, t2 = sum r^2.
+//   k_search_mfma<S>  range x position x isometry correlation as a GEMM
+//                [8 ranges x 8 isometries] x [32 positions] per wave tile,
+//                K = S*S, on v_mfma_i32_32x32x{16,32}_i8 with hi/lo split of
+//                D, then the certified epilogue described at the kernel.
+//   k_search_fp32<S>  the plain baseline: FP32 FMA correlation + binary32
+//                fit (encode.rs:295-337) for every candidate.
+//   k_merge      per range: min over position chunks by (rms, key), the
+//                winner's moments and (qalfa, qbeta) recomputed exactly.
+//
+// Every parity-bearing floating-point expression is compiled with
+// -ffp-contract=off (and `#pragma clang fp contract(off)` below) so the
+// binary64 fit rounds exactly like the reference's Rust.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace mars {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// ---------------------------------------------------------------- fits
+
+// encode.rs:446-448 quantise_f64
+__device__ __forceinline__ uint32_t quantise_f64(double x, uint32_t max)
+{
+    double t = trunc(0.5 + x);
+    t = t < 0.0 ? 0.0 : t;
+    t = t > (double)max ? (double)max : t;
+    return (uint32_t)t;
+}
+
+__device__ __forceinline__ uint32_t quantise_f32(float x, uint32_t max)
+{
+    float t = truncf(0.5f + x);
+    t = t < 0.0f ? 0.0f : t;
+    t = t > (float)max ? (float)max : t;
+    return (uint32_t)t;
+}
+
+// encode.rs:247-289 fit_f64, same operation order.  Also returns `sum`
+// (the SSE before /s0) for the pruning threshold.
+__device__ __forceinline__ void fit_f64(int64_t ms0, int64_t ms1_x4, int64_t ms2_x16, int64_t mt0,
+                                     int64_t mt1_x4, int64_t mt2, double max_alfa,
+                                     uint32_t bits_alfa, uint32_t bits_beta, uint32_t *qa_out,
+                                     uint32_t *qb_out, double *rms_out, double *sum_out)
+{
+    double s0 = (double)ms0;
+    double s1 = (double)ms1_x4 / 4.0;
+    double s2 = (double)ms2_x16 / 16.0;
+    double t0 = (double)mt0;
+    double t1 = (double)mt1_x4 / 4.0;
+    double t2 = (double)mt2;
+
+    double det = s0 * s2 - s1 * s1;
+    double alfa = det == 0.0 ? 0.0 : (s0 * t1 - s1 * t0) / det;
+    if (alfa < 0.0) alfa = 0.0;
+
+    uint32_t max_qalfa = (1u << bits_alfa) - 1;
+    uint32_t qalfa = quantise_f64(alfa / max_alfa * (double)(1u << bits_alfa), max_qalfa);
+    double alfa2 = (double)qalfa / (double)(1u << bits_alfa) * max_alfa;
+
+    double beta = (t0 - alfa2 * s1) / s0;
+    if (alfa2 > 0.0) beta += alfa2 * 255.0;
+    uint32_t max_qbeta = (1u << bits_beta) - 1;
+    uint32_t qbeta =
+        quantise_f64(beta / ((1.0 + fabs(alfa2)) * 255.0) * (double)max_qbeta, max_qbeta);
+    double beta2 = (double)qbeta / (double)max_qbeta * ((1.0 + fabs(alfa2)) * 255.0);
+    if (alfa2 > 0.0) beta2 -= alfa2 * 255.0;
+
+    double sum = t2 - 2.0 * alfa2 * t1 - 2.0 * beta2 * t0 + alfa2 * alfa2 * s2 +
+                 2.0 * alfa2 * beta2 * s1 + s0 * beta2 * beta2;
+    double q = sum / s0;
+    *rms_out = sqrt(q > 0.0 ? q : 0.0);
+    *qa_out = qalfa;
+    *qb_out = qbeta;
+    *sum_out = sum;
+}
+
+// encode.rs:295-337 fit_f32 (the reference's own binary32 twin), used by the
+// plain FP32 baseline kernel.
+__device__ __forceinline__ void fit_f32(float s0, float s1_x4, float s2_x16, float t0,
+                                        float t1_x4, float t2, float max_alfa,
+                                        uint32_t bits_alfa, uint32_t bits_beta, uint32_t *qa_out,
+                                        uint32_t *qb_out, float *rms_out)
+{
+    float s1 = s1_x4 / 4.0f;
+    float s2 = s2_x16 / 16.0f;
+    float t1 = t1_x4 / 4.0f;
+    float det = s0 * s2 - s1 * s1;
+    float alfa = det == 0.0f ? 0.0f : (s0 * t1 - s1 * t0) / det;
+    if (alfa < 0.0f) alfa = 0.0f;
+    uint32_t max_qalfa = (1u << bits_alfa) - 1;
+    uint32_t qalfa = quantise_f32(alfa / max_alfa * (float)(1u << bits_alfa), max_qalfa);
+    float alfa2 = (float)qalfa / (float)(1u << bits_alfa) * max_alfa;
+    float beta = (t0 - alfa2 * s1) / s0;
+    if (alfa2 > 0.0f) beta += alfa2 * 255.0f;
+    uint32_t max_qbeta = (1u << bits_beta) - 1;
+    uint32_t qbeta =
+        quantise_f32(beta / ((1.0f + fabsf(alfa2)) * 255.0f) * (float)max_qbeta, max_qbeta);
+    float beta2 = (float)qbeta / (float)max_qbeta * ((1.0f + fabsf(alfa2)) * 255.0f);
+    if (alfa2 > 0.0f) beta2 -= alfa2 * 255.0f;
+    float sum = t2 - 2.0f * alfa2 * t1 - 2.0f * beta2 * t0 + alfa2 * alfa2 * s2 +
+                2.0f * alfa2 * beta2 * s1 + s0 * beta2 * beta2;
+    float q = sum / s0;
+    *rms_out = sqrtf(q > 0.0f ? q : 0.0f);
+    *qa_out = qalfa;
+    *qb_out = qbeta;
+}
+
+// (rms, key) total order of the reference scan: strict `<` on rms keeps the
+// first enumerated candidate, and enumeration order is the key order
+// (dom_row, dom_col, isometry) -- encode.rs:676-685, 744-786.
+__device__ __forceinline__ bool better(double rms, uint32_t key, double brms, uint32_t bkey)
+{
+    return rms < brms || (rms == brms && key < bkey);
+}
+
+__device__ __forceinline__ int32_t sel32(const int32_t (&v)[4], int i)
+{
+    return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
+}
+__device__ __forceinline__ int64_t sel64(const int64_t (&v)[4], int i)
+{
+    return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
+}
+
+// isometry.rs:7-19
+__device__ __forceinline__ void iso_map(uint32_t k, uint32_t u, uint32_t v, uint32_t size,
+                                        uint32_t *i, uint32_t *j)
+{
+    switch (k) {
+    case 0: *i = u; *j = v; break;
+    case 1: *i = size - 1 - v; *j = u; break;
+    case 2: *i = v; *j = size - 1 - u; break;
+    case 3: *i = size - 1 - u; *j = size - 1 - v; break;
+    case 4: *i = u; *j = size - 1 - v; break;
+    case 5: *i = size - 1 - u; *j = v; break;
+    case 6: *i = v; *j = u; break;
+    default: *i = size - 1 - v; *j = size - 1 - u; break;
+    }
+}
+
+// ---------------------------------------------------------------- pool
+
+// One thread per contracted sample.  Writes D (i32), D as f32, and the
+// i8 planes lo = (D & 255) - 128, hi = D >> 8 into copies c = 0..15 at
+// x = j - c (copy c holds the row shifted left by c bytes).
+__global__ void k_contract(const uint8_t *__restrict__ px, uint32_t w, uint32_t wc, uint32_t hc,
+                           uint32_t pitch, size_t copy_stride, int32_t *__restrict__ d,
+                           float *__restrict__ df, int8_t *__restrict__ lo,
+                           int8_t *__restrict__ hi, unsigned long long *__restrict__ counters,
+                           uint32_t n_counters)
+{
+    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n_counters) counters[idx] = 0;
+    if (idx >= wc * hc) return;
+    uint32_t i = idx / wc, j = idx - i * wc;
+    const uint8_t *p0 = px + (size_t)(2 * i) * w + 2 * j;
+    int32_t v = (int32_t)p0[0] + (int32_t)p0[1] + (int32_t)p0[w] + (int32_t)p0[w + 1];
+    d[idx] = v;
+    df[idx] = (float)v;
+    int8_t l = (int8_t)((v & 255) - 128);
+    int8_t hh = (int8_t)(v >> 8);
+#pragma unroll
+    for (uint32_t c = 0; c < 16; c++) {
+        if (j >= c) {
+            size_t o = c * copy_stride + (size_t)i * pitch + (j - c);
+            lo[o] = l;
+            hi[o] = hh;
+        }
+    }
+}
+
+// One thread per domain position of a table: window sums over D.
+__global__ void k_possums(const int32_t *__restrict__ d, uint32_t wc, uint32_t size,
+                          uint32_t shift, uint32_t npc, uint32_t P, int32_t *__restrict__ pos,
+                          float *__restrict__ posdi)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    uint32_t pi = p / npc, pj = p - pi * npc;
+    uint32_t dr = (pi * shift) >> 1, dc = (pj * shift) >> 1;
+    int64_t s1 = 0, s2 = 0;
+    for (uint32_t u = 0; u < size; u++) {
+        const int32_t *row = d + (size_t)(dr + u) * wc + dc;
+        for (uint32_t v = 0; v < size; v++) {
+            int64_t x = row[v];
+            s1 += x;
+            s2 += x * x;
+        }
+    }
+    pos[2 * p] = (int32_t)s1;
+    pos[2 * p + 1] = (int32_t)s2;
+    int64_t n = (int64_t)size * size;
+    posdi[p] = (float)(n * s2 - s1 * s1);
+}
+
+// One block per (padded) range.  rng8 layout [range][iso][kmem] (i8, r-128,
+// zero padded to kmem); rngf layout [range/4][k][iso*4 + range%4] (f32 r).
+__global__ void k_ranges(const uint8_t *__restrict__ px, uint32_t w,
+                         const uint32_t *__restrict__ rows, const uint32_t *__restrict__ cols,
+                         uint32_t nr, uint32_t size, uint32_t kmem, int8_t *__restrict__ rng8,
+                         float *__restrict__ rngf, int32_t *__restrict__ rstat,
+                         uint32_t *__restrict__ gthr)
+{
+    uint32_t r = blockIdx.x;
+    uint32_t K = size * size;
+    __shared__ int32_t st[2];
+    if (threadIdx.x == 0) {
+        st[0] = 0;
+        st[1] = 0;
+    }
+    __syncthreads();
+    bool valid = r < nr;
+    uint32_t row = valid ? rows[r] : 0, col = valid ? cols[r] : 0;
+    for (uint32_t e = threadIdx.x; e < 8 * kmem; e += blockDim.x) {
+        uint32_t iso = e / kmem, k = e - iso * kmem;
+        int8_t val = 0;
+        float fv = 0.0f;
+        if (valid && k < K) {
+            uint32_t u = k / size, v = k - u * size, i, j;
+            iso_map(iso, u, v, size, &i, &j);
+            uint32_t pix = px[(size_t)(row + i) * w + col + j];
+            val = (int8_t)((int)pix - 128);
+            fv = (float)pix;
+        }
+        rng8[((size_t)r * 8 + iso) * kmem + k] = val;
+        if (k < K) rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = fv;
+    }
+    int32_t s = 0, s2 = 0;
+    if (valid)
+        for (uint32_t e = threadIdx.x; e < K; e += blockDim.x) {
+            uint32_t u = e / size, v = e - u * size;
+            int32_t pix = px[(size_t)(row + u) * w + col + v];
+            s += pix;
+            s2 += pix * pix;
+        }
+    atomicAdd(&st[0], s);
+    atomicAdd(&st[1], s2);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        rstat[2 * r] = st[0];
+        rstat[2 * r + 1] = st[1];
+        gthr[r] = 0x7f800000u; // +inf: no threshold yet
+    }
+}
+
+// ---------------------------------------------------------------- MFMA search
+
+template <int S> struct Geo {
+    static constexpr int K = S * S;
+    static constexpr bool SMALLK = S <= 4;                 // v_mfma_i32_32x32x16_i8
+    static constexpr int KSTEP = SMALLK ? 16 : 32;         // K per instruction
+    static constexpr int KMEM = K < 16 ? 16 : K;           // padded range row length
+    static constexpr int NK = (KMEM + KSTEP - 1) / KSTEP;  // K steps
+    static constexpr bool PRELOAD_A = NK <= 2;             // keep A in registers
+};
+
+// Load 4/8/16 aligned bytes of a split plane: plane[row][col .. col+n).
+__device__ __forceinline__ uint32_t ld4(const int8_t *base, size_t cs, uint32_t pitch,
+                                        uint32_t row, uint32_t col)
+{
+    uint32_t c = col & 3;
+    return *(const uint32_t *)(base + c * cs + (size_t)row * pitch + (col - c));
+}
+__device__ __forceinline__ uint2 ld8(const int8_t *base, size_t cs, uint32_t pitch,
+                                     uint32_t row, uint32_t col)
+{
+    uint32_t c = col & 7;
+    return *(const uint2 *)(base + c * cs + (size_t)row * pitch + (col - c));
+}
+__device__ __forceinline__ v4i ld16(const int8_t *base, size_t cs, uint32_t pitch,
+                                    uint32_t row, uint32_t col)
+{
+    uint32_t c = col & 15;
+    return *(const v4i *)(base + c * cs + (size_t)row * pitch + (col - c));
+}
+
+// B fragment (one split plane) of lane half h for K-step kk at window
+// origin (dr, dc).  The k order inside a fragment is the same one used for
+// the A fragment, so the product does not depend on how the instruction
+// orders K internally.
+template <int S>
+__device__ __forceinline__ void load_b(const int8_t *plane, size_t cs, uint32_t pitch,
+                                       uint32_t dr, uint32_t dc, int kk, int h, long *b8, v4i *b16)
+{
+    if constexpr (S == 2) {
+        // K = 4 padded to 16: half 0 holds k 0..3 = (0,0) (0,1) (1,0) (1,1).
+        uint32_t x = 0;
+        if (h == 0) {
+            const uint8_t *p = (const uint8_t *)plane; // copy 0 = unshifted plane
+            uint32_t a0 = p[(size_t)dr * pitch + dc], a1 = p[(size_t)dr * pitch + dc + 1];
+            uint32_t a2 = p[(size_t)(dr + 1) * pitch + dc],
+                     a3 = p[(size_t)(dr + 1) * pitch + dc + 1];
+            x = a0 | (a1 << 8) | (a2 << 16) | (a3 << 24);
+        }
+        *b8 = (long)x;
+    } else if constexpr (S == 4) {
+        uint32_t r0 = ld4(plane, cs, pitch, dr + 2 * h, dc);
+        uint32_t r1 = ld4(plane, cs, pitch, dr + 2 * h + 1, dc);
+        *b8 = (long)((uint64_t)r0 | ((uint64_t)r1 << 32));
+    } else if constexpr (S == 8) {
+        uint32_t u = 4 * kk + 2 * h;
+        uint2 r0 = ld8(plane, cs, pitch, dr + u, dc);
+        uint2 r1 = ld8(plane, cs, pitch, dr + u + 1, dc);
+        v4i t;
+        t.x = (int)r0.x;
+        t.y = (int)r0.y;
+        t.z = (int)r1.x;
+        t.w = (int)r1.y;
+        *b16 = t;
+    } else {
+        uint32_t k0 = 32 * kk + 16 * h;
+        uint32_t u = k0 / S, v0 = k0 % S;
+        *b16 = ld16(plane, cs, pitch, dr + u, dc + v0);
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void load_a(const int8_t *rng8, uint32_t g, int tt, int kk, int lane,
+                                       long *a8, v4i *a16)
+{
+    constexpr int KMEM = Geo<S>::KMEM;
+    constexpr int KSTEP = Geo<S>::KSTEP;
+    int m = lane & 31, h = lane >> 5;
+    int rl = ((m >> 2) & 1) * 4 + (m & 3);
+    int iso = 4 * tt + (m >> 3);
+    const int8_t *p = rng8 + ((size_t)(g * 8 + rl) * 8 + iso) * KMEM + kk * KSTEP + h * (KSTEP / 2);
+    if constexpr (Geo<S>::SMALLK)
+        *a8 = *(const long *)p;
+    else
+        *a16 = *(const v4i *)p;
+}
+
+struct BestRec {
+    double rms;
+    uint32_t key;
+    uint32_t pad;
+};
+
+constexpr int T1_STRIDE = 36; // ints per lane in the survivor staging area
+
+// Certified split-INT8 search for one range size S.
+//
+// Grid: x = position chunk, y = group of 8 ranges.  Block = 4 waves; each
+// wave takes the chunk's 32-position tiles round robin.  A wave tile is
+// 64 GEMM rows (8 ranges x 8 isometries, two 32-row MFMA tiles) x 32
+// positions.  Output layout of v_mfma_i32_32x32: item t of lane l is
+// row 8*(t/4) + 4*(l/32) + t%4, column l%32; rows are arranged so that lane
+// half h owns ranges 4h..4h+3 and item t of row tile tt is range t%4,
+// isometry 4*tt + t/4.
+//
+// Correlation.  r' = r - 128 (i8), D = 256*Dh + Dl, Dl' = Dl - 128 (i8):
+//   t1_x4 = sum r*D = 256*sum r'Dh + sum r'Dl' + 128*(sum r' + S1)     (exact)
+//
+// Epilogue.  For every candidate the least-squares residual with free slope
+// and offset is a lower bound of its quantised SSE:
+//   n*SSE >= A - Ni^2/Di,  A = n*t2 - t0^2, Di = n*S2 - S1^2,
+//   Ni = n*t1 - S1*t0 (moments in the x4/x16 units of encode.rs:224-232).
+// Given a threshold tau (an SSE some candidate of this range already
+// reached, inflated so that anything above it has a strictly larger binary64
+// rms), a candidate can only win if Ni >= sqrt((A - n tau) Di), i.e.
+//   t1_x4 > floor((S1*t0 + sqrt((A - n tau) Di)) / n) - 1.
+// That is one integer compare per candidate; the limit is computed once per
+// (range, position) with a 1e-5 relative safety margin on the square root.
+// Candidates with Ni < 0 are covered too (their qalfa is 0, SSE >= A/n).
+// Survivors get the exact binary64 fit of encode.rs:247-289 and compete by
+// (rms, key); winners tighten tau through LDS and global atomics.
+template <int S>
+__global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
+{
+    using G = Geo<S>;
+    constexpr int NK = G::NK;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane >> 5, jj = lane & 31;
+    const uint32_t g = blockIdx.y, chunk = blockIdx.x;
+    const int64_t n = (int64_t)S * S;
+    const double inv_n = 1.0 / (double)n;
+
+    __shared__ uint32_t s_thr[8];
+    __shared__ BestRec s_best[4][64][4];
+    __shared__ __align__(16) int32_t s_t1[4][64 * T1_STRIDE];
+
+    if (threadIdx.x < 8) {
+        uint32_t r = g * 8 + threadIdx.x;
+        s_thr[threadIdx.x] = r < a.nr ? a.gthr[r] : 0x7f800000u;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        s_best[wave][lane][i].rms = __longlong_as_double(0x7ff0000000000000ll);
+        s_best[wave][lane][i].key = 0xffffffffu;
+    }
+
+    // Range statistics of the lane's 4 ranges.
+    int64_t T0[4], T2[4];
+    double A[4];
+    int32_t srp[4];
+    bool rvalid[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        uint32_t r = g * 8 + 4 * h + i;
+        rvalid[i] = r < a.nr;
+        uint32_t rr = rvalid[i] ? r : 0;
+        T0[i] = a.rstat[2 * rr];
+        T2[i] = a.rstat[2 * rr + 1];
+        A[i] = (double)(n * T2[i] - T0[i] * T0[i]);
+        srp[i] = (int32_t)(T0[i] - 128 * n);
+    }
+
+    long a8[2][NK];
+    v4i a16[2][NK];
+    if constexpr (G::PRELOAD_A) {
+#pragma unroll
+        for (int tt = 0; tt < 2; tt++)
+#pragma unroll
+            for (int kk = 0; kk < NK; kk++) load_a<S>(a.rng8, g, tt, kk, lane, &a8[tt][kk], &a16[tt][kk]);
+    }
+    __syncthreads();
+
+    const uint32_t ntiles = (a.P + 31) / 32;
+    const uint32_t t_begin = chunk * a.tiles_per_chunk;
+    uint32_t t_end = t_begin + a.tiles_per_chunk;
+    if (t_end > ntiles) t_end = ntiles;
+    unsigned long long survivors = 0;
+    int32_t *t1buf = &s_t1[wave][lane * T1_STRIDE];
+    uint32_t tcount = 0;
+
+    for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4, tcount++) {
+        // Refresh the thresholds from the other blocks of this range group.
+        if ((tcount & 7) == 7 && lane < 8) {
+            uint32_t r = g * 8 + lane;
+            if (r < a.nr) atomicMin(&s_thr[lane], a.gthr[r]);
+        }
+        const uint32_t p = tile * 32 + jj;
+        const bool pv = p < a.P;
+        const uint32_t pc = pv ? p : a.P - 1;
+        const uint32_t pi = pc / a.npc, pj = pc - pi * a.npc;
+        const uint32_t dr = (pi * a.shift) >> 1, dc = (pj * a.shift) >> 1;
+
+        v16i acc_lo[2], acc_hi[2];
+#pragma unroll
+        for (int tt = 0; tt < 2; tt++) {
+            acc_lo[tt] = (v16i){};
+            acc_hi[tt] = (v16i){};
+        }
+        if constexpr (G::PRELOAD_A) {
+#pragma unroll
+            for (int kk = 0; kk < NK; kk++) {
+                long bl8 = 0, bh8 = 0;
+                v4i bl16 = {}, bh16 = {};
+                load_b<S>(a.lo, a.copy_stride, a.pitch, dr, dc, kk, h, &bl8, &bl16);
+                load_b<S>(a.hi, a.copy_stride, a.pitch, dr, dc, kk, h, &bh8, &bh16);
+#pragma unroll
+                for (int tt = 0; tt < 2; tt++) {
+                    if constexpr (G::SMALLK) {
+                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], bl8, acc_lo[tt], 0, 0, 0);
+                        acc_hi[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], bh8, acc_hi[tt], 0, 0, 0);
+                    } else {
+                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], bl16, acc_lo[tt], 0, 0, 0);
+                        acc_hi[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], bh16, acc_hi[tt], 0, 0, 0);
+                    }
+                }
+            }
+        } else {
+#pragma unroll 2
+            for (int kk = 0; kk < NK; kk++) {
+                long dummy8 = 0;
+                v4i bl16 = {}, bh16 = {}, a0 = {}, a1 = {};
+                load_b<S>(a.lo, a.copy_stride, a.pitch, dr, dc, kk, h, &dummy8, &bl16);
+                load_b<S>(a.hi, a.copy_stride, a.pitch, dr, dc, kk, h, &dummy8, &bh16);
+                load_a<S>(a.rng8, g, 0, kk, lane, &dummy8, &a0);
+                load_a<S>(a.rng8, g, 1, kk, lane, &dummy8, &a1);
+                acc_lo[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bl16, acc_lo[0], 0, 0, 0);
+                acc_hi[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bh16, acc_hi[0], 0, 0, 0);
+                acc_lo[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bl16, acc_lo[1], 0, 0, 0);
+                acc_hi[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bh16, acc_hi[1], 0, 0, 0);
+            }
+        }
+
+        // Per (range, position) pruning limit on the raw correlation.
+        const int32_t S1p = a.pos[2 * pc];
+        const float Dif = a.posdi[pc];
+        int32_t lim[4], off[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            off[i] = 128 * (srp[i] + S1p);
+            float thr = __uint_as_float(s_thr[4 * h + i]);
+            double nthr = (double)n * (double)thr;
+            if (!pv || !rvalid[i]) {
+                lim[i] = 0x7fffffff;
+            } else if (nthr < A[i]) {
+                float amt = (float)(A[i] - nthr);
+                float nt = sqrtf(amt * Dif) * (1.0f - 1e-5f);
+                double X = ((double)S1p * (double)T0[i] + (double)nt) * inv_n;
+                lim[i] = (int32_t)floor(X) - 1 - off[i];
+            } else {
+                lim[i] = (int32_t)0x80000000;
+            }
+        }
+
+        uint32_t surv = 0;
+#pragma unroll
+        for (int tt = 0; tt < 2; tt++)
+#pragma unroll
+            for (int t = 0; t < 16; t++) {
+                int32_t v = (acc_hi[tt][t] << 8) + acc_lo[tt][t];
+                surv |= (v > lim[t & 3] ? 1u : 0u) << (tt * 16 + t);
+            }
+        if (surv) {
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    v4i o;
+                    o.x = (acc_hi[tt][4 * q + 0] << 8) + acc_lo[tt][4 * q + 0];
+                    o.y = (acc_hi[tt][4 * q + 1] << 8) + acc_lo[tt][4 * q + 1];
+                    o.z = (acc_hi[tt][4 * q + 2] << 8) + acc_lo[tt][4 * q + 2];
+                    o.w = (acc_hi[tt][4 * q + 3] << 8) + acc_lo[tt][4 * q + 3];
+                    *(v4i *)&t1buf[tt * 16 + 4 * q] = o;
+                }
+        }
+        const int32_t S2p = a.pos[2 * pc + 1];
+        while (surv) {
+            int b = __builtin_ctz(surv);
+            surv &= surv - 1;
+            int t = b & 15, tt = b >> 4;
+            int i = t & 3;
+            uint32_t iso = 4 * tt + (t >> 2);
+            int32_t t1 = t1buf[b] + sel32(off, i);
+            uint32_t qa, qb;
+            double rms, sum;
+            int64_t t0i = sel64(T0, i), t2i = sel64(T2, i);
+            fit_f64(n, S1p, S2p, t0i, t1, t2i, a.max_alfa, a.bits_alfa, a.bits_beta, &qa, &qb,
+                    &rms, &sum);
+            survivors++;
+            uint32_t key = p * 8 + iso;
+            BestRec &br = s_best[wave][lane][i];
+            if (better(rms, key, br.rms, br.key)) {
+                br.rms = rms;
+                br.key = key;
+                double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
+                uint32_t bits = __float_as_uint((float)(tau * (1.0 + 1e-7)));
+                atomicMin(&s_thr[4 * h + i], bits);
+                atomicMin(&a.gthr[g * 8 + 4 * h + i], bits);
+            }
+        }
+    }
+    if (a.counters) atomicAdd(&a.counters[0], survivors);
+    __syncthreads();
+
+    if (threadIdx.x < 8) {
+        uint32_t rl = threadIdx.x, hh = rl >> 2, i = rl & 3;
+        uint32_t r = g * 8 + rl;
+        if (r < a.nr) {
+            double brms = __longlong_as_double(0x7ff0000000000000ll);
+            uint32_t bkey = 0xffffffffu;
+            for (int wv = 0; wv < 4; wv++)
+                for (int l = 0; l < 32; l++) {
+                    const BestRec &e = s_best[wv][hh * 32 + l][i];
+                    if (better(e.rms, e.key, brms, bkey)) {
+                        brms = e.rms;
+                        bkey = e.key;
+                    }
+                }
+            Partial &out = a.partial[(size_t)r * a.nchunks + chunk];
+            out.rms = brms;
+            out.key = bkey;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP32 baseline
+
+// Plain FP32 search: a wave covers 4 ranges x 8 isometries (32 rows) and
+// 64 positions (one per lane).  The range operand is wave-uniform (scalar
+// loads), the window sample is per lane; 32 FMAs per k.  Every candidate
+// then takes the binary32 fit of encode.rs:295-337 and competes by
+// (rms_f32, key).  No certification: results may differ from the binary64
+// reference wherever binary32 rounding moves a quantisation decision or an
+// argmin near-tie (and t1 is inexact above 2^24).
+template <int S>
+__global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
+{
+    constexpr int K = S * S;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t g = blockIdx.y, chunk = blockIdx.x;
+    __shared__ BestRec s_best[4][64][4];
+
+    float T0[4], T2[4];
+    bool rvalid[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        uint32_t r = g * 4 + i;
+        rvalid[i] = r < a.nr;
+        uint32_t rr = rvalid[i] ? r : 0;
+        T0[i] = (float)a.rstat[2 * rr];
+        T2[i] = (float)a.rstat[2 * rr + 1];
+    }
+    float brms[4];
+    uint32_t bkey[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        brms[i] = __int_as_float(0x7f800000);
+        bkey[i] = 0xffffffffu;
+    }
+    const float s0 = (float)K;
+    const uint32_t ntiles = (a.P + 63) / 64;
+    const uint32_t t_begin = chunk * a.tiles_per_chunk;
+    uint32_t t_end = t_begin + a.tiles_per_chunk;
+    if (t_end > ntiles) t_end = ntiles;
+    const float *A = a.rngf + (size_t)g * K * 32;
+
+    for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4) {
+        const uint32_t p = tile * 64 + lane;
+        const bool pv = p < a.P;
+        const uint32_t pc = pv ? p : a.P - 1;
+        const uint32_t pi = pc / a.npc, pj = pc - pi * a.npc;
+        const uint32_t dr = (pi * a.shift) >> 1, dc = (pj * a.shift) >> 1;
+        const float *D = a.dplanef + (size_t)dr * a.wc + dc;
+        float acc[32];
+#pragma unroll
+        for (int r = 0; r < 32; r++) acc[r] = 0.0f;
+        for (int u = 0; u < S; u++) {
+#pragma unroll 8
+            for (int v = 0; v < S; v++) {
+                float d = D[(size_t)u * a.wc + v];
+                const float *ar = A + (size_t)(u * S + v) * 32;
+#pragma unroll
+                for (int r = 0; r < 32; r++) acc[r] = __builtin_fmaf(ar[r], d, acc[r]);
+            }
+        }
+        if (!pv) continue;
+        const float S1 = (float)a.pos[2 * pc], S2 = (float)a.pos[2 * pc + 1];
+#pragma unroll
+        for (int r = 0; r < 32; r++) {
+            const int i = r & 3;
+            const uint32_t iso = r >> 2;
+            uint32_t qa, qb;
+            float rms;
+            fit_f32(s0, S1, S2, T0[i], acc[r], T2[i], (float)a.max_alfa, a.bits_alfa,
+                    a.bits_beta, &qa, &qb, &rms);
+            uint32_t key = p * 8 + iso;
+            if (rms < brms[i] || (rms == brms[i] && key < bkey[i])) {
+                brms[i] = rms;
+                bkey[i] = key;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        s_best[wave][lane][i].rms = (double)brms[i];
+        s_best[wave][lane][i].key = bkey[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        uint32_t i = threadIdx.x, r = g * 4 + i;
+        if (r < a.nr) {
+            double b = __longlong_as_double(0x7ff0000000000000ll);
+            uint32_t k = 0xffffffffu;
+            for (int wv = 0; wv < 4; wv++)
+                for (int l = 0; l < 64; l++) {
+                    const BestRec &e = s_best[wv][l][i];
+                    if (better(e.rms, e.key, b, k)) {
+                        b = e.rms;
+                        k = e.key;
+                    }
+                }
+            Partial &out = a.partial[(size_t)r * a.nchunks + chunk];
+            out.rms = b;
+            out.key = k;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- merge
+
+// One thread per range: min over chunks by (rms, key); the winner's exact
+// moments and quantised coefficients are recomputed from the pool (a direct
+// window correlation and the binary64 fit) so the table holds the
+// reference's Candidate (encode.rs:234-244) field for field.  For the FP32
+// baseline the recomputed rms replaces the binary32 one.
+__global__ void k_merge(MergeArgs a)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.nr) return;
+    mars_hip_candidate c = {};
+    uint8_t found = 0;
+    if (a.P > 0) {
+        double b = __longlong_as_double(0x7ff0000000000000ll);
+        uint32_t k = 0xffffffffu;
+        for (uint32_t ch = 0; ch < a.nchunks; ch++) {
+            const Partial &e = a.partial[(size_t)r * a.nchunks + ch];
+            if (better(e.rms, e.key, b, k)) {
+                b = e.rms;
+                k = e.key;
+            }
+        }
+        if (k != 0xffffffffu) {
+            found = 1;
+            uint32_t p = k >> 3, iso = k & 7;
+            uint32_t pi = p / a.npc, pj = p - pi * a.npc;
+            c.dom_row = pi * a.shift;
+            c.dom_col = pj * a.shift;
+            c.isometry = iso;
+            uint32_t dr = c.dom_row >> 1, dc = c.dom_col >> 1;
+            uint32_t S = a.size, row = a.rows[r], col = a.cols[r];
+            int64_t t1 = 0;
+            for (uint32_t u = 0; u < S; u++)
+                for (uint32_t v = 0; v < S; v++) {
+                    uint32_t i, j;
+                    iso_map(iso, u, v, S, &i, &j);
+                    t1 += (int64_t)a.px[(size_t)(row + i) * a.w + col + j] *
+                          (int64_t)a.d[(size_t)(dr + u) * a.wc + dc + v];
+                }
+            c.s0 = (int64_t)S * S;
+            c.s1_x4 = a.pos[2 * p];
+            c.s2_x16 = a.pos[2 * p + 1];
+            c.t0 = a.rstat[2 * r];
+            c.t1_x4 = t1;
+            c.t2 = a.rstat[2 * r + 1];
+            uint32_t qa, qb;
+            double rms, sum;
+            fit_f64(c.s0, c.s1_x4, c.s2_x16, c.t0, c.t1_x4, c.t2, a.max_alfa, a.bits_alfa,
+                    a.bits_beta, &qa, &qb, &rms, &sum);
+            c.qalfa = qa;
+            c.qbeta = qb;
+            c.rms = rms;
+        }
+    }
+    a.out[r] = c;
+    a.found[r] = found;
+}
+
+__global__ void k_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
+                               uint32_t bits_beta, uint32_t *qa, uint32_t *qb, double *rms)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t *x = m + 6 * (size_t)i;
+    double sum;
+    fit_f64(x[0], x[1], x[2], x[3], x[4], x[5], max_alfa, bits_alfa, bits_beta, &qa[i], &qb[i],
+            &rms[i], &sum);
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <int S> static void launch_mfma(const SearchArgs &a, hipStream_t st)
+{
+    dim3 grid(a.nchunks, a.ngroups);
+    hipLaunchKernelGGL(k_search_mfma<S>, grid, dim3(256), 0, st, a);
+}
+template <int S> static void launch_fp32(const SearchArgs &a, hipStream_t st)
+{
+    dim3 grid(a.nchunks, a.ngroups);
+    hipLaunchKernelGGL(k_search_fp32<S>, grid, dim3(256), 0, st, a);
+}
+
+int launch_search(int kernel, uint32_t size, const SearchArgs &a, hipStream_t st)
+{
+    if (kernel == 0) {
+        switch (size) {
+        case 2: launch_mfma<2>(a, st); return 0;
+        case 4: launch_mfma<4>(a, st); return 0;
+        case 8: launch_mfma<8>(a, st); return 0;
+        case 16: launch_mfma<16>(a, st); return 0;
+        case 32: launch_mfma<32>(a, st); return 0;
+        }
+    } else {
+        switch (size) {
+        case 2: launch_fp32<2>(a, st); return 0;
+        case 4: launch_fp32<4>(a, st); return 0;
+        case 8: launch_fp32<8>(a, st); return 0;
+        case 16: launch_fp32<16>(a, st); return 0;
+        case 32: launch_fp32<32>(a, st); return 0;
+        }
+    }
+    return -1;
+}
+
+void launch_contract(const uint8_t *px, uint32_t w, uint32_t wc, uint32_t hc, uint32_t pitch,
+                     size_t copy_stride, int32_t *d, float *df, int8_t *lo, int8_t *hi,
+                     unsigned long long *counters, uint32_t n_counters, hipStream_t st)
+{
+    uint32_t n = wc * hc;
+    if (n < n_counters) n = n_counters;
+    hipLaunchKernelGGL(k_contract, dim3((n + 255) / 256), dim3(256), 0, st, px, w, wc, hc, pitch,
+                       copy_stride, d, df, lo, hi, counters, n_counters);
+}
+
+void launch_possums(const int32_t *d, uint32_t wc, uint32_t size, uint32_t shift, uint32_t npc,
+                    uint32_t P, int32_t *pos, float *posdi, hipStream_t st)
+{
+    if (P == 0) return;
+    hipLaunchKernelGGL(k_possums, dim3((P + 255) / 256), dim3(256), 0, st, d, wc, size, shift,
+                       npc, P, pos, posdi);
+}
+
+void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const uint32_t *cols,
+                   uint32_t nr, uint32_t nr_pad, uint32_t size, uint32_t kmem, int8_t *rng8,
+                   float *rngf, int32_t *rstat, uint32_t *gthr, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ranges, dim3(nr_pad), dim3(256), 0, st, px, w, rows, cols, nr, size,
+                       kmem, rng8, rngf, rstat, gthr);
+}
+
+void launch_merge(const MergeArgs &a, hipStream_t st)
+{
+    if (a.nr == 0) return;
+    hipLaunchKernelGGL(k_merge, dim3((a.nr + 127) / 128), dim3(128), 0, st, a);
+}
+
+void launch_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
+                         uint32_t bits_beta, uint32_t *qa, uint32_t *qb, double *rms,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fit_selftest, dim3((n + 255) / 256), dim3(256), 0, st, m, n, max_alfa,
+                       bits_alfa, bits_beta, qa, qb, rms);
+}
+
+} // namespace mars

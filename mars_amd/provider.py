@@ -1,0 +1,210 @@
+"""Host-side mirror of the reference search-provider interface over the C ABI.
+
+Reference: crates/mars-codec/src/encode.rs
+  * EncodeParams          27-60
+  * RawMoments, Candidate 224-244
+  * SearchRequest         530-540
+  * SearchOutcome         542-546
+  * SearchProvider        557-579 (search / search_candidates)
+  * encode_image_with_search's factory, called once per plane, 1054-1076
+
+``HipSearchProvider(image, params)`` is that factory's product for the HIP
+backend: construction runs ``mars_hip_prepare`` (pool + whole-plane search
+table on the GPU), ``search`` is the thread-safe lookup.  Like the reference
+interface, ``search`` never raises for a legal request; an illegal one (a
+range the walk could not request) raises ``ValueError`` the way the
+reference's misuse panics (crates/mars-search/src/lib.rs:520-527).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+
+
+@dataclass(frozen=True)
+class EncodeParams:
+    """encode.rs:27-60 (only the fields the search and walks read)."""
+    min_size: int = 4
+    max_size: int = 16
+    shift: int = 4
+    bits_alfa: int = 4
+    bits_beta: int = 7
+    max_alfa: float = 1.0
+    t_rms: float = 8.0
+    zero_threshold: int = 0
+    lambda_: Optional[float] = None
+
+
+@dataclass(frozen=True)
+class RawMoments:
+    """encode.rs:224-232; s1/s2/t1 are 4x/16x/4x their value (exact integers)."""
+    s0: int
+    s1_x4: int
+    s2_x16: int
+    t0: int
+    t1_x4: int
+    t2: int
+
+
+@dataclass(frozen=True)
+class Candidate:
+    """encode.rs:234-244."""
+    dom_row: int
+    dom_col: int
+    isometry: int
+    qalfa: int
+    qbeta: int
+    rms: float
+    moments: RawMoments
+
+
+@dataclass(frozen=True)
+class SearchRequest:
+    """encode.rs:530-540 (the plane is the provider's own copy)."""
+    row: int
+    col: int
+    size: int
+    shift: int
+
+
+@dataclass
+class SearchOutcome:
+    """encode.rs:542-546."""
+    candidate: Optional[Candidate]
+    evals: int
+
+
+@dataclass
+class RankedCandidates:
+    """encode.rs:548-555."""
+    candidates: list = field(default_factory=list)
+    evals: int = 0
+
+
+def _cand(c: N.CCandidate) -> Candidate:
+    return Candidate(c.dom_row, c.dom_col, c.isometry, c.qalfa, c.qbeta, c.rms,
+                     RawMoments(c.s0, c.s1_x4, c.s2_x16, c.t0, c.t1_x4, c.t2))
+
+
+class HipContext:
+    """Owns one ``mars_hip_ctx`` (one device stream, one prepared plane)."""
+
+    def __init__(self, device: int = 0, kernel: int = N.KERNEL_MFMA_I8):
+        self._L = N.lib()
+        self._ctx = C.c_void_p()
+        rc = self._L.mars_hip_open(device, C.byref(self._ctx))
+        if rc != N.OK:
+            raise RuntimeError(f"mars_hip_open({device}) failed with status {rc}")
+        self.set_kernel(kernel)
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != N.OK:
+            msg = self._L.mars_hip_last_error(self._ctx).decode(errors="replace")
+            raise RuntimeError(f"{what} failed with status {rc}: {msg}")
+
+    def set_kernel(self, kernel: int) -> None:
+        self._check(self._L.mars_hip_set_kernel(self._ctx, kernel), "mars_hip_set_kernel")
+
+    def prepare(self, image: np.ndarray, params: EncodeParams, shifts: Sequence[int]) -> None:
+        img = np.ascontiguousarray(image, dtype=np.uint8)
+        if img.ndim != 2:
+            raise ValueError("image must be a 2-D u8 plane")
+        h, w = img.shape
+        p = N.Params(params.min_size, params.max_size, params.shift, params.bits_alfa,
+                     params.bits_beta, 0, params.max_alfa)
+        sh = (C.c_uint32 * len(shifts))(*shifts)
+        self._check(self._L.mars_hip_prepare(
+            self._ctx, img.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, C.byref(p), sh,
+            len(shifts)), "mars_hip_prepare")
+
+    def run(self, iters: int = 1, use_graph: bool = True) -> N.Timing:
+        t = N.Timing()
+        self._check(self._L.mars_hip_run(self._ctx, iters, int(use_graph), C.byref(t)),
+                    "mars_hip_run")
+        return t
+
+    def search_raw(self, row: int, col: int, size: int, shift: int):
+        out = N.CCandidate()
+        n = C.c_uint32()
+        ev = C.c_uint64()
+        rc = self._L.mars_hip_search(self._ctx, row, col, size, shift, 1, C.byref(out),
+                                     C.byref(n), C.byref(ev))
+        return rc, (out if n.value else None), ev.value
+
+    def tables(self):
+        """List of (size, shift, rows, cols, candidates[list or None per range], n_positions)."""
+        n = C.c_uint32()
+        self._check(self._L.mars_hip_table_count(self._ctx, C.byref(n)), "table_count")
+        res = []
+        for i in range(n.value):
+            s, sh, nr, P = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+            self._check(self._L.mars_hip_table_info(self._ctx, i, C.byref(s), C.byref(sh),
+                                                    C.byref(nr), C.byref(P)), "table_info")
+            rows = (C.c_uint32 * max(nr.value, 1))()
+            cols = (C.c_uint32 * max(nr.value, 1))()
+            cands = (N.CCandidate * max(nr.value, 1))()
+            found = (C.c_uint8 * max(nr.value, 1))()
+            self._check(self._L.mars_hip_table_read(self._ctx, i, rows, cols, cands, found),
+                        "table_read")
+            res.append((s.value, sh.value, list(rows[:nr.value]), list(cols[:nr.value]),
+                        [cands[j] if found[j] else None for j in range(nr.value)], P.value))
+        return res
+
+    def fit_f64(self, moments: np.ndarray, max_alfa: float, bits_alfa: int, bits_beta: int):
+        m = np.ascontiguousarray(moments, dtype=np.int64).reshape(-1, 6)
+        n = m.shape[0]
+        qa = np.zeros(n, np.uint32)
+        qb = np.zeros(n, np.uint32)
+        rms = np.zeros(n, np.float64)
+        self._check(self._L.mars_hip_fit_f64(
+            self._ctx, m.ctypes.data_as(C.POINTER(C.c_int64)), n, max_alfa, bits_alfa, bits_beta,
+            qa.ctypes.data_as(C.POINTER(C.c_uint32)), qb.ctypes.data_as(C.POINTER(C.c_uint32)),
+            rms.ctypes.data_as(C.POINTER(C.c_double))), "mars_hip_fit_f64")
+        return qa, qb, rms
+
+    def close(self) -> None:
+        if self._ctx:
+            self._L.mars_hip_close(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HipSearchProvider:
+    """SearchProvider (encode.rs:557-579) backed by the MI355X search table.
+
+    Built once per plane, as encode_image_with_search's factory is
+    (encode.rs:1054-1076).  ``shifts`` lists every domain stride the walk
+    may request: ``params.shift``, plus ``2 * params.shift`` when adaptive
+    density is on (encode.rs:432-437).
+    """
+
+    def __init__(self, image: np.ndarray, params: EncodeParams,
+                 shifts: Optional[Sequence[int]] = None, device: int = 0,
+                 kernel: int = N.KERNEL_MFMA_I8):
+        self.params = params
+        self.ctx = HipContext(device, kernel)
+        self.ctx.prepare(image, params, list(shifts) if shifts else [params.shift])
+
+    def search(self, request: SearchRequest) -> SearchOutcome:
+        rc, c, evals = self.ctx.search_raw(request.row, request.col, request.size, request.shift)
+        if rc != N.OK:
+            msg = self.ctx._L.mars_hip_last_error(self.ctx._ctx).decode(errors="replace")
+            raise ValueError(msg)
+        return SearchOutcome(_cand(c) if c is not None else None, evals)
+
+    def search_candidates(self, request: SearchRequest, max_: int) -> RankedCandidates:
+        """The trait's default (encode.rs:572-578): the winner only."""
+        out = self.search(request)
+        cands = [out.candidate] if out.candidate is not None else []
+        return RankedCandidates(cands[:max(max_, 1)], out.evals)
