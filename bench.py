@@ -61,6 +61,10 @@ def _gpu_leg(steps: int, warmup: int):
                          ms_search=float(np.mean([b.ms_search for b in br])),
                          ms_by_size=by_size, evals_by_size=evals_by_size,
                          survivors=int(br[0].survivors),
+                         survivors_by_size={str(2 << i): int(br[0].survivors_by_size[i])
+                                            for i in range(6) if br[0].ms_by_size[i] > 0},
+                         exact_fits_by_size={str(2 << i): int(br[0].exact_fits_by_size[i])
+                                             for i in range(6) if br[0].ms_by_size[i] > 0},
                          winners=[[(c.dom_row, c.dom_col, c.isometry, c.qalfa, c.qbeta)
                                    if c is not None else None for c in tt[4]] for tt in tabs])
         ctx.close()
@@ -147,6 +151,8 @@ def main() -> int:
                          "search_by_size": mf["ms_by_size"],
                          "fp32_search_by_size": fp["ms_by_size"]},
         "survivor_fraction": mf["survivors"] / evals,
+        "survivors_by_size": mf["survivors_by_size"],
+        "exact_fits_by_size": mf["exact_fits_by_size"],
         "roofline": {"bound": "mfma", "kernel": f"k_search_mfma<{s}>", "achieved": achieved / 1e12,
                      "peak": PEAK_INT8_OPS / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_INT8_OPS,
                      "traffic": None, "op": "int8 ops (2*s^2 per eval)",

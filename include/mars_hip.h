@@ -70,7 +70,9 @@ typedef struct {
     double ms_search;   /* search kernels, all sizes */
     double ms_by_size[6]; /* search kernel per size 2,4,8,16,32,64 (index log2(size)-1) */
     uint64_t evals;     /* range x position x isometry candidates per step */
-    uint64_t survivors; /* candidates that reached the binary64 fit (MFMA kernel) */
+    uint64_t survivors; /* candidates past the integer bound (MFMA kernel) */
+    uint64_t survivors_by_size[6];  /* the same per size 2..64 */
+    uint64_t exact_fits_by_size[6]; /* candidates that took the verbatim binary64 fit */
 } mars_hip_timing;
 
 /* Open a context on HIP device `device` (0-based).  Creates its own stream. */

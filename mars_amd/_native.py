@@ -39,7 +39,8 @@ class CCandidate(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("ms_step", C.c_double), ("ms_pool", C.c_double), ("ms_search", C.c_double),
                 ("ms_by_size", C.c_double * 6), ("evals", C.c_uint64),
-                ("survivors", C.c_uint64)]
+                ("survivors", C.c_uint64), ("survivors_by_size", C.c_uint64 * 6),
+                ("exact_fits_by_size", C.c_uint64 * 6)]
 
 
 _lib = None

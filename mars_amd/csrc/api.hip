@@ -29,7 +29,7 @@ using mars::SearchArgs;
 
 constexpr uint32_t kMaxSize = 32;
 constexpr uint32_t kTargetBlocks = 2048; // >> 256 CUs: fill the chip per launch
-constexpr uint32_t kCounters = 4;
+constexpr uint32_t kCounters = 16;
 
 struct Table {
     uint32_t size = 0, shift = 0;
@@ -44,6 +44,7 @@ struct Table {
     uint32_t *d_gthr = nullptr;
     int32_t *d_pos = nullptr;
     float *d_posdi = nullptr;
+    double *d_posd = nullptr;
     Partial *d_partial = nullptr;
     mars_hip_candidate *d_out = nullptr;
     uint8_t *d_found = nullptr;
@@ -68,6 +69,9 @@ struct mars_hip_ctx {
     float *d_df = nullptr;
     int8_t *d_lo = nullptr, *d_hi = nullptr;
     unsigned long long *d_counters = nullptr;
+    double *d_yscale = nullptr;
+    int stage2 = 0;
+    double alfa_c = 0, alfa_unit = 0;
     std::vector<Table> tables;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -138,6 +142,7 @@ void free_tables(mars_hip_ctx *ctx)
         (void)hipFree(t.d_gthr);
         (void)hipFree(t.d_pos);
         (void)hipFree(t.d_posdi);
+        (void)hipFree(t.d_posd);
         (void)hipFree(t.d_partial);
         (void)hipFree(t.d_out);
         (void)hipFree(t.d_found);
@@ -148,6 +153,8 @@ void free_tables(mars_hip_ctx *ctx)
     (void)hipFree(ctx->d_df);
     (void)hipFree(ctx->d_lo);
     (void)hipFree(ctx->d_hi);
+    (void)hipFree(ctx->d_yscale);
+    ctx->d_yscale = nullptr;
     ctx->d_px = nullptr;
     ctx->d_d = nullptr;
     ctx->d_df = nullptr;
@@ -185,6 +192,7 @@ SearchArgs search_args(const mars_hip_ctx *ctx, const Table &t, int kernel)
     a.gthr = t.d_gthr;
     a.pos = t.d_pos;
     a.posdi = t.d_posdi;
+    a.posd = t.d_posd;
     a.lo = ctx->d_lo;
     a.hi = ctx->d_hi;
     a.copy_stride = ctx->copy_stride;
@@ -200,6 +208,13 @@ SearchArgs search_args(const mars_hip_ctx *ctx, const Table &t, int kernel)
     a.bits_beta = ctx->params.bits_beta;
     a.partial = t.d_partial;
     a.counters = ctx->d_counters;
+    a.counter_slot = ceil_log2(t.size);
+    a.stage2 = ctx->stage2;
+    a.alfa_c = ctx->alfa_c;
+    a.alfa_unit = ctx->alfa_unit;
+    a.qa_max = (double)((1u << ctx->params.bits_alfa) - 1);
+    a.qb_max = (double)((1u << ctx->params.bits_beta) - 1);
+    a.yscale = ctx->d_yscale;
     if (kernel == MARS_HIP_KERNEL_MFMA_I8) {
         a.ngroups = t.nr_pad / 8;
         a.nchunks = t.nchunks_mfma;
@@ -249,9 +264,13 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
                           st);
     for (Table &t : ctx->tables) {
         mars::launch_possums(ctx->d_d, ctx->wc, t.size, t.shift, t.npc, t.P, t.d_pos, t.d_posdi,
-                             st);
+                             t.d_posd, st);
         mars::launch_ranges(ctx->d_px, ctx->w, t.d_rows, t.d_cols, t.nr, t.nr_pad, t.size,
                             t.kmem, t.d_rng8, t.d_rngf, t.d_rstat, t.d_gthr, st);
+        if (kernel == MARS_HIP_KERNEL_MFMA_I8)
+            mars::launch_seed(t.d_rng8, t.d_rstat, ctx->d_d, ctx->wc, t.size, t.kmem, t.shift,
+                              t.npc, t.P, t.d_pos, t.nr, ctx->params.max_alfa,
+                              ctx->params.bits_alfa, ctx->params.bits_beta, t.d_gthr, st);
     }
     if (ev) HIP_TRY(ctx, hipEventRecord(ev[1], st));
     for (size_t i = 0; i < ctx->tables.size(); i++) {
@@ -374,6 +393,25 @@ int mars_hip_prepare(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t 
     HIP_TRY(ctx, hipMemset(ctx->d_lo, 0, 16 * ctx->copy_stride));
     HIP_TRY(ctx, hipMemset(ctx->d_hi, 0, 16 * ctx->copy_stride));
 
+    // Stage-2 constants: exact only when max_alfa is a power of two (then the
+    // alfa scaling of encode.rs:264 is exact in binary64).
+    {
+        int e = 0;
+        double mant = std::frexp(p->max_alfa, &e);
+        ctx->stage2 = (mant == 0.5 && p->bits_alfa <= 8 && p->bits_beta <= 12) ? 1 : 0;
+        ctx->alfa_c = std::ldexp(1.0, (int)p->bits_alfa + 2) / p->max_alfa;
+        ctx->alfa_unit = p->max_alfa / std::ldexp(1.0, (int)p->bits_alfa);
+        uint32_t nq = 1u << p->bits_alfa;
+        std::vector<double> ys(nq);
+        double qbm = (double)((1u << p->bits_beta) - 1);
+        for (uint32_t q = 0; q < nq; q++) {
+            double a2 = (double)q / (double)(1u << p->bits_alfa) * p->max_alfa;
+            ys[q] = qbm / ((1.0 + std::fabs(a2)) * 255.0);
+        }
+        HIP_TRY(ctx, hipMalloc((void **)&ctx->d_yscale, sizeof(double) * nq));
+        HIP_TRY(ctx, hipMemcpy(ctx->d_yscale, ys.data(), sizeof(double) * nq, hipMemcpyHostToDevice));
+    }
+
     std::vector<uint32_t> sh(shifts, shifts + n_shifts);
     std::sort(sh.begin(), sh.end());
     sh.erase(std::unique(sh.begin(), sh.end()), sh.end());
@@ -412,6 +450,7 @@ int mars_hip_prepare(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t 
             HIP_TRY(ctx, hipMalloc((void **)&t.d_gthr, sizeof(uint32_t) * t.nr_pad));
             HIP_TRY(ctx, hipMalloc((void **)&t.d_pos, sizeof(int32_t) * 2 * std::max<uint32_t>(t.P, 1)));
             HIP_TRY(ctx, hipMalloc((void **)&t.d_posdi, sizeof(float) * std::max<uint32_t>(t.P, 1)));
+            HIP_TRY(ctx, hipMalloc((void **)&t.d_posd, 2 * sizeof(double) * std::max<uint32_t>(t.P, 1)));
             HIP_TRY(ctx, hipMalloc((void **)&t.d_partial, sizeof(Partial) * (size_t)t.nr_pad * nch));
             HIP_TRY(ctx, hipMalloc((void **)&t.d_out, sizeof(mars_hip_candidate) * std::max<uint32_t>(t.nr, 1)));
             HIP_TRY(ctx, hipMalloc((void **)&t.d_found, std::max<uint32_t>(t.nr, 1)));
@@ -448,9 +487,13 @@ int mars_hip_run(mars_hip_ctx *ctx, int iters, int use_graph, mars_hip_timing *t
         uint32_t lg = ceil_log2(ctx->tables[i].size);
         if (lg >= 1 && lg <= 6) tm.ms_by_size[lg - 1] += ms;
     }
-    unsigned long long surv = 0;
-    HIP_TRY(ctx, hipMemcpy(&surv, ctx->d_counters, sizeof(surv), hipMemcpyDeviceToHost));
-    tm.survivors = surv;
+    unsigned long long cnt[kCounters] = {};
+    HIP_TRY(ctx, hipMemcpy(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 6; i++) {
+        tm.survivors_by_size[i] = cnt[i + 1];
+        tm.exact_fits_by_size[i] = cnt[8 + i + 1];
+        tm.survivors += cnt[i + 1];
+    }
     for (const Table &t : ctx->tables) tm.evals += (uint64_t)t.nr * table_evals(t);
 
     // Timed steps.

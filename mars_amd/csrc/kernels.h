@@ -23,7 +23,8 @@ struct SearchArgs {
     const int32_t *rstat;    // [nr_pad][2] t0, t2
     uint32_t *gthr;          // [nr_pad] pruning threshold (float bits, SSE units)
     const int32_t *pos;      // [P][2] S1, S2
-    const float *posdi;      // [P] n*S2 - S1^2
+    const float *posdi;      // [P] Di = n*S2 - S1^2 (f32, stage-1 limit)
+    const double *posd;      // [P][2] Di, 1/Di (binary64, stage 2)
     const int8_t *lo, *hi;   // split planes, 16 shifted copies each
     size_t copy_stride;      // bytes between copies
     uint32_t pitch;          // bytes per contracted row in a copy
@@ -35,6 +36,13 @@ struct SearchArgs {
     uint32_t bits_alfa, bits_beta;
     Partial *partial;        // [nr_pad][nchunks]
     unsigned long long *counters;
+    uint32_t counter_slot;   // log2(size): counters[slot] stage-1 survivors, [8+slot] stage 3
+    // Stage-2 constants (valid when stage2 != 0: max_alfa is a power of two).
+    int stage2;
+    double alfa_c;           // 2^(bits_alfa+2) / max_alfa: x = alfa_c * Ni / Di
+    double alfa_unit;        // max_alfa / 2^bits_alfa: alfa2 = qalfa * alfa_unit
+    double qa_max, qb_max;   // 2^bits_alfa - 1, 2^bits_beta - 1
+    const double *yscale;    // [2^bits_alfa] (2^bits_beta-1) / ((1 + alfa2) * 255)
 };
 
 struct MergeArgs {
@@ -57,7 +65,11 @@ void launch_contract(const uint8_t *px, uint32_t w, uint32_t wc, uint32_t hc, ui
                      size_t copy_stride, int32_t *d, float *df, int8_t *lo, int8_t *hi,
                      unsigned long long *counters, uint32_t n_counters, hipStream_t st);
 void launch_possums(const int32_t *d, uint32_t wc, uint32_t size, uint32_t shift, uint32_t npc,
-                    uint32_t P, int32_t *pos, float *posdi, hipStream_t st);
+                    uint32_t P, int32_t *pos, float *posdi, double *posd, hipStream_t st);
+void launch_seed(const int8_t *rng8, const int32_t *rstat, const int32_t *d, uint32_t wc,
+                 uint32_t size, uint32_t kmem, uint32_t shift, uint32_t npc, uint32_t P,
+                 const int32_t *pos, uint32_t nr, double max_alfa, uint32_t bits_alfa,
+                 uint32_t bits_beta, uint32_t *gthr, hipStream_t st);
 void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const uint32_t *cols,
                    uint32_t nr, uint32_t nr_pad, uint32_t size, uint32_t kmem, int8_t *rng8,
                    float *rngf, int32_t *rstat, uint32_t *gthr, hipStream_t st);

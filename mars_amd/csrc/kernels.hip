@@ -141,6 +141,10 @@ __device__ __forceinline__ int64_t sel64(const int64_t (&v)[4], int i)
 {
     return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
 }
+__device__ __forceinline__ double sel_d(const double (&v)[4], int i)
+{
+    return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
+}
 
 // isometry.rs:7-19
 __device__ __forceinline__ void iso_map(uint32_t k, uint32_t u, uint32_t v, uint32_t size,
@@ -192,7 +196,7 @@ __global__ void k_contract(const uint8_t *__restrict__ px, uint32_t w, uint32_t 
 // One thread per domain position of a table: window sums over D.
 __global__ void k_possums(const int32_t *__restrict__ d, uint32_t wc, uint32_t size,
                           uint32_t shift, uint32_t npc, uint32_t P, int32_t *__restrict__ pos,
-                          float *__restrict__ posdi)
+                          float *__restrict__ posdi, double *__restrict__ posd)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
@@ -210,7 +214,44 @@ __global__ void k_possums(const int32_t *__restrict__ d, uint32_t wc, uint32_t s
     pos[2 * p] = (int32_t)s1;
     pos[2 * p + 1] = (int32_t)s2;
     int64_t n = (int64_t)size * size;
-    posdi[p] = (float)(n * s2 - s1 * s1);
+    int64_t di = n * s2 - s1 * s1;
+    posdi[p] = (float)di;
+    posd[2 * p] = (double)di;
+    posd[2 * p + 1] = di > 0 ? 1.0 / (double)di : 0.0;
+}
+
+// Threshold seed: one block per range evaluates 256 spread candidates (32
+// positions x 8 isometries) by direct correlation and the binary64 fit, and
+// stores the smallest inflated SSE as the range's starting threshold.
+__global__ void k_seed(const int8_t *__restrict__ rng8, const int32_t *__restrict__ rstat,
+                       const int32_t *__restrict__ d, uint32_t wc, uint32_t size, uint32_t kmem,
+                       uint32_t shift, uint32_t npc, uint32_t P, const int32_t *__restrict__ pos,
+                       uint32_t nr, double max_alfa, uint32_t bits_alfa, uint32_t bits_beta,
+                       uint32_t *__restrict__ gthr)
+{
+    __shared__ uint32_t s_min;
+    const uint32_t r = blockIdx.x;
+    if (threadIdx.x == 0) s_min = 0x7f800000u;
+    __syncthreads();
+    const uint32_t j = threadIdx.x >> 3, iso = threadIdx.x & 7;
+    const uint32_t p = P >= 32 ? (uint32_t)(((uint64_t)j * (P - 1)) / 31) : j % P;
+    const uint32_t pi = p / npc, pj = p - pi * npc;
+    const uint32_t dr = (pi * shift) >> 1, dc = (pj * shift) >> 1;
+    const int8_t *rk = rng8 + ((size_t)r * 8 + iso) * kmem;
+    int64_t t1 = 0;
+    for (uint32_t u = 0; u < size; u++) {
+        const int32_t *row = d + (size_t)(dr + u) * wc + dc;
+        for (uint32_t v = 0; v < size; v++) t1 += (int64_t)((int)rk[u * size + v] + 128) * row[v];
+    }
+    const int64_t n = (int64_t)size * size;
+    uint32_t qa, qb;
+    double rms, sum;
+    fit_f64(n, pos[2 * p], pos[2 * p + 1], rstat[2 * r], t1, rstat[2 * r + 1], max_alfa,
+            bits_alfa, bits_beta, &qa, &qb, &rms, &sum);
+    double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
+    atomicMin(&s_min, __float_as_uint((float)(tau * (1.0 + 1e-7))));
+    __syncthreads();
+    if (threadIdx.x == 0 && r < nr) gthr[r] = s_min;
 }
 
 // One block per (padded) range.  rng8 layout [range][iso][kmem] (i8, r-128,
@@ -382,8 +423,19 @@ constexpr int T1_STRIDE = 36; // ints per lane in the survivor staging area
 // That is one integer compare per candidate; the limit is computed once per
 // (range, position) with a 1e-5 relative safety margin on the square root.
 // Candidates with Ni < 0 are covered too (their qalfa is 0, SSE >= A/n).
-// Survivors get the exact binary64 fit of encode.rs:247-289 and compete by
-// (rms, key); winners tighten tau through LDS and global atomics.
+// Stage 2 (per stage-1 survivor, all in binary64 with exact operands): the
+// candidate's quantised codes and SSE without any IEEE divide or square root.
+// qalfa is decided exactly from integers (2c*Ni >= (2k-1)*Di, c the alfa
+// scale); beta, alfa2 and the slope part of the SSE
+//   n*SSE(alfa2, b*) = A - alfa2*Ni/2 + alfa2^2*Di/16
+// are exact dyadic numbers; qbeta is the rounding of beta*(2^bb-1)/((1+a)255)
+// unless that product lies within 1e-9 of a rounding boundary; then
+//   SSE = n*SSE(alfa2, b*)/n + n*(b* - beta2)^2
+// is within 1e-12 of the exact SSE.  A candidate whose SSE exceeds tau is
+// dropped; the rest (new bests, near ties, boundary cases) take stage 3, the
+// verbatim binary64 fit of encode.rs:247-289, and compete by (rms, key).
+// Winners tighten tau through LDS; blocks exchange tau through a global
+// array every 8 tiles.  Tau starts from k_seed's sample of each range.
 template <int S>
 __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
 {
@@ -396,9 +448,11 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
     const double inv_n = 1.0 / (double)n;
 
     __shared__ uint32_t s_thr[8];
+    __shared__ unsigned long long s_count[2];
     __shared__ BestRec s_best[4][64][4];
     __shared__ __align__(16) int32_t s_t1[4][64 * T1_STRIDE];
 
+    if (threadIdx.x < 2) s_count[threadIdx.x] = 0;
     if (threadIdx.x < 8) {
         uint32_t r = g * 8 + threadIdx.x;
         s_thr[threadIdx.x] = r < a.nr ? a.gthr[r] : 0x7f800000u;
@@ -439,15 +493,15 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
     const uint32_t t_begin = chunk * a.tiles_per_chunk;
     uint32_t t_end = t_begin + a.tiles_per_chunk;
     if (t_end > ntiles) t_end = ntiles;
-    unsigned long long survivors = 0;
+    unsigned long long survivors = 0, stage3 = 0;
     int32_t *t1buf = &s_t1[wave][lane * T1_STRIDE];
     uint32_t tcount = 0;
 
     for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4, tcount++) {
-        // Refresh the thresholds from the other blocks of this range group.
+        // Exchange thresholds with the other blocks of this range group.
         if ((tcount & 7) == 7 && lane < 8) {
             uint32_t r = g * 8 + lane;
-            if (r < a.nr) atomicMin(&s_thr[lane], a.gthr[r]);
+            if (r < a.nr) atomicMin(&s_thr[lane], atomicMin(&a.gthr[r], s_thr[lane]));
         }
         const uint32_t p = tile * 32 + jj;
         const bool pv = p < a.P;
@@ -538,6 +592,7 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
                 }
         }
         const int32_t S2p = a.pos[2 * pc + 1];
+        const double Did = a.posd[2 * pc], invDi = a.posd[2 * pc + 1];
         while (surv) {
             int b = __builtin_ctz(surv);
             surv &= surv - 1;
@@ -545,26 +600,63 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
             int i = t & 3;
             uint32_t iso = 4 * tt + (t >> 2);
             int32_t t1 = t1buf[b] + sel32(off, i);
+            int64_t t0i = sel64(T0, i), t2i = sel64(T2, i);
+            survivors++;
+            bool exact = !a.stage2;
+            if (a.stage2) {
+                const double nd = (double)n, t0d = (double)t0i;
+                const double Ni = fma(nd, (double)t1, -((double)S1p * t0d));
+                double qa = 0.0;
+                if (Ni > 0.0 && Did > 0.0) {
+                    // m is within one of floor(x + 0.5); settle it with exact compares.
+                    double m = floor(Ni * a.alfa_c * invDi + 0.5);
+                    if (m <= a.qa_max + 1.0) {
+                        const double tc = 2.0 * a.alfa_c * Ni;
+                        if (fma(-(2.0 * m + 1.0), Did, tc) >= 0.0) m += 1.0;
+                        else if (m >= 1.0 && fma(-(2.0 * m - 1.0), Did, tc) < 0.0) m -= 1.0;
+                    }
+                    qa = m > a.qa_max ? a.qa_max : m;
+                }
+                const double a2 = qa * a.alfa_unit;
+                const double bs = (t0d - a2 * ((double)S1p * 0.25)) * inv_n;
+                double beta = bs;
+                if (a2 > 0.0) beta += a2 * 255.0;
+                const double yp = beta * a.yscale[(int)qa] + 0.5;
+                double mb = floor(yp);
+                const double fr = yp - mb;
+                if (fr < 1e-9 || fr > 1.0 - 1e-9) exact = true;
+                mb = mb < 0.0 ? 0.0 : (mb > a.qb_max ? a.qb_max : mb);
+                double b2 = mb / a.qb_max * ((1.0 + a2) * 255.0);
+                if (a2 > 0.0) b2 -= a2 * 255.0;
+                const double f = fma(a2 * a2 * 0.0625, Did, fma(-0.5 * a2, Ni, sel_d(A, i)));
+                const double dd = bs - b2;
+                const double sse = f * inv_n + nd * dd * dd;
+                const double tau = (double)__uint_as_float(s_thr[4 * h + i]);
+                if (sse <= tau) exact = true;
+            }
+            if (!exact) continue;
+            stage3++;
             uint32_t qa, qb;
             double rms, sum;
-            int64_t t0i = sel64(T0, i), t2i = sel64(T2, i);
             fit_f64(n, S1p, S2p, t0i, t1, t2i, a.max_alfa, a.bits_alfa, a.bits_beta, &qa, &qb,
                     &rms, &sum);
-            survivors++;
             uint32_t key = p * 8 + iso;
             BestRec &br = s_best[wave][lane][i];
             if (better(rms, key, br.rms, br.key)) {
                 br.rms = rms;
                 br.key = key;
                 double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
-                uint32_t bits = __float_as_uint((float)(tau * (1.0 + 1e-7)));
-                atomicMin(&s_thr[4 * h + i], bits);
-                atomicMin(&a.gthr[g * 8 + 4 * h + i], bits);
+                atomicMin(&s_thr[4 * h + i], __float_as_uint((float)(tau * (1.0 + 1e-7))));
             }
         }
     }
-    if (a.counters) atomicAdd(&a.counters[0], survivors);
+    atomicAdd(&s_count[0], survivors);
+    atomicAdd(&s_count[1], stage3);
     __syncthreads();
+    if (threadIdx.x == 0 && a.counters) {
+        atomicAdd(&a.counters[a.counter_slot], s_count[0]);
+        atomicAdd(&a.counters[8 + a.counter_slot], s_count[1]);
+    }
 
     if (threadIdx.x < 8) {
         uint32_t rl = threadIdx.x, hh = rl >> 2, i = rl & 3;
@@ -806,11 +898,21 @@ void launch_contract(const uint8_t *px, uint32_t w, uint32_t wc, uint32_t hc, ui
 }
 
 void launch_possums(const int32_t *d, uint32_t wc, uint32_t size, uint32_t shift, uint32_t npc,
-                    uint32_t P, int32_t *pos, float *posdi, hipStream_t st)
+                    uint32_t P, int32_t *pos, float *posdi, double *posd, hipStream_t st)
 {
     if (P == 0) return;
     hipLaunchKernelGGL(k_possums, dim3((P + 255) / 256), dim3(256), 0, st, d, wc, size, shift,
-                       npc, P, pos, posdi);
+                       npc, P, pos, posdi, posd);
+}
+
+void launch_seed(const int8_t *rng8, const int32_t *rstat, const int32_t *d, uint32_t wc,
+                 uint32_t size, uint32_t kmem, uint32_t shift, uint32_t npc, uint32_t P,
+                 const int32_t *pos, uint32_t nr, double max_alfa, uint32_t bits_alfa,
+                 uint32_t bits_beta, uint32_t *gthr, hipStream_t st)
+{
+    if (P == 0 || nr == 0) return;
+    hipLaunchKernelGGL(k_seed, dim3(nr), dim3(256), 0, st, rng8, rstat, d, wc, size, kmem, shift,
+                       npc, P, pos, nr, max_alfa, bits_alfa, bits_beta, gthr);
 }
 
 void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const uint32_t *cols,
