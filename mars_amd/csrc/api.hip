@@ -42,8 +42,7 @@ struct Table {
     float *d_rngf = nullptr;
     int32_t *d_rstat = nullptr;
     uint32_t *d_gthr = nullptr;
-    int32_t *d_pos = nullptr;
-    float *d_posdi = nullptr;
+    uint4 *d_posinfo = nullptr;
     double *d_posd = nullptr;
     Partial *d_partial = nullptr;
     mars_hip_candidate *d_out = nullptr;
@@ -68,6 +67,7 @@ struct mars_hip_ctx {
     int32_t *d_d = nullptr;
     float *d_df = nullptr;
     int8_t *d_lo = nullptr, *d_hi = nullptr;
+    int64_t *d_sat1 = nullptr, *d_sat2 = nullptr;
     unsigned long long *d_counters = nullptr;
     double *d_yscale = nullptr;
     int stage2 = 0;
@@ -140,8 +140,7 @@ void free_tables(mars_hip_ctx *ctx)
         (void)hipFree(t.d_rngf);
         (void)hipFree(t.d_rstat);
         (void)hipFree(t.d_gthr);
-        (void)hipFree(t.d_pos);
-        (void)hipFree(t.d_posdi);
+        (void)hipFree(t.d_posinfo);
         (void)hipFree(t.d_posd);
         (void)hipFree(t.d_partial);
         (void)hipFree(t.d_out);
@@ -154,7 +153,11 @@ void free_tables(mars_hip_ctx *ctx)
     (void)hipFree(ctx->d_lo);
     (void)hipFree(ctx->d_hi);
     (void)hipFree(ctx->d_yscale);
+    (void)hipFree(ctx->d_sat1);
+    (void)hipFree(ctx->d_sat2);
     ctx->d_yscale = nullptr;
+    ctx->d_sat1 = nullptr;
+    ctx->d_sat2 = nullptr;
     ctx->d_px = nullptr;
     ctx->d_d = nullptr;
     ctx->d_df = nullptr;
@@ -190,8 +193,7 @@ SearchArgs search_args(const mars_hip_ctx *ctx, const Table &t, int kernel)
     a.rngf = t.d_rngf;
     a.rstat = t.d_rstat;
     a.gthr = t.d_gthr;
-    a.pos = t.d_pos;
-    a.posdi = t.d_posdi;
+    a.posinfo = t.d_posinfo;
     a.posd = t.d_posd;
     a.lo = ctx->d_lo;
     a.hi = ctx->d_hi;
@@ -231,6 +233,7 @@ MergeArgs merge_args(const mars_hip_ctx *ctx, const Table &t, int kernel)
 {
     MergeArgs m{};
     m.partial = t.d_partial;
+    m.t1_exact = kernel == MARS_HIP_KERNEL_MFMA_I8 ? 1 : 0;
     m.nchunks = kernel == MARS_HIP_KERNEL_MFMA_I8 ? t.nchunks_mfma : t.nchunks_fp32;
     m.nr = t.nr;
     m.P = t.P;
@@ -243,7 +246,7 @@ MergeArgs merge_args(const mars_hip_ctx *ctx, const Table &t, int kernel)
     m.w = ctx->w;
     m.d = ctx->d_d;
     m.wc = ctx->wc;
-    m.pos = t.d_pos;
+    m.posinfo = t.d_posinfo;
     m.rstat = t.d_rstat;
     m.max_alfa = ctx->params.max_alfa;
     m.bits_alfa = ctx->params.bits_alfa;
@@ -262,15 +265,18 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
     mars::launch_contract(ctx->d_px, ctx->w, ctx->wc, ctx->hc, ctx->pitch, ctx->copy_stride,
                           ctx->d_d, ctx->d_df, ctx->d_lo, ctx->d_hi, ctx->d_counters, kCounters,
                           st);
+    mars::launch_sat(ctx->d_d, ctx->wc, ctx->hc, ctx->d_sat1, ctx->d_sat2, st);
     for (Table &t : ctx->tables) {
-        mars::launch_possums(ctx->d_d, ctx->wc, t.size, t.shift, t.npc, t.P, t.d_pos, t.d_posdi,
-                             t.d_posd, st);
+        mars::launch_possums(ctx->d_sat1, ctx->d_sat2, ctx->wc, t.size, t.shift, t.npc, t.P,
+                             t.d_posinfo, t.d_posd, st);
         mars::launch_ranges(ctx->d_px, ctx->w, t.d_rows, t.d_cols, t.nr, t.nr_pad, t.size,
-                            t.kmem, t.d_rng8, t.d_rngf, t.d_rstat, t.d_gthr, st);
+                            t.kmem, t.d_rng8,
+                            kernel == MARS_HIP_KERNEL_FP32 ? t.d_rngf : nullptr, t.d_rstat,
+                            t.d_gthr, st);
         if (kernel == MARS_HIP_KERNEL_MFMA_I8)
-            mars::launch_seed(t.d_rng8, t.d_rstat, ctx->d_d, ctx->wc, t.size, t.kmem, t.shift,
-                              t.npc, t.P, t.d_pos, t.nr, ctx->params.max_alfa,
-                              ctx->params.bits_alfa, ctx->params.bits_beta, t.d_gthr, st);
+            mars::launch_seed(t.d_rng8, t.d_rstat, ctx->d_d, ctx->wc, t.size, t.kmem, t.npc, t.P,
+                              t.d_posinfo, t.nr, ctx->params.max_alfa, ctx->params.bits_alfa,
+                              ctx->params.bits_beta, t.d_gthr, st);
     }
     if (ev) HIP_TRY(ctx, hipEventRecord(ev[1], st));
     for (size_t i = 0; i < ctx->tables.size(); i++) {
@@ -392,6 +398,9 @@ int mars_hip_prepare(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t 
     HIP_TRY(ctx, hipMalloc((void **)&ctx->d_hi, 16 * ctx->copy_stride));
     HIP_TRY(ctx, hipMemset(ctx->d_lo, 0, 16 * ctx->copy_stride));
     HIP_TRY(ctx, hipMemset(ctx->d_hi, 0, 16 * ctx->copy_stride));
+    size_t nsat = (size_t)(ctx->wc + 1) * (ctx->hc + 1);
+    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_sat1, nsat * sizeof(int64_t)));
+    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_sat2, nsat * sizeof(int64_t)));
 
     // Stage-2 constants: exact only when max_alfa is a power of two (then the
     // alfa scaling of encode.rs:264 is exact in binary64).
@@ -448,8 +457,7 @@ int mars_hip_prepare(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t 
             HIP_TRY(ctx, hipMalloc((void **)&t.d_rngf, (size_t)t.nr_pad / 4 * K * 32 * sizeof(float)));
             HIP_TRY(ctx, hipMalloc((void **)&t.d_rstat, sizeof(int32_t) * 2 * t.nr_pad));
             HIP_TRY(ctx, hipMalloc((void **)&t.d_gthr, sizeof(uint32_t) * t.nr_pad));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_pos, sizeof(int32_t) * 2 * std::max<uint32_t>(t.P, 1)));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_posdi, sizeof(float) * std::max<uint32_t>(t.P, 1)));
+            HIP_TRY(ctx, hipMalloc((void **)&t.d_posinfo, sizeof(uint4) * std::max<uint32_t>(t.P, 1)));
             HIP_TRY(ctx, hipMalloc((void **)&t.d_posd, 2 * sizeof(double) * std::max<uint32_t>(t.P, 1)));
             HIP_TRY(ctx, hipMalloc((void **)&t.d_partial, sizeof(Partial) * (size_t)t.nr_pad * nch));
             HIP_TRY(ctx, hipMalloc((void **)&t.d_out, sizeof(mars_hip_candidate) * std::max<uint32_t>(t.nr, 1)));

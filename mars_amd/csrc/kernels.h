@@ -14,7 +14,7 @@ namespace mars {
 struct Partial {
     double rms;
     uint32_t key; // (position index * 8 + isometry); 0xffffffff = none
-    uint32_t pad;
+    int32_t t1;   // exact t1_x4 of the winner (MFMA kernel), 0 otherwise
 };
 
 struct SearchArgs {
@@ -22,9 +22,8 @@ struct SearchArgs {
     const float *rngf;       // [nr_pad/4][K][32] r (FP32 baseline)
     const int32_t *rstat;    // [nr_pad][2] t0, t2
     uint32_t *gthr;          // [nr_pad] pruning threshold (float bits, SSE units)
-    const int32_t *pos;      // [P][2] S1, S2
-    const float *posdi;      // [P] Di = n*S2 - S1^2 (f32, stage-1 limit)
-    const double *posd;      // [P][2] Di, 1/Di (binary64, stage 2)
+    const uint4 *posinfo;    // [P] {dr | dc << 16, S1, S2, (float)(n*S2 - S1^2)}
+    const double *posd;      // [P][2] Di = n*S2 - S1^2, 1/Di (binary64, stage 2)
     const int8_t *lo, *hi;   // split planes, 16 shifted copies each
     size_t copy_stride;      // bytes between copies
     uint32_t pitch;          // bytes per contracted row in a copy
@@ -47,13 +46,15 @@ struct SearchArgs {
 
 struct MergeArgs {
     const Partial *partial;
+    int t1_exact;
     uint32_t nchunks, nr, P, npc, shift, size;
     const uint32_t *rows, *cols;
     const uint8_t *px;
     uint32_t w;
     const int32_t *d;
     uint32_t wc;
-    const int32_t *pos, *rstat;
+    const uint4 *posinfo;
+    const int32_t *rstat;
     double max_alfa;
     uint32_t bits_alfa, bits_beta;
     mars_hip_candidate *out;
@@ -64,12 +65,15 @@ int launch_search(int kernel, uint32_t size, const SearchArgs &a, hipStream_t st
 void launch_contract(const uint8_t *px, uint32_t w, uint32_t wc, uint32_t hc, uint32_t pitch,
                      size_t copy_stride, int32_t *d, float *df, int8_t *lo, int8_t *hi,
                      unsigned long long *counters, uint32_t n_counters, hipStream_t st);
-void launch_possums(const int32_t *d, uint32_t wc, uint32_t size, uint32_t shift, uint32_t npc,
-                    uint32_t P, int32_t *pos, float *posdi, double *posd, hipStream_t st);
+void launch_sat(const int32_t *d, uint32_t wc, uint32_t hc, int64_t *sat1, int64_t *sat2,
+                hipStream_t st);
+void launch_possums(const int64_t *sat1, const int64_t *sat2, uint32_t wc, uint32_t size,
+                    uint32_t shift, uint32_t npc, uint32_t P, uint4 *posinfo, double *posd,
+                    hipStream_t st);
 void launch_seed(const int8_t *rng8, const int32_t *rstat, const int32_t *d, uint32_t wc,
-                 uint32_t size, uint32_t kmem, uint32_t shift, uint32_t npc, uint32_t P,
-                 const int32_t *pos, uint32_t nr, double max_alfa, uint32_t bits_alfa,
-                 uint32_t bits_beta, uint32_t *gthr, hipStream_t st);
+                 uint32_t size, uint32_t kmem, uint32_t npc, uint32_t P, const uint4 *posinfo,
+                 uint32_t nr, double max_alfa, uint32_t bits_alfa, uint32_t bits_beta,
+                 uint32_t *gthr, hipStream_t st);
 void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const uint32_t *cols,
                    uint32_t nr, uint32_t nr_pad, uint32_t size, uint32_t kmem, int8_t *rng8,
                    float *rngf, int32_t *rstat, uint32_t *gthr, hipStream_t st);

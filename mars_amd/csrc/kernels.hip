@@ -193,69 +193,130 @@ __global__ void k_contract(const uint8_t *__restrict__ px, uint32_t w, uint32_t 
     }
 }
 
-// One thread per domain position of a table: window sums over D.
-__global__ void k_possums(const int32_t *__restrict__ d, uint32_t wc, uint32_t size,
-                          uint32_t shift, uint32_t npc, uint32_t P, int32_t *__restrict__ pos,
-                          float *__restrict__ posdi, double *__restrict__ posd)
+// Summed-area tables of D and D^2: sat[(i)(wc+1) + j] = sum over rows < i,
+// columns < j.  Pass 1 (one block per contracted row, plus block hc for the
+// zero row): inclusive row prefix sums by a block-wide scan.  Pass 2 (one
+// thread per column): running sums down the columns.
+__global__ void k_sat_rows(const int32_t *__restrict__ d, uint32_t wc, uint32_t hc,
+                           int64_t *__restrict__ sat1, int64_t *__restrict__ sat2)
+{
+    __shared__ int64_t s1[256], s2[256];
+    const uint32_t i = blockIdx.x, t = threadIdx.x, W1 = wc + 1;
+    if (i == hc) { // the zero row 0
+        for (uint32_t j = t; j < W1; j += 256) {
+            sat1[j] = 0;
+            sat2[j] = 0;
+        }
+        return;
+    }
+    int64_t *o1 = sat1 + (size_t)(i + 1) * W1, *o2 = sat2 + (size_t)(i + 1) * W1;
+    if (t == 0) {
+        o1[0] = 0;
+        o2[0] = 0;
+    }
+    int64_t c1 = 0, c2 = 0;
+    for (uint32_t base = 0; base < wc; base += 256) {
+        uint32_t j = base + t;
+        int64_t x = j < wc ? d[(size_t)i * wc + j] : 0;
+        s1[t] = x;
+        s2[t] = x * x;
+        __syncthreads();
+        for (uint32_t off = 1; off < 256; off <<= 1) {
+            int64_t a1 = t >= off ? s1[t - off] : 0, a2 = t >= off ? s2[t - off] : 0;
+            __syncthreads();
+            s1[t] += a1;
+            s2[t] += a2;
+            __syncthreads();
+        }
+        if (j < wc) {
+            o1[j + 1] = c1 + s1[t];
+            o2[j + 1] = c2 + s2[t];
+        }
+        c1 += s1[255];
+        c2 += s2[255];
+        __syncthreads();
+    }
+}
+
+__global__ void k_sat_cols(uint32_t wc, uint32_t hc, int64_t *__restrict__ sat1,
+                           int64_t *__restrict__ sat2)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x, W1 = wc + 1;
+    if (j >= W1) return;
+    int64_t a1 = 0, a2 = 0;
+    for (uint32_t i = 1; i <= hc; i++) {
+        a1 += sat1[(size_t)i * W1 + j];
+        a2 += sat2[(size_t)i * W1 + j];
+        sat1[(size_t)i * W1 + j] = a1;
+        sat2[(size_t)i * W1 + j] = a2;
+    }
+}
+
+// One thread per domain position of a table: window sums from the SATs,
+// packed as {dr | dc << 16, S1, S2, (float)Di} plus {Di, 1/Di} in binary64.
+__global__ void k_possums(const int64_t *__restrict__ sat1, const int64_t *__restrict__ sat2,
+                          uint32_t wc, uint32_t size, uint32_t shift, uint32_t npc, uint32_t P,
+                          uint4 *__restrict__ posinfo, double *__restrict__ posd)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
+    const uint32_t W1 = wc + 1;
     uint32_t pi = p / npc, pj = p - pi * npc;
     uint32_t dr = (pi * shift) >> 1, dc = (pj * shift) >> 1;
-    int64_t s1 = 0, s2 = 0;
-    for (uint32_t u = 0; u < size; u++) {
-        const int32_t *row = d + (size_t)(dr + u) * wc + dc;
-        for (uint32_t v = 0; v < size; v++) {
-            int64_t x = row[v];
-            s1 += x;
-            s2 += x * x;
-        }
-    }
-    pos[2 * p] = (int32_t)s1;
-    pos[2 * p + 1] = (int32_t)s2;
+    size_t a = (size_t)dr * W1 + dc, b = (size_t)(dr + size) * W1 + dc;
+    int64_t s1 = sat1[b + size] - sat1[a + size] - sat1[b] + sat1[a];
+    int64_t s2 = sat2[b + size] - sat2[a + size] - sat2[b] + sat2[a];
     int64_t n = (int64_t)size * size;
     int64_t di = n * s2 - s1 * s1;
-    posdi[p] = (float)di;
+    posinfo[p] = make_uint4(dr | (dc << 16), (uint32_t)s1, (uint32_t)s2, __float_as_uint((float)di));
     posd[2 * p] = (double)di;
     posd[2 * p + 1] = di > 0 ? 1.0 / (double)di : 0.0;
 }
 
-// Threshold seed: one block per range evaluates 256 spread candidates (32
-// positions x 8 isometries) by direct correlation and the binary64 fit, and
-// stores the smallest inflated SSE as the range's starting threshold.
+// Threshold seed: one block per range fits 64 spread candidates (8 positions
+// x 8 isometries); each candidate's correlation is split over 4 threads.
+// The smallest inflated SSE becomes the range's starting threshold.
 __global__ void k_seed(const int8_t *__restrict__ rng8, const int32_t *__restrict__ rstat,
                        const int32_t *__restrict__ d, uint32_t wc, uint32_t size, uint32_t kmem,
-                       uint32_t shift, uint32_t npc, uint32_t P, const int32_t *__restrict__ pos,
-                       uint32_t nr, double max_alfa, uint32_t bits_alfa, uint32_t bits_beta,
+                       uint32_t npc, uint32_t P, const uint4 *__restrict__ posinfo,
+                       double max_alfa, uint32_t bits_alfa, uint32_t bits_beta,
                        uint32_t *__restrict__ gthr)
 {
     __shared__ uint32_t s_min;
-    const uint32_t r = blockIdx.x;
-    if (threadIdx.x == 0) s_min = 0x7f800000u;
+    __shared__ int32_t s_t1[64];
+    const uint32_t r = blockIdx.x, t = threadIdx.x, c = t & 63, q = t >> 6;
+    if (t == 0) s_min = 0x7f800000u;
+    if (t < 64) s_t1[t] = 0;
     __syncthreads();
-    const uint32_t j = threadIdx.x >> 3, iso = threadIdx.x & 7;
-    const uint32_t p = P >= 32 ? (uint32_t)(((uint64_t)j * (P - 1)) / 31) : j % P;
-    const uint32_t pi = p / npc, pj = p - pi * npc;
-    const uint32_t dr = (pi * shift) >> 1, dc = (pj * shift) >> 1;
+    const uint32_t j = c >> 3, iso = c & 7;
+    const uint32_t p = P >= 8 ? (uint32_t)(((uint64_t)j * (P - 1)) / 7) : j % P;
+    const uint4 pin = posinfo[p];
+    const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
     const int8_t *rk = rng8 + ((size_t)r * 8 + iso) * kmem;
-    int64_t t1 = 0;
-    for (uint32_t u = 0; u < size; u++) {
-        const int32_t *row = d + (size_t)(dr + u) * wc + dc;
-        for (uint32_t v = 0; v < size; v++) t1 += (int64_t)((int)rk[u * size + v] + 128) * row[v];
+    const uint32_t K = size * size, k0 = q * K / 4, k1 = (q + 1) * K / 4;
+    int32_t t1 = 0;
+    for (uint32_t k = k0; k < k1; k++) {
+        uint32_t u = k / size, v = k - u * size;
+        t1 += ((int32_t)rk[k] + 128) * d[(size_t)(dr + u) * wc + dc + v];
     }
-    const int64_t n = (int64_t)size * size;
-    uint32_t qa, qb;
-    double rms, sum;
-    fit_f64(n, pos[2 * p], pos[2 * p + 1], rstat[2 * r], t1, rstat[2 * r + 1], max_alfa,
-            bits_alfa, bits_beta, &qa, &qb, &rms, &sum);
-    double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
-    atomicMin(&s_min, __float_as_uint((float)(tau * (1.0 + 1e-7))));
+    atomicAdd(&s_t1[c], t1);
     __syncthreads();
-    if (threadIdx.x == 0 && r < nr) gthr[r] = s_min;
+    if (t < 64) {
+        const int64_t n = (int64_t)K;
+        uint32_t qa, qb;
+        double rms, sum;
+        fit_f64(n, (int32_t)pin.y, (int32_t)pin.z, rstat[2 * r], s_t1[c], rstat[2 * r + 1],
+                max_alfa, bits_alfa, bits_beta, &qa, &qb, &rms, &sum);
+        double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
+        atomicMin(&s_min, __float_as_uint((float)(tau * (1.0 + 1e-7))));
+    }
+    __syncthreads();
+    if (t == 0) gthr[r] = s_min;
 }
 
 // One block per (padded) range.  rng8 layout [range][iso][kmem] (i8, r-128,
-// zero padded to kmem); rngf layout [range/4][k][iso*4 + range%4] (f32 r).
+// zero padded to kmem); rngf (FP32 baseline only, may be null) layout
+// [range/4][k][iso*4 + range%4] (f32 r).
 __global__ void k_ranges(const uint8_t *__restrict__ px, uint32_t w,
                          const uint32_t *__restrict__ rows, const uint32_t *__restrict__ cols,
                          uint32_t nr, uint32_t size, uint32_t kmem, int8_t *__restrict__ rng8,
@@ -284,7 +345,7 @@ __global__ void k_ranges(const uint8_t *__restrict__ px, uint32_t w,
             fv = (float)pix;
         }
         rng8[((size_t)r * 8 + iso) * kmem + k] = val;
-        if (k < K) rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = fv;
+        if (rngf && k < K) rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = fv;
     }
     int32_t s = 0, s2 = 0;
     if (valid)
@@ -394,10 +455,56 @@ __device__ __forceinline__ void load_a(const int8_t *rng8, uint32_t g, int tt, i
 struct BestRec {
     double rms;
     uint32_t key;
-    uint32_t pad;
+    int32_t t1; // exact t1_x4 of the record
 };
 
-constexpr int T1_STRIDE = 36; // ints per lane in the survivor staging area
+// Inflated threshold of a candidate whose binary64 SSE (`sum` of
+// encode.rs:283-285) is `sum`: anything whose exact SSE exceeds it has a
+// strictly larger binary64 rms (relative 1e-9 plus absolute 1e-6*n cover the
+// rounding of `sum`, of /s0 and of sqrt).  Rounded up to binary32.
+__device__ __forceinline__ uint32_t tau_bits(double sum, int64_t n)
+{
+    double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
+    return __float_as_uint((float)(tau * (1.0 + 1e-7)));
+}
+
+// Stage 2 (see k_search_mfma): true when the candidate may beat or tie the
+// threshold, or sits on a quantisation boundary -- i.e. needs stage 3.
+__device__ __forceinline__ bool stage2_keep(const SearchArgs &a, double nd, double inv_n,
+                                            int32_t t1, int32_t S1, int64_t t0, double Di,
+                                            double invDi, double A, float thr)
+{
+    const double t0d = (double)t0;
+    const double Ni = fma(nd, (double)t1, -((double)S1 * t0d));
+    double qa = 0.0;
+    if (Ni > 0.0 && Di > 0.0) {
+        // m is within one of floor(x + 0.5); settle it with exact compares.
+        double m = floor(Ni * a.alfa_c * invDi + 0.5);
+        if (m <= a.qa_max + 1.0) {
+            const double tc = 2.0 * a.alfa_c * Ni;
+            if (fma(-(2.0 * m + 1.0), Di, tc) >= 0.0) m += 1.0;
+            else if (m >= 1.0 && fma(-(2.0 * m - 1.0), Di, tc) < 0.0) m -= 1.0;
+        }
+        qa = m > a.qa_max ? a.qa_max : m;
+    }
+    const double a2 = qa * a.alfa_unit;
+    const double bs = (t0d - a2 * ((double)S1 * 0.25)) * inv_n;
+    double beta = bs;
+    if (a2 > 0.0) beta += a2 * 255.0;
+    const double yp = beta * a.yscale[(int)qa] + 0.5;
+    double mb = floor(yp);
+    const double fr = yp - mb;
+    const bool boundary = fr < 1e-9 || fr > 1.0 - 1e-9;
+    mb = mb < 0.0 ? 0.0 : (mb > a.qb_max ? a.qb_max : mb);
+    double b2 = mb / a.qb_max * ((1.0 + a2) * 255.0);
+    if (a2 > 0.0) b2 -= a2 * 255.0;
+    const double f = fma(a2 * a2 * 0.0625, Di, fma(-0.5 * a2, Ni, A));
+    const double dd = bs - b2;
+    const double sse = f * inv_n + nd * dd * dd;
+    return boundary || sse <= (double)thr;
+}
+
+constexpr int QCAP = 64 + 2048; // survivor queue entries per wave (one tile adds <= 2048)
 
 // Certified split-INT8 search for one range size S.
 //
@@ -412,18 +519,20 @@ constexpr int T1_STRIDE = 36; // ints per lane in the survivor staging area
 // Correlation.  r' = r - 128 (i8), D = 256*Dh + Dl, Dl' = Dl - 128 (i8):
 //   t1_x4 = sum r*D = 256*sum r'Dh + sum r'Dl' + 128*(sum r' + S1)     (exact)
 //
-// Epilogue.  For every candidate the least-squares residual with free slope
-// and offset is a lower bound of its quantised SSE:
+// Stage 1 (every candidate).  The least-squares residual with free slope
+// and offset is a lower bound of the quantised SSE:
 //   n*SSE >= A - Ni^2/Di,  A = n*t2 - t0^2, Di = n*S2 - S1^2,
 //   Ni = n*t1 - S1*t0 (moments in the x4/x16 units of encode.rs:224-232).
 // Given a threshold tau (an SSE some candidate of this range already
 // reached, inflated so that anything above it has a strictly larger binary64
 // rms), a candidate can only win if Ni >= sqrt((A - n tau) Di), i.e.
 //   t1_x4 > floor((S1*t0 + sqrt((A - n tau) Di)) / n) - 1.
-// That is one integer compare per candidate; the limit is computed once per
-// (range, position) with a 1e-5 relative safety margin on the square root.
+// One integer compare per candidate; the limit is computed once per (range,
+// position), rounded so that it never excludes more than the exact bound.
 // Candidates with Ni < 0 are covered too (their qalfa is 0, SSE >= A/n).
-// Stage 2 (per stage-1 survivor, all in binary64 with exact operands): the
+// Survivors are appended to a per-wave LDS queue (wave ballot + mbcnt).
+//
+// Stage 2 (queue batches of 64, binary64 with exact operands): the
 // candidate's quantised codes and SSE without any IEEE divide or square root.
 // qalfa is decided exactly from integers (2c*Ni >= (2k-1)*Di, c the alfa
 // scale); beta, alfa2 and the slope part of the SSE
@@ -433,39 +542,49 @@ constexpr int T1_STRIDE = 36; // ints per lane in the survivor staging area
 //   SSE = n*SSE(alfa2, b*)/n + n*(b* - beta2)^2
 // is within 1e-12 of the exact SSE.  A candidate whose SSE exceeds tau is
 // dropped; the rest (new bests, near ties, boundary cases) take stage 3, the
-// verbatim binary64 fit of encode.rs:247-289, and compete by (rms, key).
-// Winners tighten tau through LDS; blocks exchange tau through a global
-// array every 8 tiles.  Tau starts from k_seed's sample of each range.
+// verbatim binary64 fit of encode.rs:247-289, and compete with the wave's
+// best by (rms, key).  Winners tighten tau through LDS; blocks exchange tau
+// through a global array every 8 tiles.  Tau starts from k_seed's sample.
 template <int S>
 __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
 {
     using G = Geo<S>;
     constexpr int NK = G::NK;
+    constexpr int64_t n = (int64_t)S * S;
+    constexpr int LOG2N = S == 2 ? 2 : S == 4 ? 4 : S == 8 ? 6 : S == 16 ? 8 : 10;
+    constexpr bool INT_LIMIT = S <= 8; // every stage-1 quantity fits 32 bits
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5, jj = lane & 31;
     const uint32_t g = blockIdx.y, chunk = blockIdx.x;
-    const int64_t n = (int64_t)S * S;
-    const double inv_n = 1.0 / (double)n;
+    const double nd = (double)n, inv_n = 1.0 / (double)n;
 
     __shared__ uint32_t s_thr[8];
     __shared__ unsigned long long s_count[2];
-    __shared__ BestRec s_best[4][64][4];
-    __shared__ __align__(16) int32_t s_t1[4][64 * T1_STRIDE];
+    __shared__ int64_t s_T0[8], s_T2[8];
+    __shared__ double s_A[8];
+    __shared__ BestRec s_wb[4][8];
+    __shared__ uint2 s_q[4][QCAP];
 
     if (threadIdx.x < 2) s_count[threadIdx.x] = 0;
     if (threadIdx.x < 8) {
         uint32_t r = g * 8 + threadIdx.x;
-        s_thr[threadIdx.x] = r < a.nr ? a.gthr[r] : 0x7f800000u;
+        bool v = r < a.nr;
+        s_thr[threadIdx.x] = v ? a.gthr[r] : 0x7f800000u;
+        int64_t t0 = v ? a.rstat[2 * r] : 0, t2 = v ? a.rstat[2 * r + 1] : 0;
+        s_T0[threadIdx.x] = t0;
+        s_T2[threadIdx.x] = t2;
+        s_A[threadIdx.x] = (double)(n * t2 - t0 * t0);
     }
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        s_best[wave][lane][i].rms = __longlong_as_double(0x7ff0000000000000ll);
-        s_best[wave][lane][i].key = 0xffffffffu;
+    if (lane < 8) {
+        s_wb[wave][lane].rms = __longlong_as_double(0x7ff0000000000000ll);
+        s_wb[wave][lane].key = 0xffffffffu;
+        s_wb[wave][lane].t1 = 0;
     }
 
     // Range statistics of the lane's 4 ranges.
-    int64_t T0[4], T2[4];
+    int64_t T0[4];
     double A[4];
+    float Alo[4];
     int32_t srp[4];
     bool rvalid[4];
 #pragma unroll
@@ -474,8 +593,9 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
         rvalid[i] = r < a.nr;
         uint32_t rr = rvalid[i] ? r : 0;
         T0[i] = a.rstat[2 * rr];
-        T2[i] = a.rstat[2 * rr + 1];
-        A[i] = (double)(n * T2[i] - T0[i] * T0[i]);
+        const int64_t t2 = a.rstat[2 * rr + 1];
+        A[i] = (double)(n * t2 - T0[i] * T0[i]);
+        Alo[i] = (float)A[i] * (1.0f - 3e-7f); // <= A after both roundings
         srp[i] = (int32_t)(T0[i] - 128 * n);
     }
 
@@ -493,9 +613,51 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
     const uint32_t t_begin = chunk * a.tiles_per_chunk;
     uint32_t t_end = t_begin + a.tiles_per_chunk;
     if (t_end > ntiles) t_end = ntiles;
-    unsigned long long survivors = 0, stage3 = 0;
-    int32_t *t1buf = &s_t1[wave][lane * T1_STRIDE];
+    unsigned long long qtotal = 0, stage3 = 0; // stage-1 survivors (wave), stage-3 fits (lane)
     uint32_t tcount = 0;
+    uint32_t qn = 0; // wave-uniform queue length
+    uint2 *q = s_q[wave];
+    BestRec *wb = s_wb[wave];
+
+    // Stage 2 + 3 over the queue entries [b0, b0 + cnt), cnt <= 64.
+    auto process = [&](uint32_t b0, uint32_t cnt) {
+        const bool act = (uint32_t)lane < cnt;
+        const uint2 e = q[b0 + (act ? lane : 0)];
+        const int32_t t1 = (int32_t)(e.x & 0x0fffffffu);
+        const uint32_t slot = e.x >> 28, key = e.y, p = key >> 3;
+        bool keep = false, win = false;
+        double rms = 0.0, sum = 0.0;
+        if (act) {
+            const uint4 pin = a.posinfo[p];
+            const int32_t S1 = (int32_t)pin.y, S2 = (int32_t)pin.z;
+            const double Di = a.posd[2 * p], invDi = a.posd[2 * p + 1];
+            const int64_t t0 = s_T0[slot], t2 = s_T2[slot];
+            keep = !a.stage2 || stage2_keep(a, nd, inv_n, t1, S1, t0, Di, invDi, s_A[slot],
+                                            __uint_as_float(s_thr[slot]));
+            if (keep) {
+                uint32_t qa, qb;
+                fit_f64(n, S1, S2, t0, t1, t2, a.max_alfa, a.bits_alfa, a.bits_beta, &qa, &qb,
+                        &rms, &sum);
+                win = better(rms, key, wb[slot].rms, wb[slot].key);
+            }
+        }
+        stage3 += keep ? 1 : 0;
+        // Rare: serialise the updates of the wave's best records.
+        unsigned long long m = __ballot(win);
+        while (m) {
+            const int l = __builtin_ctzll(m);
+            m &= m - 1;
+            const double r_l = __shfl(rms, l), s_l = __shfl(sum, l);
+            const uint32_t k_l = __shfl(key, l), sl_l = __shfl(slot, l);
+            const int32_t t_l = __shfl(t1, l);
+            if (lane == 0 && better(r_l, k_l, wb[sl_l].rms, wb[sl_l].key)) {
+                wb[sl_l].rms = r_l;
+                wb[sl_l].key = k_l;
+                wb[sl_l].t1 = t_l;
+                atomicMin(&s_thr[sl_l], tau_bits(s_l, n));
+            }
+        }
+    };
 
     for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4, tcount++) {
         // Exchange thresholds with the other blocks of this range group.
@@ -506,8 +668,10 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
         const uint32_t p = tile * 32 + jj;
         const bool pv = p < a.P;
         const uint32_t pc = pv ? p : a.P - 1;
-        const uint32_t pi = pc / a.npc, pj = pc - pi * a.npc;
-        const uint32_t dr = (pi * a.shift) >> 1, dc = (pj * a.shift) >> 1;
+        const uint4 pin = a.posinfo[pc];
+        const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
+        const int32_t S1p = (int32_t)pin.y;
+        const float Dif = __uint_as_float(pin.w);
 
         v16i acc_lo[2], acc_hi[2];
 #pragma unroll
@@ -549,108 +713,71 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
             }
         }
 
-        // Per (range, position) pruning limit on the raw correlation.
-        const int32_t S1p = a.pos[2 * pc];
-        const float Dif = a.posdi[pc];
+        // Stage-1 limit per (range, position) on the raw correlation.
         int32_t lim[4], off[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             off[i] = 128 * (srp[i] + S1p);
-            float thr = __uint_as_float(s_thr[4 * h + i]);
-            double nthr = (double)n * (double)thr;
-            if (!pv || !rvalid[i]) {
-                lim[i] = 0x7fffffff;
-            } else if (nthr < A[i]) {
-                float amt = (float)(A[i] - nthr);
-                float nt = sqrtf(amt * Dif) * (1.0f - 1e-5f);
-                double X = ((double)S1p * (double)T0[i] + (double)nt) * inv_n;
-                lim[i] = (int32_t)floor(X) - 1 - off[i];
+            const float thr = __uint_as_float(s_thr[4 * h + i]);
+            int32_t L;
+            if constexpr (INT_LIMIT) {
+                const float amt = Alo[i] - (float)n * thr;
+                if (amt > 0.0f) {
+                    const float nt = sqrtf(amt * Dif) * (1.0f - 1e-5f);
+                    const uint64_t X = (uint64_t)(uint32_t)S1p * (uint64_t)(uint32_t)T0[i] +
+                                       (uint64_t)(uint32_t)nt;
+                    L = (int32_t)(X >> LOG2N) - 1 - off[i];
+                } else {
+                    L = (int32_t)0x80000000;
+                }
             } else {
-                lim[i] = (int32_t)0x80000000;
+                const double nthr = nd * (double)thr;
+                if (nthr < A[i]) {
+                    const float amt = (float)(A[i] - nthr);
+                    const float nt = sqrtf(amt * Dif) * (1.0f - 1e-5f);
+                    const double X = ((double)S1p * (double)T0[i] + (double)nt) * inv_n;
+                    L = (int32_t)floor(X) - 1 - off[i];
+                } else {
+                    L = (int32_t)0x80000000;
+                }
             }
+            lim[i] = (pv && rvalid[i]) ? L : 0x7fffffff;
         }
 
-        uint32_t surv = 0;
+        // One compare per value; appends happen only for values that
+        // survive in some lane (a wave-uniform branch).
 #pragma unroll
         for (int tt = 0; tt < 2; tt++)
 #pragma unroll
             for (int t = 0; t < 16; t++) {
-                int32_t v = (acc_hi[tt][t] << 8) + acc_lo[tt][t];
-                surv |= (v > lim[t & 3] ? 1u : 0u) << (tt * 16 + t);
-            }
-        if (surv) {
-#pragma unroll
-            for (int tt = 0; tt < 2; tt++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    v4i o;
-                    o.x = (acc_hi[tt][4 * q + 0] << 8) + acc_lo[tt][4 * q + 0];
-                    o.y = (acc_hi[tt][4 * q + 1] << 8) + acc_lo[tt][4 * q + 1];
-                    o.z = (acc_hi[tt][4 * q + 2] << 8) + acc_lo[tt][4 * q + 2];
-                    o.w = (acc_hi[tt][4 * q + 3] << 8) + acc_lo[tt][4 * q + 3];
-                    *(v4i *)&t1buf[tt * 16 + 4 * q] = o;
-                }
-        }
-        const int32_t S2p = a.pos[2 * pc + 1];
-        const double Did = a.posd[2 * pc], invDi = a.posd[2 * pc + 1];
-        while (surv) {
-            int b = __builtin_ctz(surv);
-            surv &= surv - 1;
-            int t = b & 15, tt = b >> 4;
-            int i = t & 3;
-            uint32_t iso = 4 * tt + (t >> 2);
-            int32_t t1 = t1buf[b] + sel32(off, i);
-            int64_t t0i = sel64(T0, i), t2i = sel64(T2, i);
-            survivors++;
-            bool exact = !a.stage2;
-            if (a.stage2) {
-                const double nd = (double)n, t0d = (double)t0i;
-                const double Ni = fma(nd, (double)t1, -((double)S1p * t0d));
-                double qa = 0.0;
-                if (Ni > 0.0 && Did > 0.0) {
-                    // m is within one of floor(x + 0.5); settle it with exact compares.
-                    double m = floor(Ni * a.alfa_c * invDi + 0.5);
-                    if (m <= a.qa_max + 1.0) {
-                        const double tc = 2.0 * a.alfa_c * Ni;
-                        if (fma(-(2.0 * m + 1.0), Did, tc) >= 0.0) m += 1.0;
-                        else if (m >= 1.0 && fma(-(2.0 * m - 1.0), Did, tc) < 0.0) m -= 1.0;
+                const int32_t v = (acc_hi[tt][t] << 8) + acc_lo[tt][t];
+                const int i = t & 3;
+                const bool sv = v > lim[i];
+                const unsigned long long m = __ballot(sv);
+                if (m) {
+                    const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                    if (sv) {
+                        const uint32_t slot = 4 * h + i;
+                        q[qn + pre] = make_uint2((uint32_t)(v + off[i]) | (slot << 28),
+                                                 p * 8 + 4 * tt + (t >> 2));
                     }
-                    qa = m > a.qa_max ? a.qa_max : m;
+                    qn += (uint32_t)__popcll(m);
+                    qtotal += (uint32_t)__popcll(m);
                 }
-                const double a2 = qa * a.alfa_unit;
-                const double bs = (t0d - a2 * ((double)S1p * 0.25)) * inv_n;
-                double beta = bs;
-                if (a2 > 0.0) beta += a2 * 255.0;
-                const double yp = beta * a.yscale[(int)qa] + 0.5;
-                double mb = floor(yp);
-                const double fr = yp - mb;
-                if (fr < 1e-9 || fr > 1.0 - 1e-9) exact = true;
-                mb = mb < 0.0 ? 0.0 : (mb > a.qb_max ? a.qb_max : mb);
-                double b2 = mb / a.qb_max * ((1.0 + a2) * 255.0);
-                if (a2 > 0.0) b2 -= a2 * 255.0;
-                const double f = fma(a2 * a2 * 0.0625, Did, fma(-0.5 * a2, Ni, sel_d(A, i)));
-                const double dd = bs - b2;
-                const double sse = f * inv_n + nd * dd * dd;
-                const double tau = (double)__uint_as_float(s_thr[4 * h + i]);
-                if (sse <= tau) exact = true;
             }
-            if (!exact) continue;
-            stage3++;
-            uint32_t qa, qb;
-            double rms, sum;
-            fit_f64(n, S1p, S2p, t0i, t1, t2i, a.max_alfa, a.bits_alfa, a.bits_beta, &qa, &qb,
-                    &rms, &sum);
-            uint32_t key = p * 8 + iso;
-            BestRec &br = s_best[wave][lane][i];
-            if (better(rms, key, br.rms, br.key)) {
-                br.rms = rms;
-                br.key = key;
-                double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
-                atomicMin(&s_thr[4 * h + i], __float_as_uint((float)(tau * (1.0 + 1e-7))));
-            }
+        // Drain full batches; keep the tail (< 64) for the next tile.
+        if (qn >= 64) {
+            uint32_t b0 = 0;
+            for (; qn - b0 >= 64; b0 += 64) process(b0, 64);
+            const uint32_t rem = qn - b0;
+            const uint2 tail = q[b0 + ((uint32_t)lane < rem ? lane : 0)];
+            if ((uint32_t)lane < rem) q[lane] = tail;
+            qn = rem;
         }
     }
-    atomicAdd(&s_count[0], survivors);
+    if (qn > 0) process(0, qn);
+    if (lane == 0) atomicAdd(&s_count[0], qtotal);
     atomicAdd(&s_count[1], stage3);
     __syncthreads();
     if (threadIdx.x == 0 && a.counters) {
@@ -659,22 +786,17 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
     }
 
     if (threadIdx.x < 8) {
-        uint32_t rl = threadIdx.x, hh = rl >> 2, i = rl & 3;
-        uint32_t r = g * 8 + rl;
+        const uint32_t sl = threadIdx.x, r = g * 8 + sl;
         if (r < a.nr) {
-            double brms = __longlong_as_double(0x7ff0000000000000ll);
-            uint32_t bkey = 0xffffffffu;
-            for (int wv = 0; wv < 4; wv++)
-                for (int l = 0; l < 32; l++) {
-                    const BestRec &e = s_best[wv][hh * 32 + l][i];
-                    if (better(e.rms, e.key, brms, bkey)) {
-                        brms = e.rms;
-                        bkey = e.key;
-                    }
-                }
+            BestRec b = s_wb[0][sl];
+            for (int wv = 1; wv < 4; wv++) {
+                const BestRec &e = s_wb[wv][sl];
+                if (better(e.rms, e.key, b.rms, b.key)) b = e;
+            }
             Partial &out = a.partial[(size_t)r * a.nchunks + chunk];
-            out.rms = brms;
-            out.key = bkey;
+            out.rms = b.rms;
+            out.key = b.key;
+            out.t1 = b.t1;
         }
     }
 }
@@ -718,14 +840,14 @@ __global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
     const uint32_t t_begin = chunk * a.tiles_per_chunk;
     uint32_t t_end = t_begin + a.tiles_per_chunk;
     if (t_end > ntiles) t_end = ntiles;
-    const float *A = a.rngf + (size_t)g * K * 32;
+    const float *Ar = a.rngf + (size_t)g * K * 32;
 
     for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4) {
         const uint32_t p = tile * 64 + lane;
         const bool pv = p < a.P;
         const uint32_t pc = pv ? p : a.P - 1;
-        const uint32_t pi = pc / a.npc, pj = pc - pi * a.npc;
-        const uint32_t dr = (pi * a.shift) >> 1, dc = (pj * a.shift) >> 1;
+        const uint4 pin = a.posinfo[pc];
+        const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
         const float *D = a.dplanef + (size_t)dr * a.wc + dc;
         float acc[32];
 #pragma unroll
@@ -734,13 +856,13 @@ __global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
 #pragma unroll 8
             for (int v = 0; v < S; v++) {
                 float d = D[(size_t)u * a.wc + v];
-                const float *ar = A + (size_t)(u * S + v) * 32;
+                const float *ar = Ar + (size_t)(u * S + v) * 32;
 #pragma unroll
                 for (int r = 0; r < 32; r++) acc[r] = __builtin_fmaf(ar[r], d, acc[r]);
             }
         }
         if (!pv) continue;
-        const float S1 = (float)a.pos[2 * pc], S2 = (float)a.pos[2 * pc + 1];
+        const float S1 = (float)(int32_t)pin.y, S2 = (float)(int32_t)pin.z;
 #pragma unroll
         for (int r = 0; r < 32; r++) {
             const int i = r & 3;
@@ -778,67 +900,97 @@ __global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
             Partial &out = a.partial[(size_t)r * a.nchunks + chunk];
             out.rms = b;
             out.key = k;
+            out.t1 = 0;
         }
     }
 }
 
 // ---------------------------------------------------------------- merge
 
-// One thread per range: min over chunks by (rms, key); the winner's exact
-// moments and quantised coefficients are recomputed from the pool (a direct
-// window correlation and the binary64 fit) so the table holds the
-// reference's Candidate (encode.rs:234-244) field for field.  For the FP32
-// baseline the recomputed rms replaces the binary32 one.
+// One 64-thread block per range: min over position chunks by (rms, key),
+// then the winner's RawMoments and (qalfa, qbeta, rms) through the binary64
+// fit, so the table holds the reference's Candidate (encode.rs:234-244)
+// field for field.  The MFMA kernel hands over the winner's exact t1; for the
+// FP32 baseline t1 is recomputed by a direct window correlation.
 __global__ void k_merge(MergeArgs a)
 {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.nr) return;
-    mars_hip_candidate c = {};
-    uint8_t found = 0;
-    if (a.P > 0) {
-        double b = __longlong_as_double(0x7ff0000000000000ll);
-        uint32_t k = 0xffffffffu;
-        for (uint32_t ch = 0; ch < a.nchunks; ch++) {
+    __shared__ double s_rms[64];
+    __shared__ uint32_t s_key[64];
+    __shared__ int32_t s_t1[64];
+    __shared__ long long s_acc;
+    const uint32_t r = blockIdx.x, t = threadIdx.x;
+    double b = __longlong_as_double(0x7ff0000000000000ll);
+    uint32_t k = 0xffffffffu;
+    int32_t t1 = 0;
+    if (a.P > 0)
+        for (uint32_t ch = t; ch < a.nchunks; ch += 64) {
             const Partial &e = a.partial[(size_t)r * a.nchunks + ch];
             if (better(e.rms, e.key, b, k)) {
                 b = e.rms;
                 k = e.key;
+                t1 = e.t1;
             }
         }
-        if (k != 0xffffffffu) {
-            found = 1;
-            uint32_t p = k >> 3, iso = k & 7;
-            uint32_t pi = p / a.npc, pj = p - pi * a.npc;
-            c.dom_row = pi * a.shift;
-            c.dom_col = pj * a.shift;
-            c.isometry = iso;
-            uint32_t dr = c.dom_row >> 1, dc = c.dom_col >> 1;
-            uint32_t S = a.size, row = a.rows[r], col = a.cols[r];
-            int64_t t1 = 0;
-            for (uint32_t u = 0; u < S; u++)
-                for (uint32_t v = 0; v < S; v++) {
-                    uint32_t i, j;
-                    iso_map(iso, u, v, S, &i, &j);
-                    t1 += (int64_t)a.px[(size_t)(row + i) * a.w + col + j] *
-                          (int64_t)a.d[(size_t)(dr + u) * a.wc + dc + v];
-                }
-            c.s0 = (int64_t)S * S;
-            c.s1_x4 = a.pos[2 * p];
-            c.s2_x16 = a.pos[2 * p + 1];
-            c.t0 = a.rstat[2 * r];
-            c.t1_x4 = t1;
-            c.t2 = a.rstat[2 * r + 1];
-            uint32_t qa, qb;
-            double rms, sum;
-            fit_f64(c.s0, c.s1_x4, c.s2_x16, c.t0, c.t1_x4, c.t2, a.max_alfa, a.bits_alfa,
-                    a.bits_beta, &qa, &qb, &rms, &sum);
-            c.qalfa = qa;
-            c.qbeta = qb;
-            c.rms = rms;
-        }
+    s_rms[t] = b;
+    s_key[t] = k;
+    s_t1[t] = t1;
+    if (t == 0) s_acc = 0;
+    __syncthreads();
+    if (t == 0) {
+        for (int l = 1; l < 64; l++)
+            if (better(s_rms[l], s_key[l], b, k)) {
+                b = s_rms[l];
+                k = s_key[l];
+                t1 = s_t1[l];
+            }
+        s_key[0] = k;
+        s_t1[0] = t1;
     }
+    __syncthreads();
+    k = s_key[0];
+    if (k == 0xffffffffu) {
+        if (t == 0) {
+            a.out[r] = mars_hip_candidate{};
+            a.found[r] = 0;
+        }
+        return;
+    }
+    const uint32_t p = k >> 3, iso = k & 7, S = a.size;
+    const uint4 pin = a.posinfo[p];
+    const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
+    if (!a.t1_exact) {
+        const uint32_t row = a.rows[r], col = a.cols[r];
+        long long part = 0;
+        for (uint32_t e = t; e < S * S; e += 64) {
+            uint32_t u = e / S, v = e - u * S, i, j;
+            iso_map(iso, u, v, S, &i, &j);
+            part += (long long)a.px[(size_t)(row + i) * a.w + col + j] *
+                    (long long)a.d[(size_t)(dr + u) * a.wc + dc + v];
+        }
+        atomicAdd((unsigned long long *)&s_acc, (unsigned long long)part);
+        __syncthreads();
+    }
+    if (t != 0) return;
+    mars_hip_candidate c = {};
+    const uint32_t pi = p / a.npc, pj = p - pi * a.npc;
+    c.dom_row = pi * a.shift;
+    c.dom_col = pj * a.shift;
+    c.isometry = iso;
+    c.s0 = (int64_t)S * S;
+    c.s1_x4 = (int32_t)pin.y;
+    c.s2_x16 = (int32_t)pin.z;
+    c.t0 = a.rstat[2 * r];
+    c.t1_x4 = a.t1_exact ? (int64_t)s_t1[0] : (int64_t)s_acc;
+    c.t2 = a.rstat[2 * r + 1];
+    uint32_t qa, qb;
+    double rms, sum;
+    fit_f64(c.s0, c.s1_x4, c.s2_x16, c.t0, c.t1_x4, c.t2, a.max_alfa, a.bits_alfa, a.bits_beta,
+            &qa, &qb, &rms, &sum);
+    c.qalfa = qa;
+    c.qbeta = qb;
+    c.rms = rms;
     a.out[r] = c;
-    a.found[r] = found;
+    a.found[r] = 1;
 }
 
 __global__ void k_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
@@ -897,22 +1049,31 @@ void launch_contract(const uint8_t *px, uint32_t w, uint32_t wc, uint32_t hc, ui
                        copy_stride, d, df, lo, hi, counters, n_counters);
 }
 
-void launch_possums(const int32_t *d, uint32_t wc, uint32_t size, uint32_t shift, uint32_t npc,
-                    uint32_t P, int32_t *pos, float *posdi, double *posd, hipStream_t st)
+void launch_sat(const int32_t *d, uint32_t wc, uint32_t hc, int64_t *sat1, int64_t *sat2,
+                hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sat_rows, dim3(hc + 1), dim3(256), 0, st, d, wc, hc, sat1, sat2);
+    hipLaunchKernelGGL(k_sat_cols, dim3((wc + 1 + 255) / 256), dim3(256), 0, st, wc, hc, sat1,
+                       sat2);
+}
+
+void launch_possums(const int64_t *sat1, const int64_t *sat2, uint32_t wc, uint32_t size,
+                    uint32_t shift, uint32_t npc, uint32_t P, uint4 *posinfo, double *posd,
+                    hipStream_t st)
 {
     if (P == 0) return;
-    hipLaunchKernelGGL(k_possums, dim3((P + 255) / 256), dim3(256), 0, st, d, wc, size, shift,
-                       npc, P, pos, posdi, posd);
+    hipLaunchKernelGGL(k_possums, dim3((P + 255) / 256), dim3(256), 0, st, sat1, sat2, wc, size,
+                       shift, npc, P, posinfo, posd);
 }
 
 void launch_seed(const int8_t *rng8, const int32_t *rstat, const int32_t *d, uint32_t wc,
-                 uint32_t size, uint32_t kmem, uint32_t shift, uint32_t npc, uint32_t P,
-                 const int32_t *pos, uint32_t nr, double max_alfa, uint32_t bits_alfa,
-                 uint32_t bits_beta, uint32_t *gthr, hipStream_t st)
+                 uint32_t size, uint32_t kmem, uint32_t npc, uint32_t P, const uint4 *posinfo,
+                 uint32_t nr, double max_alfa, uint32_t bits_alfa, uint32_t bits_beta,
+                 uint32_t *gthr, hipStream_t st)
 {
     if (P == 0 || nr == 0) return;
-    hipLaunchKernelGGL(k_seed, dim3(nr), dim3(256), 0, st, rng8, rstat, d, wc, size, kmem, shift,
-                       npc, P, pos, nr, max_alfa, bits_alfa, bits_beta, gthr);
+    hipLaunchKernelGGL(k_seed, dim3(nr), dim3(256), 0, st, rng8, rstat, d, wc, size, kmem, npc, P,
+                       posinfo, max_alfa, bits_alfa, bits_beta, gthr);
 }
 
 void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const uint32_t *cols,
@@ -926,7 +1087,7 @@ void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const ui
 void launch_merge(const MergeArgs &a, hipStream_t st)
 {
     if (a.nr == 0) return;
-    hipLaunchKernelGGL(k_merge, dim3((a.nr + 127) / 128), dim3(128), 0, st, a);
+    hipLaunchKernelGGL(k_merge, dim3(a.nr), dim3(64), 0, st, a);
 }
 
 void launch_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
