@@ -22,6 +22,9 @@ ARCH = os.environ.get("MARS_HIP_ARCH", "gfx950")
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs: the epilogue reads every accumulator, so
+    # AGPR copies would cost one instruction per value.
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall", "-Wno-unused-function",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]

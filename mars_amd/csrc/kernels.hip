@@ -504,7 +504,7 @@ __device__ __forceinline__ bool stage2_keep(const SearchArgs &a, double nd, doub
     return boundary || sse <= (double)thr;
 }
 
-constexpr int QCAP = 64 + 2048; // survivor queue entries per wave (one tile adds <= 2048)
+constexpr int QCAP = 64 + 1024; // survivor queue entries per wave (one row tile adds <= 1024)
 
 // Certified split-INT8 search for one range size S.
 //
@@ -659,16 +659,43 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
         }
     };
 
+    // Software pipeline (sizes whose A fragments live in registers): the
+    // position record and the B fragments of the next tile are loaded while
+    // the current tile is multiplied and screened.
+    struct Frag {
+        uint4 pin;
+        long l8[NK], h8[NK];
+        v4i l16[NK], h16[NK];
+    };
+    auto fetch = [&](uint32_t tile, Frag &f) {
+        const uint32_t p = tile * 32 + jj;
+        const uint32_t pc = p < a.P ? p : a.P - 1;
+        f.pin = a.posinfo[pc];
+        if constexpr (G::PRELOAD_A) {
+            const uint32_t dr = f.pin.x & 0xffff, dc = f.pin.x >> 16;
+#pragma unroll
+            for (int kk = 0; kk < NK; kk++) {
+                load_b<S>(a.lo, a.copy_stride, a.pitch, dr, dc, kk, h, &f.l8[kk], &f.l16[kk]);
+                load_b<S>(a.hi, a.copy_stride, a.pitch, dr, dc, kk, h, &f.h8[kk], &f.h16[kk]);
+            }
+        }
+    };
+    Frag nxt;
+    if (t_begin + wave < t_end) fetch(t_begin + wave, nxt);
+    const bool exchange = a.nchunks > 1;
+
     for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4, tcount++) {
         // Exchange thresholds with the other blocks of this range group.
-        if ((tcount & 7) == 7 && lane < 8) {
+        if (exchange && (tcount & 15) == 15 && lane < 8) {
             uint32_t r = g * 8 + lane;
             if (r < a.nr) atomicMin(&s_thr[lane], atomicMin(&a.gthr[r], s_thr[lane]));
         }
+        const Frag cur = nxt;
+        if (tile + 4 < t_end) fetch(tile + 4, nxt);
         const uint32_t p = tile * 32 + jj;
         const bool pv = p < a.P;
         const uint32_t pc = pv ? p : a.P - 1;
-        const uint4 pin = a.posinfo[pc];
+        const uint4 pin = cur.pin;
         const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
         const int32_t S1p = (int32_t)pin.y;
         const float Dif = __uint_as_float(pin.w);
@@ -682,18 +709,14 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
         if constexpr (G::PRELOAD_A) {
 #pragma unroll
             for (int kk = 0; kk < NK; kk++) {
-                long bl8 = 0, bh8 = 0;
-                v4i bl16 = {}, bh16 = {};
-                load_b<S>(a.lo, a.copy_stride, a.pitch, dr, dc, kk, h, &bl8, &bl16);
-                load_b<S>(a.hi, a.copy_stride, a.pitch, dr, dc, kk, h, &bh8, &bh16);
 #pragma unroll
                 for (int tt = 0; tt < 2; tt++) {
                     if constexpr (G::SMALLK) {
-                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], bl8, acc_lo[tt], 0, 0, 0);
-                        acc_hi[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], bh8, acc_hi[tt], 0, 0, 0);
+                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], cur.l8[kk], acc_lo[tt], 0, 0, 0);
+                        acc_hi[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], cur.h8[kk], acc_hi[tt], 0, 0, 0);
                     } else {
-                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], bl16, acc_lo[tt], 0, 0, 0);
-                        acc_hi[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], bh16, acc_hi[tt], 0, 0, 0);
+                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], cur.l16[kk], acc_lo[tt], 0, 0, 0);
+                        acc_hi[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], cur.h16[kk], acc_hi[tt], 0, 0, 0);
                     }
                 }
             }
@@ -723,7 +746,7 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
             if constexpr (INT_LIMIT) {
                 const float amt = Alo[i] - (float)n * thr;
                 if (amt > 0.0f) {
-                    const float nt = sqrtf(amt * Dif) * (1.0f - 1e-5f);
+                    const float nt = __builtin_amdgcn_sqrtf(amt * Dif) * (1.0f - 1e-5f);
                     const uint64_t X = (uint64_t)(uint32_t)S1p * (uint64_t)(uint32_t)T0[i] +
                                        (uint64_t)(uint32_t)nt;
                     L = (int32_t)(X >> LOG2N) - 1 - off[i];
@@ -734,7 +757,7 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
                 const double nthr = nd * (double)thr;
                 if (nthr < A[i]) {
                     const float amt = (float)(A[i] - nthr);
-                    const float nt = sqrtf(amt * Dif) * (1.0f - 1e-5f);
+                    const float nt = __builtin_amdgcn_sqrtf(amt * Dif) * (1.0f - 1e-5f);
                     const double X = ((double)S1p * (double)T0[i] + (double)nt) * inv_n;
                     L = (int32_t)floor(X) - 1 - off[i];
                 } else {
@@ -745,9 +768,10 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
         }
 
         // One compare per value; appends happen only for values that
-        // survive in some lane (a wave-uniform branch).
+        // survive in some lane (a wave-uniform branch).  The queue is
+        // drained after each 32-row tile, so one tile adds at most 1024.
 #pragma unroll
-        for (int tt = 0; tt < 2; tt++)
+        for (int tt = 0; tt < 2; tt++) {
 #pragma unroll
             for (int t = 0; t < 16; t++) {
                 const int32_t v = (acc_hi[tt][t] << 8) + acc_lo[tt][t];
@@ -766,14 +790,15 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
                     qtotal += (uint32_t)__popcll(m);
                 }
             }
-        // Drain full batches; keep the tail (< 64) for the next tile.
-        if (qn >= 64) {
-            uint32_t b0 = 0;
-            for (; qn - b0 >= 64; b0 += 64) process(b0, 64);
-            const uint32_t rem = qn - b0;
-            const uint2 tail = q[b0 + ((uint32_t)lane < rem ? lane : 0)];
-            if ((uint32_t)lane < rem) q[lane] = tail;
-            qn = rem;
+            // Drain full batches; keep the tail (< 64) for later.
+            if (qn >= 64) {
+                uint32_t b0 = 0;
+                for (; qn - b0 >= 64; b0 += 64) process(b0, 64);
+                const uint32_t rem = qn - b0;
+                const uint2 tail = q[b0 + ((uint32_t)lane < rem ? lane : 0)];
+                if ((uint32_t)lane < rem) q[lane] = tail;
+                qn = rem;
+            }
         }
     }
     if (qn > 0) process(0, qn);
