@@ -244,7 +244,23 @@ __global__ void k_sat_cols(uint32_t wc, uint32_t hc, int64_t *__restrict__ sat1,
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x, W1 = wc + 1;
     if (j >= W1) return;
     int64_t a1 = 0, a2 = 0;
-    for (uint32_t i = 1; i <= hc; i++) {
+    uint32_t i = 1;
+    for (; i + 7 <= hc; i += 8) {
+        int64_t x1[8], x2[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            x1[u] = sat1[(size_t)(i + u) * W1 + j];
+            x2[u] = sat2[(size_t)(i + u) * W1 + j];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            a1 += x1[u];
+            a2 += x2[u];
+            sat1[(size_t)(i + u) * W1 + j] = a1;
+            sat2[(size_t)(i + u) * W1 + j] = a2;
+        }
+    }
+    for (; i <= hc; i++) {
         a1 += sat1[(size_t)i * W1 + j];
         a2 += sat2[(size_t)i * W1 + j];
         sat1[(size_t)i * W1 + j] = a1;
@@ -273,9 +289,10 @@ __global__ void k_possums(const int64_t *__restrict__ sat1, const int64_t *__res
     posd[2 * p + 1] = di > 0 ? 1.0 / (double)di : 0.0;
 }
 
-// Threshold seed: one block per range fits 64 spread candidates (8 positions
-// x 8 isometries); each candidate's correlation is split over 4 threads.
-// The smallest inflated SSE becomes the range's starting threshold.
+// Threshold seed: one 256-thread block per range fits 256/q spread
+// candidates (positions x 8 isometries), each candidate's correlation split
+// over q = max(1, S*S/128) threads.  The smallest inflated SSE becomes the
+// range's starting threshold.
 __global__ void k_seed(const int8_t *__restrict__ rng8, const int32_t *__restrict__ rstat,
                        const int32_t *__restrict__ d, uint32_t wc, uint32_t size, uint32_t kmem,
                        uint32_t npc, uint32_t P, const uint4 *__restrict__ posinfo,
@@ -283,25 +300,28 @@ __global__ void k_seed(const int8_t *__restrict__ rng8, const int32_t *__restric
                        uint32_t *__restrict__ gthr)
 {
     __shared__ uint32_t s_min;
-    __shared__ int32_t s_t1[64];
-    const uint32_t r = blockIdx.x, t = threadIdx.x, c = t & 63, q = t >> 6;
+    __shared__ int32_t s_t1[256];
+    const uint32_t K = size * size;
+    const uint32_t q = K / 128 > 1 ? K / 128 : 1, ncand = 256 / q, npos = ncand / 8;
+    const uint32_t r = blockIdx.x, t = threadIdx.x, c = t % ncand, sl = t / ncand;
     if (t == 0) s_min = 0x7f800000u;
-    if (t < 64) s_t1[t] = 0;
+    s_t1[t] = 0;
     __syncthreads();
     const uint32_t j = c >> 3, iso = c & 7;
-    const uint32_t p = P >= 8 ? (uint32_t)(((uint64_t)j * (P - 1)) / 7) : j % P;
+    const uint32_t p = P >= npos ? (uint32_t)(((uint64_t)j * (P - 1)) / (npos - 1)) : j % P;
     const uint4 pin = posinfo[p];
     const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
     const int8_t *rk = rng8 + ((size_t)r * 8 + iso) * kmem;
-    const uint32_t K = size * size, k0 = q * K / 4, k1 = (q + 1) * K / 4;
+    const uint32_t k0 = sl * K / q, k1 = (sl + 1) * K / q;
     int32_t t1 = 0;
+#pragma unroll 8
     for (uint32_t k = k0; k < k1; k++) {
         uint32_t u = k / size, v = k - u * size;
-        t1 += ((int32_t)rk[k] + 128) * d[(size_t)(dr + u) * wc + dc + v];
+        t1 += ((int32_t)rk[k] + 128) * d[(dr + u) * wc + dc + v];
     }
     atomicAdd(&s_t1[c], t1);
     __syncthreads();
-    if (t < 64) {
+    if (t < ncand) {
         const int64_t n = (int64_t)K;
         uint32_t qa, qb;
         double rms, sum;
@@ -314,53 +334,65 @@ __global__ void k_seed(const int8_t *__restrict__ rng8, const int32_t *__restric
     if (t == 0) gthr[r] = s_min;
 }
 
-// One block per (padded) range.  rng8 layout [range][iso][kmem] (i8, r-128,
-// zero padded to kmem); rngf (FP32 baseline only, may be null) layout
-// [range/4][k][iso*4 + range%4] (f32 r).
+// Range operands: one thread per 4 bytes of rng8, layout [range][iso][kmem]
+// (i8 r-128, zero padded to kmem); rngf (FP32 baseline only, may be null)
+// layout [range/4][k][iso*4 + range%4] (f32 r).  lg_size/lg_kmem: log2.
 __global__ void k_ranges(const uint8_t *__restrict__ px, uint32_t w,
                          const uint32_t *__restrict__ rows, const uint32_t *__restrict__ cols,
-                         uint32_t nr, uint32_t size, uint32_t kmem, int8_t *__restrict__ rng8,
-                         float *__restrict__ rngf, int32_t *__restrict__ rstat,
-                         uint32_t *__restrict__ gthr)
+                         uint32_t nr, uint32_t nr_pad, uint32_t lg_size, uint32_t lg_kmem,
+                         int8_t *__restrict__ rng8, float *__restrict__ rngf)
 {
-    uint32_t r = blockIdx.x;
-    uint32_t K = size * size;
-    __shared__ int32_t st[2];
-    if (threadIdx.x == 0) {
-        st[0] = 0;
-        st[1] = 0;
-    }
-    __syncthreads();
-    bool valid = r < nr;
-    uint32_t row = valid ? rows[r] : 0, col = valid ? cols[r] : 0;
-    for (uint32_t e = threadIdx.x; e < 8 * kmem; e += blockDim.x) {
-        uint32_t iso = e / kmem, k = e - iso * kmem;
-        int8_t val = 0;
-        float fv = 0.0f;
+    const uint32_t e = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    const uint32_t size = 1u << lg_size, K = size * size, kmem = 1u << lg_kmem;
+    if (e >= nr_pad * 8 * kmem) return;
+    const uint32_t r = e >> (lg_kmem + 3), iso = (e >> lg_kmem) & 7, k0 = e & (kmem - 1);
+    const bool valid = r < nr;
+    const uint32_t row = valid ? rows[r] : 0, col = valid ? cols[r] : 0;
+    uint32_t packed = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < 4; b++) {
+        const uint32_t k = k0 + b;
         if (valid && k < K) {
-            uint32_t u = k / size, v = k - u * size, i, j;
+            const uint32_t u = k >> lg_size, v = k & (size - 1);
+            uint32_t i, j;
             iso_map(iso, u, v, size, &i, &j);
-            uint32_t pix = px[(size_t)(row + i) * w + col + j];
-            val = (int8_t)((int)pix - 128);
-            fv = (float)pix;
+            const uint32_t pix = px[(row + i) * w + col + j];
+            packed |= ((pix - 128u) & 0xffu) << (8 * b);
+            if (rngf) rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = (float)pix;
+        } else if (rngf && k < K) {
+            rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = 0.0f;
         }
-        rng8[((size_t)r * 8 + iso) * kmem + k] = val;
-        if (rngf && k < K) rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = fv;
     }
-    int32_t s = 0, s2 = 0;
-    if (valid)
-        for (uint32_t e = threadIdx.x; e < K; e += blockDim.x) {
-            uint32_t u = e / size, v = e - u * size;
-            int32_t pix = px[(size_t)(row + u) * w + col + v];
-            s += pix;
+    *(uint32_t *)(rng8 + e) = packed;
+}
+
+// Range sums t0 = sum r, t2 = sum r^2: one wave per (padded) range.  Also
+// resets the range's pruning threshold.
+__global__ void k_rstats(const uint8_t *__restrict__ px, uint32_t w,
+                         const uint32_t *__restrict__ rows, const uint32_t *__restrict__ cols,
+                         uint32_t nr, uint32_t nr_pad, uint32_t lg_size,
+                         int32_t *__restrict__ rstat, uint32_t *__restrict__ gthr)
+{
+    const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (r >= nr_pad) return;
+    const uint32_t size = 1u << lg_size, K = size * size;
+    int32_t s1 = 0, s2 = 0;
+    if (r < nr) {
+        const uint32_t row = rows[r], col = cols[r];
+        for (uint32_t e = lane; e < K; e += 64) {
+            const int32_t pix = px[(row + (e >> lg_size)) * w + col + (e & (size - 1))];
+            s1 += pix;
             s2 += pix * pix;
         }
-    atomicAdd(&st[0], s);
-    atomicAdd(&st[1], s2);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        rstat[2 * r] = st[0];
-        rstat[2 * r + 1] = st[1];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 += __shfl_xor(s1, off);
+        s2 += __shfl_xor(s2, off);
+    }
+    if (lane == 0) {
+        rstat[2 * r] = s1;
+        rstat[2 * r + 1] = s2;
         gthr[r] = 0x7f800000u; // +inf: no threshold yet
     }
 }
@@ -377,23 +409,24 @@ template <int S> struct Geo {
 };
 
 // Load 4/8/16 aligned bytes of a split plane: plane[row][col .. col+n).
+// All 16 copies of a plane span < 4 GiB, so offsets are 32-bit.
 __device__ __forceinline__ uint32_t ld4(const int8_t *base, size_t cs, uint32_t pitch,
                                         uint32_t row, uint32_t col)
 {
     uint32_t c = col & 3;
-    return *(const uint32_t *)(base + c * cs + (size_t)row * pitch + (col - c));
+    return *(const uint32_t *)(base + (c * (uint32_t)cs + row * pitch + (col - c)));
 }
 __device__ __forceinline__ uint2 ld8(const int8_t *base, size_t cs, uint32_t pitch,
                                      uint32_t row, uint32_t col)
 {
     uint32_t c = col & 7;
-    return *(const uint2 *)(base + c * cs + (size_t)row * pitch + (col - c));
+    return *(const uint2 *)(base + (c * (uint32_t)cs + row * pitch + (col - c)));
 }
 __device__ __forceinline__ v4i ld16(const int8_t *base, size_t cs, uint32_t pitch,
                                     uint32_t row, uint32_t col)
 {
     uint32_t c = col & 15;
-    return *(const v4i *)(base + c * cs + (size_t)row * pitch + (col - c));
+    return *(const v4i *)(base + (c * (uint32_t)cs + row * pitch + (col - c)));
 }
 
 // B fragment (one split plane) of lane half h for K-step kk at window
@@ -409,9 +442,8 @@ __device__ __forceinline__ void load_b(const int8_t *plane, size_t cs, uint32_t 
         uint32_t x = 0;
         if (h == 0) {
             const uint8_t *p = (const uint8_t *)plane; // copy 0 = unshifted plane
-            uint32_t a0 = p[(size_t)dr * pitch + dc], a1 = p[(size_t)dr * pitch + dc + 1];
-            uint32_t a2 = p[(size_t)(dr + 1) * pitch + dc],
-                     a3 = p[(size_t)(dr + 1) * pitch + dc + 1];
+            uint32_t a0 = p[dr * pitch + dc], a1 = p[dr * pitch + dc + 1];
+            uint32_t a2 = p[(dr + 1) * pitch + dc], a3 = p[(dr + 1) * pitch + dc + 1];
             x = a0 | (a1 << 8) | (a2 << 16) | (a3 << 24);
         }
         *b8 = (long)x;
@@ -659,20 +691,22 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
         }
     };
 
-    // Software pipeline (sizes whose A fragments live in registers): the
-    // position record and the B fragments of the next tile are loaded while
-    // the current tile is multiplied and screened.
+    // Software pipeline: the position record is loaded two tiles ahead and
+    // (for sizes whose A fragments live in registers) the B fragments one
+    // tile ahead, so neither load's latency sits in front of the MFMAs.
     struct Frag {
         uint4 pin;
         long l8[NK], h8[NK];
         v4i l16[NK], h16[NK];
     };
-    auto fetch = [&](uint32_t tile, Frag &f) {
+    auto load_pin = [&](uint32_t tile) {
         const uint32_t p = tile * 32 + jj;
-        const uint32_t pc = p < a.P ? p : a.P - 1;
-        f.pin = a.posinfo[pc];
+        return a.posinfo[p < a.P ? p : a.P - 1];
+    };
+    auto fetch_b = [&](const uint4 pin, Frag &f) {
+        f.pin = pin;
         if constexpr (G::PRELOAD_A) {
-            const uint32_t dr = f.pin.x & 0xffff, dc = f.pin.x >> 16;
+            const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
 #pragma unroll
             for (int kk = 0; kk < NK; kk++) {
                 load_b<S>(a.lo, a.copy_stride, a.pitch, dr, dc, kk, h, &f.l8[kk], &f.l16[kk]);
@@ -681,7 +715,9 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
         }
     };
     Frag nxt;
-    if (t_begin + wave < t_end) fetch(t_begin + wave, nxt);
+    uint4 pin_ahead = make_uint4(0, 0, 0, 0);
+    if (t_begin + wave < t_end) fetch_b(load_pin(t_begin + wave), nxt);
+    if (t_begin + wave + 4 < t_end) pin_ahead = load_pin(t_begin + wave + 4);
     const bool exchange = a.nchunks > 1;
 
     for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4, tcount++) {
@@ -691,10 +727,10 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
             if (r < a.nr) atomicMin(&s_thr[lane], atomicMin(&a.gthr[r], s_thr[lane]));
         }
         const Frag cur = nxt;
-        if (tile + 4 < t_end) fetch(tile + 4, nxt);
+        if (tile + 4 < t_end) fetch_b(pin_ahead, nxt);
+        if (tile + 8 < t_end) pin_ahead = load_pin(tile + 8);
         const uint32_t p = tile * 32 + jj;
         const bool pv = p < a.P;
-        const uint32_t pc = pv ? p : a.P - 1;
         const uint4 pin = cur.pin;
         const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
         const int32_t S1p = (int32_t)pin.y;
@@ -932,11 +968,65 @@ __global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
 
 // ---------------------------------------------------------------- merge
 
-// One 64-thread block per range: min over position chunks by (rms, key),
-// then the winner's RawMoments and (qalfa, qbeta, rms) through the binary64
-// fit, so the table holds the reference's Candidate (encode.rs:234-244)
-// field for field.  The MFMA kernel hands over the winner's exact t1; for the
-// FP32 baseline t1 is recomputed by a direct window correlation.
+__device__ __forceinline__ void write_winner(const MergeArgs &a, uint32_t r, uint32_t k,
+                                             int64_t t1)
+{
+    mars_hip_candidate c = {};
+    const uint32_t p = k >> 3, iso = k & 7, S = a.size;
+    const uint4 pin = a.posinfo[p];
+    const uint32_t pi = p / a.npc, pj = p - pi * a.npc;
+    c.dom_row = pi * a.shift;
+    c.dom_col = pj * a.shift;
+    c.isometry = iso;
+    c.s0 = (int64_t)S * S;
+    c.s1_x4 = (int32_t)pin.y;
+    c.s2_x16 = (int32_t)pin.z;
+    c.t0 = a.rstat[2 * r];
+    c.t1_x4 = t1;
+    c.t2 = a.rstat[2 * r + 1];
+    uint32_t qa, qb;
+    double rms, sum;
+    fit_f64(c.s0, c.s1_x4, c.s2_x16, c.t0, c.t1_x4, c.t2, a.max_alfa, a.bits_alfa, a.bits_beta,
+            &qa, &qb, &rms, &sum);
+    c.qalfa = qa;
+    c.qbeta = qb;
+    c.rms = rms;
+    a.out[r] = c;
+    a.found[r] = 1;
+}
+
+// One thread per range, for partials that carry the winner's exact t1 (the
+// MFMA search): min over position chunks by (rms, key), then the winner's
+// RawMoments and (qalfa, qbeta, rms) through the binary64 fit.
+__global__ void k_merge_exact(MergeArgs a)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.nr) return;
+    double b = __longlong_as_double(0x7ff0000000000000ll);
+    uint32_t k = 0xffffffffu;
+    int32_t t1 = 0;
+    if (a.P > 0)
+        for (uint32_t ch = 0; ch < a.nchunks; ch++) {
+            const Partial e = a.partial[(size_t)r * a.nchunks + ch];
+            if (better(e.rms, e.key, b, k)) {
+                b = e.rms;
+                k = e.key;
+                t1 = e.t1;
+            }
+        }
+    if (k == 0xffffffffu) {
+        a.out[r] = mars_hip_candidate{};
+        a.found[r] = 0;
+        return;
+    }
+    write_winner(a, r, k, t1);
+}
+
+// One 64-thread block per range, for partials without an exact t1 (the FP32
+// baseline): min over position chunks by (rms, key), the winner's t1 by a
+// direct window correlation, then its RawMoments and (qalfa, qbeta, rms)
+// through the binary64 fit, so the table holds the reference's Candidate
+// (encode.rs:234-244) field for field.
 __global__ void k_merge(MergeArgs a)
 {
     __shared__ double s_rms[64];
@@ -983,7 +1073,7 @@ __global__ void k_merge(MergeArgs a)
     const uint32_t p = k >> 3, iso = k & 7, S = a.size;
     const uint4 pin = a.posinfo[p];
     const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
-    if (!a.t1_exact) {
+    {
         const uint32_t row = a.rows[r], col = a.cols[r];
         long long part = 0;
         for (uint32_t e = t; e < S * S; e += 64) {
@@ -996,26 +1086,7 @@ __global__ void k_merge(MergeArgs a)
         __syncthreads();
     }
     if (t != 0) return;
-    mars_hip_candidate c = {};
-    const uint32_t pi = p / a.npc, pj = p - pi * a.npc;
-    c.dom_row = pi * a.shift;
-    c.dom_col = pj * a.shift;
-    c.isometry = iso;
-    c.s0 = (int64_t)S * S;
-    c.s1_x4 = (int32_t)pin.y;
-    c.s2_x16 = (int32_t)pin.z;
-    c.t0 = a.rstat[2 * r];
-    c.t1_x4 = a.t1_exact ? (int64_t)s_t1[0] : (int64_t)s_acc;
-    c.t2 = a.rstat[2 * r + 1];
-    uint32_t qa, qb;
-    double rms, sum;
-    fit_f64(c.s0, c.s1_x4, c.s2_x16, c.t0, c.t1_x4, c.t2, a.max_alfa, a.bits_alfa, a.bits_beta,
-            &qa, &qb, &rms, &sum);
-    c.qalfa = qa;
-    c.qbeta = qb;
-    c.rms = rms;
-    a.out[r] = c;
-    a.found[r] = 1;
+    write_winner(a, r, k, (int64_t)s_acc);
 }
 
 __global__ void k_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
@@ -1105,14 +1176,21 @@ void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const ui
                    uint32_t nr, uint32_t nr_pad, uint32_t size, uint32_t kmem, int8_t *rng8,
                    float *rngf, int32_t *rstat, uint32_t *gthr, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ranges, dim3(nr_pad), dim3(256), 0, st, px, w, rows, cols, nr, size,
-                       kmem, rng8, rngf, rstat, gthr);
+    const uint32_t lg_size = __builtin_ctz(size), lg_kmem = __builtin_ctz(kmem);
+    const uint32_t nthreads = nr_pad * 8 * kmem / 4;
+    hipLaunchKernelGGL(k_ranges, dim3((nthreads + 255) / 256), dim3(256), 0, st, px, w, rows, cols,
+                       nr, nr_pad, lg_size, lg_kmem, rng8, rngf);
+    hipLaunchKernelGGL(k_rstats, dim3((nr_pad + 3) / 4), dim3(256), 0, st, px, w, rows, cols, nr,
+                       nr_pad, lg_size, rstat, gthr);
 }
 
 void launch_merge(const MergeArgs &a, hipStream_t st)
 {
     if (a.nr == 0) return;
-    hipLaunchKernelGGL(k_merge, dim3(a.nr), dim3(64), 0, st, a);
+    if (a.t1_exact)
+        hipLaunchKernelGGL(k_merge_exact, dim3((a.nr + 127) / 128), dim3(128), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_merge, dim3(a.nr), dim3(64), 0, st, a);
 }
 
 void launch_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
