@@ -238,34 +238,39 @@ __global__ void k_sat_rows(const int32_t *__restrict__ d, uint32_t wc, uint32_t 
     }
 }
 
+// Column pass: a block covers 16 columns x 16 row segments; each thread
+// scans its segment, the segment totals are combined in LDS, and a second
+// sweep adds the preceding segments' total.
 __global__ void k_sat_cols(uint32_t wc, uint32_t hc, int64_t *__restrict__ sat1,
                            int64_t *__restrict__ sat2)
 {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x, W1 = wc + 1;
-    if (j >= W1) return;
+    __shared__ int64_t tot1[16][16], tot2[16][16];
+    const uint32_t cl = threadIdx.x & 15, seg = threadIdx.x >> 4, W1 = wc + 1;
+    const uint32_t j = blockIdx.x * 16 + cl;
+    const uint32_t L = (hc + 15) / 16, r0 = 1 + seg * L;
+    uint32_t r1 = r0 + L;
+    if (r1 > hc + 1) r1 = hc + 1;
     int64_t a1 = 0, a2 = 0;
-    uint32_t i = 1;
-    for (; i + 7 <= hc; i += 8) {
-        int64_t x1[8], x2[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            x1[u] = sat1[(size_t)(i + u) * W1 + j];
-            x2[u] = sat2[(size_t)(i + u) * W1 + j];
+    if (j < W1)
+        for (uint32_t i = r0; i < r1; i++) {
+            a1 += sat1[(size_t)i * W1 + j];
+            a2 += sat2[(size_t)i * W1 + j];
+            sat1[(size_t)i * W1 + j] = a1;
+            sat2[(size_t)i * W1 + j] = a2;
         }
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            a1 += x1[u];
-            a2 += x2[u];
-            sat1[(size_t)(i + u) * W1 + j] = a1;
-            sat2[(size_t)(i + u) * W1 + j] = a2;
+    tot1[seg][cl] = a1;
+    tot2[seg][cl] = a2;
+    __syncthreads();
+    int64_t o1 = 0, o2 = 0;
+    for (uint32_t s2 = 0; s2 < seg; s2++) {
+        o1 += tot1[s2][cl];
+        o2 += tot2[s2][cl];
+    }
+    if (j < W1 && seg > 0)
+        for (uint32_t i = r0; i < r1; i++) {
+            sat1[(size_t)i * W1 + j] += o1;
+            sat2[(size_t)i * W1 + j] += o2;
         }
-    }
-    for (; i <= hc; i++) {
-        a1 += sat1[(size_t)i * W1 + j];
-        a2 += sat2[(size_t)i * W1 + j];
-        sat1[(size_t)i * W1 + j] = a1;
-        sat2[(size_t)i * W1 + j] = a2;
-    }
 }
 
 // One thread per domain position of a table: window sums from the SATs,
@@ -289,9 +294,9 @@ __global__ void k_possums(const int64_t *__restrict__ sat1, const int64_t *__res
     posd[2 * p + 1] = di > 0 ? 1.0 / (double)di : 0.0;
 }
 
-// Threshold seed: one 256-thread block per range fits 256/q spread
-// candidates (positions x 8 isometries), each candidate's correlation split
-// over q = max(1, S*S/128) threads.  The smallest inflated SSE becomes the
+// Threshold seed: one 256-thread block per range fits 64 spread candidates
+// (8 positions x 8 isometries; 32 = 4 x 8 at S >= 32), each candidate's
+// correlation split over 256/candidates threads.  The smallest inflated SSE becomes the
 // range's starting threshold.
 __global__ void k_seed(const int8_t *__restrict__ rng8, const int32_t *__restrict__ rstat,
                        const int32_t *__restrict__ d, uint32_t wc, uint32_t size, uint32_t kmem,
@@ -302,7 +307,7 @@ __global__ void k_seed(const int8_t *__restrict__ rng8, const int32_t *__restric
     __shared__ uint32_t s_min;
     __shared__ int32_t s_t1[256];
     const uint32_t K = size * size;
-    const uint32_t q = K / 128 > 1 ? K / 128 : 1, ncand = 256 / q, npos = ncand / 8;
+    const uint32_t ncand = size >= 32 ? 32 : 64, q = 256 / ncand, npos = ncand / 8;
     const uint32_t r = blockIdx.x, t = threadIdx.x, c = t % ncand, sl = t / ncand;
     if (t == 0) s_min = 0x7f800000u;
     s_t1[t] = 0;
@@ -1149,7 +1154,7 @@ void launch_sat(const int32_t *d, uint32_t wc, uint32_t hc, int64_t *sat1, int64
                 hipStream_t st)
 {
     hipLaunchKernelGGL(k_sat_rows, dim3(hc + 1), dim3(256), 0, st, d, wc, hc, sat1, sat2);
-    hipLaunchKernelGGL(k_sat_cols, dim3((wc + 1 + 255) / 256), dim3(256), 0, st, wc, hc, sat1,
+    hipLaunchKernelGGL(k_sat_cols, dim3((wc + 1 + 15) / 16), dim3(256), 0, st, wc, hc, sat1,
                        sat2);
 }
 
