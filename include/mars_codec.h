@@ -1,0 +1,104 @@
+/*
+ * mars_codec.h -- C ABI of the CPU side of the Mars encode path that sits
+ * above the search provider: the quadtree walks, RD mode decision, `.mars`
+ * v0 writer and rANS entropy coding.  These stay on the CPU (they are
+ * serial per image and cheap next to the search); they exist in this
+ * library so that a search table computed on the GPU produces the same
+ * `.mars` bytes as the reference's own CPU encode.
+ *
+ * Reference interfaces replaced:
+ *   mars_encode_plane      encode::encode_image_with_search
+ *                          (crates/mars-codec/src/encode.rs:1054-1126)
+ *                          followed by mars_format::write (mars_format.rs:232-297)
+ *   mars_search_fn         trait SearchProvider::search (encode.rs:557-579)
+ *   mars_wrap_gray         color::wrap_gray_stream (color.rs:382-384, 778-790)
+ *   mars_hip_encode_plane  encmars --method <hip> (encmars.rs:780-796): the
+ *                          provider is the GPU search table of a context.
+ *
+ * Status: 0 ok, < 0 error (MARS_HIP_E_* codes of mars_hip.h).
+ */
+#ifndef MARS_CODEC_H
+#define MARS_CODEC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "mars_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* encode.rs:27-60 EncodeParams (lambda: has_lambda = 0 is None). */
+typedef struct {
+    uint32_t min_size;
+    uint32_t max_size;
+    uint32_t shift;
+    uint32_t bits_alfa;
+    uint32_t bits_beta;
+    uint32_t zero_threshold;
+    double max_alfa;
+    double t_rms;
+    int32_t has_lambda;
+    int32_t _pad;
+    double lambda;
+} mars_encode_params;
+
+/* encode.rs:76-101 EncodeOptions (lambda_regions empty; rd_candidates 1). */
+typedef struct {
+    uint8_t allowed_modes[4];
+    uint8_t adaptive_density;
+    uint8_t lambda_adaptive_qstep; /* ResidualQuantisation::LambdaAdaptive */
+    uint8_t _pad[2];
+    double fixed_qstep;            /* ResidualQuantisation::Fixed (default 8.0) */
+    uint32_t rd_candidates;        /* must be 1 */
+    uint32_t _pad2;
+} mars_encode_options;
+
+/* encode.rs:1010-1021 EncodeCounters + 1494-1500 ModeStats. */
+typedef struct {
+    uint64_t search_evals;
+    uint64_t warmup_evals;
+    uint64_t leaf_modes[4];
+    uint64_t split_decisions;
+    uint64_t leaf_decisions;
+    uint64_t n_leaves;
+} mars_encode_stats;
+
+/*
+ * SearchProvider::search: the winner for range (row, col, size) at domain
+ * stride `shift`; *n_out = 0 when there is none.  Returns 0 or an error.
+ */
+typedef int (*mars_search_fn)(void *user, uint32_t row, uint32_t col, uint32_t size,
+                              uint32_t shift, mars_hip_candidate *out, uint32_t *n_out,
+                              uint64_t *evals);
+
+/*
+ * encode_image_with_search + mars_format::write.  `search` serves the final
+ * walk, `warmup` the RD rate warm-up (the reference hard-codes the
+ * exhaustive search there, encode.rs:1153; any provider returning the same
+ * winners keeps the bytes).  *out is malloc'd; free it with mars_free.
+ */
+int mars_encode_plane(const uint8_t *px, uint32_t w, uint32_t h, const mars_encode_params *p,
+                      const mars_encode_options *o, mars_search_fn search, void *search_user,
+                      mars_search_fn warmup, void *warmup_user, uint8_t **out, size_t *out_len,
+                      mars_encode_stats *stats);
+
+/* color::wrap_gray_stream: the MARC container around one grey plane stream. */
+int mars_wrap_gray(const uint8_t *plane, size_t len, uint8_t **out, size_t *out_len);
+
+void mars_free(void *p);
+
+/*
+ * The encode of mars_encode_plane with the GPU search table of `ctx` as the
+ * provider for both the warm-up and the final walk.  Prepares ctx for the
+ * plane (strides shift, and 2*shift when adaptive density is on).
+ */
+int mars_hip_encode_plane(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t h,
+                          const mars_encode_params *p, const mars_encode_options *o,
+                          uint8_t **out, size_t *out_len, mars_encode_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

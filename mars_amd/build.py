@@ -41,6 +41,14 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build failed ({proc.returncode}): {' '.join(map(str, cmd))}")
 
 
+CXX = os.environ.get("CXX", "g++")
+# The CPU encoder chain: binary64 must round like the reference's Rust
+# (no FMA contraction, no fast-math).
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+             f"-I{ROOT / 'include'}", f"-I{CSRC}", "-D__HIP_PLATFORM_AMD__",
+             "-I/opt/rocm/include"]
+
+
 def build_hip() -> Path:
     out = PKG / "libmars_hip.so"
     objs = []
@@ -48,6 +56,9 @@ def build_hip() -> Path:
         obj = CSRC / (Path(src).stem + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
+    obj = CSRC / "codec.o"
+    _run([CXX, *CXX_FLAGS, "-c", str(CSRC / "codec.cpp"), "-o", str(obj)])
+    objs.append(obj)
     tmp = out.with_suffix(".so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, out)

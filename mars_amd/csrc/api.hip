@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "mars_codec.h"
 #include "mars_hip.h"
 
 namespace {
@@ -626,6 +627,38 @@ int mars_hip_fit_f64(mars_hip_ctx *ctx, const int64_t *moments, uint32_t n, doub
     (void)hipFree(dqb);
     (void)hipFree(dr);
     return MARS_HIP_OK;
+}
+
+static int table_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size,
+                           uint32_t shift, mars_hip_candidate *out, uint32_t *n_out,
+                           uint64_t *evals)
+{
+    return mars_hip_search((const mars_hip_ctx *)user, row, col, size, shift, 1, out, n_out,
+                           evals);
+}
+
+int mars_hip_encode_plane(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t h,
+                          const mars_encode_params *p, const mars_encode_options *o,
+                          uint8_t **out, size_t *out_len, mars_encode_stats *stats)
+{
+    if (!ctx || !p || !o) return fail(ctx, MARS_HIP_E_ARG, "null argument");
+    mars_hip_params hp{};
+    hp.min_size = p->min_size;
+    hp.max_size = p->max_size;
+    hp.shift = p->shift;
+    hp.bits_alfa = p->bits_alfa;
+    hp.bits_beta = p->bits_beta;
+    hp.max_alfa = p->max_alfa;
+    // The walks request stride `shift`, and 2*shift for flat blocks when
+    // adaptive density is on (encode.rs:432-437, RD path only).
+    uint32_t shifts[2] = {p->shift, 2 * p->shift};
+    const uint32_t ns = (o->adaptive_density && p->has_lambda) ? 2 : 1;
+    int rc = mars_hip_prepare(ctx, px, w, h, &hp, shifts, ns);
+    if (rc) return rc;
+    rc = mars_encode_plane(px, w, h, p, o, table_search_cb, ctx, table_search_cb, ctx, out,
+                           out_len, stats);
+    if (rc && ctx->err.empty()) return fail(ctx, rc, "encode failed");
+    return rc;
 }
 
 } // extern "C"

@@ -91,6 +91,26 @@ uint64_t or_search_table(const uint8_t *px, uint32_t w, uint32_t h, uint32_t siz
                          uint32_t bits_beta, const uint32_t *rows, const uint32_t *cols,
                          uint32_t n, int threads, or_candidate *out, uint8_t *found);
 
+/*
+ * The exhaustive search as a search-provider callback (the reference's
+ * ExhaustiveSearch, encode.rs:582-602), for driving the encoder chain in
+ * tests.  `out` receives an or_candidate (the same layout as the product's
+ * candidate record).
+ */
+typedef struct {
+    const uint8_t *px;
+    uint32_t w, h;
+    int32_t *contracted;
+    double max_alfa;
+    uint32_t bits_alfa, bits_beta;
+} or_provider;
+
+or_provider *or_provider_new(const uint8_t *px, uint32_t w, uint32_t h, double max_alfa,
+                             uint32_t bits_alfa, uint32_t bits_beta);
+void or_provider_free(or_provider *p);
+int or_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
+                 void *out, uint32_t *n_out, uint64_t *evals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -323,3 +323,37 @@ uint64_t or_search_table(const uint8_t *px, uint32_t w, uint32_t h, uint32_t siz
     free(contracted);
     return evals;
 }
+
+or_provider *or_provider_new(const uint8_t *px, uint32_t w, uint32_t h, double max_alfa,
+                             uint32_t bits_alfa, uint32_t bits_beta)
+{
+    or_provider *p = (or_provider *)calloc(1, sizeof *p);
+    uint8_t *copy = (uint8_t *)malloc((size_t)w * h);
+    memcpy(copy, px, (size_t)w * h);
+    p->px = copy;
+    p->w = w;
+    p->h = h;
+    p->contracted = (int32_t *)malloc(sizeof(int32_t) * ((size_t)(w / 2) * (h / 2) + 1));
+    or_contract(copy, w, h, p->contracted);
+    p->max_alfa = max_alfa;
+    p->bits_alfa = bits_alfa;
+    p->bits_beta = bits_beta;
+    return p;
+}
+
+void or_provider_free(or_provider *p)
+{
+    if (!p) return;
+    free((void *)p->px);
+    free(p->contracted);
+    free(p);
+}
+
+int or_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
+                 void *out, uint32_t *n_out, uint64_t *evals)
+{
+    const or_provider *p = (const or_provider *)user;
+    or_search_top_k(p->px, p->w, p->h, p->contracted, row, col, size, shift, p->max_alfa,
+                    p->bits_alfa, p->bits_beta, 1, (or_candidate *)out, n_out, evals);
+    return 0;
+}

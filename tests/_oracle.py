@@ -64,6 +64,10 @@ def lib() -> C.CDLL:
     L.or_search_table.argtypes = [P(C.c_uint8), u32, u32, u32, u32, dbl, u32, u32, P(u32),
                                   P(u32), u32, C.c_int, P(OrCandidate), P(C.c_uint8)]
     L.or_search_table.restype = u64
+    L.or_provider_new.argtypes = [P(C.c_uint8), u32, u32, dbl, u32, u32]
+    L.or_provider_new.restype = C.c_void_p
+    L.or_provider_free.argtypes = [C.c_void_p]
+    L.or_provider_free.restype = None
     _lib = L
     return L
 
@@ -176,3 +180,27 @@ def nodes(w, h, min_size, max_size):
         vs *= 2
     visit(0, 0, vs)
     return out
+
+
+class Provider:
+    """The oracle's exhaustive search as a mars_search_fn callback + user pointer."""
+
+    def __init__(self, img, max_alfa=1.0, bits_alfa=4, bits_beta=7):
+        L = lib()
+        img = np.ascontiguousarray(img, np.uint8)
+        h, w = img.shape
+        self._L = L
+        self.user = C.c_void_p(L.or_provider_new(img.ctypes.data_as(C.POINTER(C.c_uint8)), w, h,
+                                                 max_alfa, bits_alfa, bits_beta))
+        self.fn = C.cast(L.or_search_cb, C.c_void_p)
+
+    def close(self):
+        if self.user:
+            self._L.or_provider_free(self.user)
+            self.user = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
