@@ -1,0 +1,101 @@
+"""Byte-identical `.mars` output: GPU search table vs the reference CPU search.
+
+The same host-side chain (walks, RD, format, rANS -- pinned to the
+reference's stream hashes in test_codec_pins.py) encodes each plane twice:
+once with the oracle's exhaustive search as provider, once with the HIP
+search table (mars_hip_encode_plane).  The bytes must be equal.  The oracle
+side precomputes its table with all host cores and serves the walk from it.
+"""
+
+import ctypes as C
+import hashlib
+
+import numpy as np
+import pytest
+
+from mars_amd import _native as N
+from mars_amd import synth
+from mars_amd.codec import SEARCH_FN, EncodeOptions, encode_plane, hip_encode_plane
+from mars_amd.provider import EncodeParams, HipContext
+from tests import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+class TableProvider:
+    """Oracle winners for every searchable node, served as a mars_search_fn."""
+
+    def __init__(self, img, params: EncodeParams, shifts):
+        h, w = img.shape
+        self.table = {}
+        nodes = O.nodes(w, h, params.min_size, params.max_size)
+        for shift in shifts:
+            for size, rc in nodes.items():
+                rows = [r for r, _ in rc]
+                cols = [c for _, c in rc]
+                res, _ = O.search_table(img, size, shift, rows, cols, threads=16,
+                                        max_alfa=params.max_alfa, bits_alfa=params.bits_alfa,
+                                        bits_beta=params.bits_beta)
+                P = 0
+                if h >= 2 * size and w >= 2 * size:
+                    P = ((h - 2 * size) // shift + 1) * ((w - 2 * size) // shift + 1)
+                for r, c, cand in zip(rows, cols, res):
+                    self.table[(r, c, size, shift)] = (cand, P * 8)
+        self.calls = 0
+
+        def cb(user, row, col, size, shift, out, n_out, evals):
+            self.calls += 1
+            cand, ev = self.table[(row, col, size, shift)]
+            evals[0] = ev
+            if cand is None:
+                n_out[0] = 0
+            else:
+                C.memmove(out, C.byref(cand), C.sizeof(N.CCandidate))
+                n_out[0] = 1
+            return 0
+
+        self._cb = SEARCH_FN(cb)
+        self.fn = C.cast(self._cb, C.c_void_p)
+
+
+def _both(img, params, options, shifts):
+    prov = TableProvider(img, params, shifts)
+    ref, ref_st = encode_plane(img, params, options, prov.fn, None)
+    ctx = HipContext(0)
+    got, st = hip_encode_plane(ctx, img, params, options)
+    return ref, ref_st, got, st
+
+
+def test_fixed_grid_256_threshold_walk():
+    """256x256 seed-1 plane, fixed 8x8 ranges, stride 8, threshold walk."""
+    img = synth.plane(256, 256, synth.WORKLOAD_SEEDS["fixed256"])
+    p = EncodeParams(min_size=8, max_size=8, shift=8, t_rms=8.0, lambda_=None)
+    ref, ref_st, got, st = _both(img, p, EncodeOptions(), [8])
+    assert st.search_evals == ref_st.search_evals == 1024 * 961 * 8
+    print(f"fixed256: {len(got)} bytes sha256 {hashlib.sha256(got).hexdigest()}", flush=True)
+    assert got == ref
+
+
+def test_quadtree_512_rd_lambda200():
+    """512x512 seed-2 plane, quadtree 32->4, stride 4, RD at lambda 200, modes {0,2}."""
+    img = synth.plane(512, 512, synth.WORKLOAD_SEEDS["quadtree512"])
+    p = EncodeParams(min_size=4, max_size=32, shift=4, lambda_=200.0)
+    o = EncodeOptions(allowed_modes=(True, False, True, False))
+    ref, ref_st, got, st = _both(img, p, o, [4])
+    assert (st.search_evals, st.warmup_evals) == (ref_st.search_evals, ref_st.warmup_evals)
+    print(f"quadtree512: {len(got)} bytes sha256 {hashlib.sha256(got).hexdigest()} "
+          f"leaves {st.n_leaves} modes {st.leaf_modes}", flush=True)
+    assert got == ref
+
+
+@pytest.mark.parametrize("lam,modes,density", [
+    (50.0, (True, True, True, True), False),
+    (800.0, (True, False, True, False), True),
+])
+def test_rd_variants_160x96(lam, modes, density):
+    img = synth.plane(160, 96, 77)
+    p = EncodeParams(min_size=4, max_size=16, shift=4, lambda_=lam)
+    o = EncodeOptions(allowed_modes=modes, adaptive_density=density,
+                      lambda_adaptive_qstep=density)
+    ref, _, got, _ = _both(img, p, o, [4, 8] if density else [4])
+    assert got == ref
