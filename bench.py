@@ -99,13 +99,51 @@ def _cpu_leg(img, budget_s: float = 12.0):
                        f"oracle/liboracle.so over {threads} threads")
 
 
+def _traffic_probe() -> int:
+    """One eager step of the MFMA search and one 256 MiB copy kernel: the
+    command rocprofv3 --pmc runs to read FETCH_SIZE / WRITE_SIZE."""
+    from mars_amd import _native as N
+    from mars_amd import synth
+    from mars_amd.provider import EncodeParams, HipContext
+
+    ctx = HipContext(0, N.KERNEL_MFMA_I8)
+    ctx.prepare(synth.plane(W, H, SEED), EncodeParams(min_size=MIN_SIZE, max_size=MAX_SIZE,
+                                                     shift=SHIFT), [SHIFT])
+    ctx.run(iters=1, use_graph=False)
+    ms = ctx.copy_probe(COPY_PROBE_BYTES)
+    print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copy_probe_ms": ms}), flush=True)
+    return 0
+
+
+COPY_PROBE_BYTES = 256 << 20
+
+
+def _profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel's reading), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "quadtree512_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(r"^\|\s*" + re.escape(kernel) + r"\s*\|.*\|\s*([0-9.eE+]+)\s*\|\s*$")
+    for line in open(path):
+        m = pat.match(line.strip())
+        if m:
+            return float(m.group(1))
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="one eager MFMA step + a 256 MiB copy kernel (for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        return _traffic_probe()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share of the search; --gpus {args.gpus} is not "
               "supported (the search shards by range block with no exchange but a per-range "
@@ -155,7 +193,8 @@ def main() -> int:
         "exact_fits_by_size": mf["exact_fits_by_size"],
         "roofline": {"bound": "mfma", "kernel": f"k_search_mfma<{s}>", "achieved": achieved / 1e12,
                      "peak": PEAK_INT8_OPS / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_INT8_OPS,
-                     "traffic": None, "op": "int8 ops (2*s^2 per eval)",
+                     "traffic": _profiled_traffic(f"k_search_mfma<{s}>"),
+                     "op": "int8 ops (2*s^2 per eval)",
                      "avg_launch_ms": dom_ms},
         "cpu_baseline": cpu,
     }

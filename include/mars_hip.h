@@ -134,6 +134,13 @@ int mars_hip_fit_f64(mars_hip_ctx *ctx, const int64_t *moments, uint32_t n, doub
                      uint32_t bits_alfa, uint32_t bits_beta, uint32_t *qalfa, uint32_t *qbeta,
                      double *rms);
 
+/*
+ * Run one plain device-to-device copy kernel (k_copy_probe) over `bytes`
+ * (a multiple of 16): the known-traffic reference against which hardware
+ * HBM counters of the search kernels are scaled.  *ms = its duration.
+ */
+int mars_hip_copy_probe(mars_hip_ctx *ctx, size_t bytes, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,7 @@ KERNEL_MFMA_I8, KERNEL_FP32 = 0, 1
 EXPORTS = (
     "mars_hip_open", "mars_hip_close", "mars_hip_prepare", "mars_hip_search",
     "mars_hip_last_error", "mars_hip_set_kernel", "mars_hip_run", "mars_hip_table_count",
-    "mars_hip_table_info", "mars_hip_table_read", "mars_hip_fit_f64",
+    "mars_hip_table_info", "mars_hip_table_read", "mars_hip_fit_f64", "mars_hip_copy_probe",
 )
 
 
@@ -71,6 +71,7 @@ def lib() -> C.CDLL:
                                       P(C.c_uint8)]
     L.mars_hip_fit_f64.argtypes = [C.c_void_p, P(C.c_int64), u32, C.c_double, u32, u32, P(u32),
                                    P(u32), P(C.c_double)]
+    L.mars_hip_copy_probe.argtypes = [C.c_void_p, C.c_size_t, P(C.c_double)]
     for name in EXPORTS:
         if name not in ("mars_hip_close", "mars_hip_last_error"):
             getattr(L, name).restype = C.c_int

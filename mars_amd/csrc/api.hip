@@ -629,6 +629,31 @@ int mars_hip_fit_f64(mars_hip_ctx *ctx, const int64_t *moments, uint32_t n, doub
     return MARS_HIP_OK;
 }
 
+int mars_hip_copy_probe(mars_hip_ctx *ctx, size_t bytes, double *ms)
+{
+    if (!ctx || bytes == 0 || bytes % 16) return fail(ctx, MARS_HIP_E_ARG, "bytes must be a positive multiple of 16");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    void *src = nullptr, *dst = nullptr;
+    HIP_TRY(ctx, hipMalloc(&src, bytes));
+    HIP_TRY(ctx, hipMalloc(&dst, bytes));
+    HIP_TRY(ctx, hipMemsetAsync(src, 1, bytes, ctx->stream));
+    hipEvent_t e0, e1;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    HIP_TRY(ctx, hipEventCreate(&e1));
+    HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
+    mars::launch_copy_probe(src, dst, bytes, ctx->stream);
+    HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float t = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return MARS_HIP_OK;
+}
+
 static int table_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size,
                            uint32_t shift, mars_hip_candidate *out, uint32_t *n_out,
                            uint64_t *evals)

@@ -78,6 +78,7 @@ void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const ui
                    uint32_t nr, uint32_t nr_pad, uint32_t size, uint32_t kmem, int8_t *rng8,
                    float *rngf, int32_t *rstat, uint32_t *gthr, hipStream_t st);
 void launch_merge(const MergeArgs &a, hipStream_t st);
+void launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t st);
 void launch_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
                          uint32_t bits_beta, uint32_t *qa, uint32_t *qb, double *rms,
                          hipStream_t st);

@@ -1094,6 +1094,21 @@ __global__ void k_merge(MergeArgs a)
     write_winner(a, r, k, (int64_t)s_acc);
 }
 
+// Plain copy of n 16-byte words: the known-traffic reference for the HBM
+// counters (it reads and writes exactly 16*n bytes).
+__global__ void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, st, (const uint4 *)src,
+                       (uint4 *)dst, bytes / 16);
+}
+
 __global__ void k_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
                                uint32_t bits_beta, uint32_t *qa, uint32_t *qb, double *rms)
 {

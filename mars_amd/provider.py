@@ -168,6 +168,12 @@ class HipContext:
             rms.ctypes.data_as(C.POINTER(C.c_double))), "mars_hip_fit_f64")
         return qa, qb, rms
 
+    def copy_probe(self, nbytes: int) -> float:
+        ms = C.c_double()
+        self._check(self._L.mars_hip_copy_probe(self._ctx, nbytes, C.byref(ms)),
+                    "mars_hip_copy_probe")
+        return ms.value
+
     def close(self) -> None:
         if self._ctx:
             self._L.mars_hip_close(self._ctx)

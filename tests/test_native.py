@@ -25,6 +25,9 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in declared if not hasattr(lib, s)]
     assert not missing, missing
     assert set(declared) == set(N.EXPORTS)
+    codec = [s for s in re.findall(r"\b(mars_[a-z0-9_]+)\s*\(",
+                                   (ROOT / "include" / "mars_codec.h").read_text())]
+    assert codec and all(hasattr(lib, s) for s in codec), codec
 
 
 def test_null_arguments_are_rejected_without_a_device():
