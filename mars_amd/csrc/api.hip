@@ -54,9 +54,15 @@ struct Table {
 
 } // namespace
 
+constexpr int kSideStreams = 4;
+
 struct mars_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Per-size search chains run on side streams forked from `stream` after
+    // the pool build (all at the default priority).
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
     mutable std::mutex err_mu;
     std::string err;
     int kernel = MARS_HIP_KERNEL_MFMA_I8;
@@ -257,8 +263,24 @@ MergeArgs merge_args(const mars_hip_ctx *ctx, const Table &t, int kernel)
     return m;
 }
 
-// Enqueue one whole step.  ev (optional, 3 + n_tables events): before pool,
-// after pool/range prep, after each table's search, after merge.
+void enqueue_table(mars_hip_ctx *ctx, Table &t, int kernel, hipStream_t st)
+{
+    mars::launch_possums(ctx->d_sat1, ctx->d_sat2, ctx->wc, t.size, t.shift, t.npc, t.P,
+                         t.d_posinfo, t.d_posd, st);
+    mars::launch_ranges(ctx->d_px, ctx->w, t.d_rows, t.d_cols, t.nr, t.nr_pad, t.size, t.kmem,
+                        t.d_rng8, kernel == MARS_HIP_KERNEL_FP32 ? t.d_rngf : nullptr, t.d_rstat,
+                        t.d_gthr, st);
+    if (kernel == MARS_HIP_KERNEL_MFMA_I8)
+        mars::launch_seed(t.d_rng8, t.d_rstat, ctx->d_d, ctx->wc, t.size, t.kmem, t.npc, t.P,
+                          t.d_posinfo, t.nr, ctx->params.max_alfa, ctx->params.bits_alfa,
+                          ctx->params.bits_beta, t.d_gthr, st);
+}
+
+// Enqueue one whole step.  With ev (3 + n_tables events: before the pool,
+// after pool/range prep, after each table's search, after the merges) the
+// step runs serially on the context stream so each launch can be timed;
+// without, each table's chain (prep, search, merge) runs on its own side
+// stream forked after the pool build and joined at the end.
 int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
 {
     hipStream_t st = ctx->stream;
@@ -267,19 +289,30 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
                           ctx->d_d, ctx->d_df, ctx->d_lo, ctx->d_hi, ctx->d_counters, kCounters,
                           st);
     mars::launch_sat(ctx->d_d, ctx->wc, ctx->hc, ctx->d_sat1, ctx->d_sat2, st);
-    for (Table &t : ctx->tables) {
-        mars::launch_possums(ctx->d_sat1, ctx->d_sat2, ctx->wc, t.size, t.shift, t.npc, t.P,
-                             t.d_posinfo, t.d_posd, st);
-        mars::launch_ranges(ctx->d_px, ctx->w, t.d_rows, t.d_cols, t.nr, t.nr_pad, t.size,
-                            t.kmem, t.d_rng8,
-                            kernel == MARS_HIP_KERNEL_FP32 ? t.d_rngf : nullptr, t.d_rstat,
-                            t.d_gthr, st);
-        if (kernel == MARS_HIP_KERNEL_MFMA_I8)
-            mars::launch_seed(t.d_rng8, t.d_rstat, ctx->d_d, ctx->wc, t.size, t.kmem, t.npc, t.P,
-                              t.d_posinfo, t.nr, ctx->params.max_alfa, ctx->params.bits_alfa,
-                              ctx->params.bits_beta, t.d_gthr, st);
+    if (!ev) {
+        HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, st));
+        const int nside = std::min<int>(kSideStreams, (int)ctx->tables.size());
+        for (int j = 0; j < nside; j++) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[j], ctx->fork_ev, 0));
+        for (size_t i = 0; i < ctx->tables.size(); i++) {
+            Table &t = ctx->tables[i];
+            hipStream_t ss = ctx->side[i % kSideStreams];
+            enqueue_table(ctx, t, kernel, ss);
+            if (t.P > 0 && t.nr > 0) {
+                SearchArgs a = search_args(ctx, t, kernel);
+                if (mars::launch_search(kernel, t.size, a, ss) != 0)
+                    return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
+            }
+            mars::launch_merge(merge_args(ctx, t, kernel), ss);
+        }
+        for (int j = 0; j < nside; j++) {
+            HIP_TRY(ctx, hipEventRecord(ctx->join_ev[j], ctx->side[j]));
+            HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join_ev[j], 0));
+        }
+        HIP_TRY(ctx, hipGetLastError());
+        return MARS_HIP_OK;
     }
-    if (ev) HIP_TRY(ctx, hipEventRecord(ev[1], st));
+    for (Table &t : ctx->tables) enqueue_table(ctx, t, kernel, st);
+    HIP_TRY(ctx, hipEventRecord(ev[1], st));
     for (size_t i = 0; i < ctx->tables.size(); i++) {
         Table &t = ctx->tables[i];
         if (t.P > 0 && t.nr > 0) {
@@ -287,10 +320,10 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
             if (mars::launch_search(kernel, t.size, a, st) != 0)
                 return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
         }
-        if (ev) HIP_TRY(ctx, hipEventRecord(ev[2 + i], st));
+        HIP_TRY(ctx, hipEventRecord(ev[2 + i], st));
     }
     for (Table &t : ctx->tables) mars::launch_merge(merge_args(ctx, t, kernel), st);
-    if (ev) HIP_TRY(ctx, hipEventRecord(ev[2 + ctx->tables.size()], st));
+    HIP_TRY(ctx, hipEventRecord(ev[2 + ctx->tables.size()], st));
     HIP_TRY(ctx, hipGetLastError());
     return MARS_HIP_OK;
 }
@@ -325,10 +358,15 @@ int mars_hip_open(int device, mars_hip_ctx **out)
         return MARS_HIP_E_DEVICE;
     mars_hip_ctx *ctx = new mars_hip_ctx();
     ctx->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
-        hipMalloc((void **)&ctx->d_counters, sizeof(unsigned long long) * kCounters) !=
-            hipSuccess) {
-        delete ctx;
+    bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreate(&ctx->stream) == hipSuccess &&
+              hipMalloc((void **)&ctx->d_counters, sizeof(unsigned long long) * kCounters) ==
+                  hipSuccess &&
+              hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess;
+    for (int j = 0; ok && j < kSideStreams; j++)
+        ok = hipStreamCreate(&ctx->side[j]) == hipSuccess &&
+             hipEventCreateWithFlags(&ctx->join_ev[j], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        mars_hip_close(ctx);
         return MARS_HIP_E_DEVICE;
     }
     *out = ctx;
@@ -340,8 +378,15 @@ void mars_hip_close(mars_hip_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (int j = 0; j < kSideStreams; j++)
+        if (ctx->side[j]) (void)hipStreamSynchronize(ctx->side[j]);
     free_tables(ctx);
     (void)hipFree(ctx->d_counters);
+    for (int j = 0; j < kSideStreams; j++) {
+        if (ctx->side[j]) (void)hipStreamDestroy(ctx->side[j]);
+        if (ctx->join_ev[j]) (void)hipEventDestroy(ctx->join_ev[j]);
+    }
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -541,7 +541,12 @@ __device__ __forceinline__ bool stage2_keep(const SearchArgs &a, double nd, doub
     return boundary || sse <= (double)thr;
 }
 
-constexpr int QCAP = 64 + 1024; // survivor queue entries per wave (one row tile adds <= 1024)
+// Survivor queue: the wave drains full batches of 64 after every DRAIN
+// values (one value per lane each), so DRAIN*64 + 63 entries always fit.
+template <int S> struct Queue {
+    static constexpr int DRAIN = S <= 8 ? 16 : 4;
+    static constexpr int CAP = 64 + 64 * DRAIN;
+};
 
 // Certified split-INT8 search for one range size S.
 //
@@ -600,7 +605,11 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
     __shared__ int64_t s_T0[8], s_T2[8];
     __shared__ double s_A[8];
     __shared__ BestRec s_wb[4][8];
-    __shared__ uint2 s_q[4][QCAP];
+    __shared__ uint2 s_q[4][Queue<S>::CAP];
+    // Range operand of the block (sizes >= 16): 64 rows of KMEM bytes,
+    // padded rows so the lanes of a fragment read spread over the banks.
+    constexpr int ASTRIDE = G::KMEM + 16;
+    __shared__ __align__(16) int8_t s_rng[G::PRELOAD_A ? 16 : 64 * ASTRIDE];
 
     if (threadIdx.x < 2) s_count[threadIdx.x] = 0;
     if (threadIdx.x < 8) {
@@ -643,8 +652,19 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
         for (int tt = 0; tt < 2; tt++)
 #pragma unroll
             for (int kk = 0; kk < NK; kk++) load_a<S>(a.rng8, g, tt, kk, lane, &a8[tt][kk], &a16[tt][kk]);
+    } else {
+        // The group's 8 ranges x 8 isometries are 64 contiguous rows of rng8.
+        const int8_t *src = a.rng8 + (size_t)g * 64 * G::KMEM;
+        constexpr int CPR = G::KMEM / 16; // 16-byte chunks per row
+        for (int c = threadIdx.x; c < 64 * CPR; c += 256) {
+            const int row = c / CPR, col = c - row * CPR;
+            *(v4i *)(s_rng + row * ASTRIDE + col * 16) = *(const v4i *)(src + (size_t)row * G::KMEM + col * 16);
+        }
     }
     __syncthreads();
+    // Row of the LDS range operand feeding lane `lane` in row tile tt.
+    const int am = lane & 31;
+    const int arow0 = (((am >> 2) & 1) * 4 + (am & 3)) * 8 + (am >> 3);
 
     const uint32_t ntiles = (a.P + 31) / 32;
     const uint32_t t_begin = chunk * a.tiles_per_chunk;
@@ -723,6 +743,18 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
     uint4 pin_ahead = make_uint4(0, 0, 0, 0);
     if (t_begin + wave < t_end) fetch_b(load_pin(t_begin + wave), nxt);
     if (t_begin + wave + 4 < t_end) pin_ahead = load_pin(t_begin + wave + 4);
+    // Sizes >= 16: window fragments in flight PD K-steps ahead, across tiles.
+    constexpr int PD = NK < 8 ? NK : 8;
+    v4i pbl[PD], pbh[PD];
+    if constexpr (!G::PRELOAD_A) {
+        const uint32_t dr0 = nxt.pin.x & 0xffff, dc0 = nxt.pin.x >> 16;
+        long dummy8 = 0;
+#pragma unroll
+        for (int d = 0; d < PD; d++) {
+            load_b<S>(a.lo, a.copy_stride, a.pitch, dr0, dc0, d, h, &dummy8, &pbl[d]);
+            load_b<S>(a.hi, a.copy_stride, a.pitch, dr0, dc0, d, h, &dummy8, &pbh[d]);
+        }
+    }
     const bool exchange = a.nchunks > 1;
 
     for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4, tcount++) {
@@ -762,18 +794,26 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
                 }
             }
         } else {
-#pragma unroll 2
+            const bool has_next = tile + 4 < t_end;
+            const uint32_t ndr = nxt.pin.x & 0xffff, ndc = nxt.pin.x >> 16;
+#pragma unroll
             for (int kk = 0; kk < NK; kk++) {
+                const int d = kk % PD;
+                const v4i a0 = *(const v4i *)(s_rng + arow0 * ASTRIDE + kk * 32 + h * 16);
+                const v4i a1 = *(const v4i *)(s_rng + (arow0 + 4) * ASTRIDE + kk * 32 + h * 16);
+                acc_lo[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbl[d], acc_lo[0], 0, 0, 0);
+                acc_hi[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbh[d], acc_hi[0], 0, 0, 0);
+                acc_lo[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbl[d], acc_lo[1], 0, 0, 0);
+                acc_hi[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbh[d], acc_hi[1], 0, 0, 0);
+                // Refill the slot with the fragment PD steps ahead.
                 long dummy8 = 0;
-                v4i bl16 = {}, bh16 = {}, a0 = {}, a1 = {};
-                load_b<S>(a.lo, a.copy_stride, a.pitch, dr, dc, kk, h, &dummy8, &bl16);
-                load_b<S>(a.hi, a.copy_stride, a.pitch, dr, dc, kk, h, &dummy8, &bh16);
-                load_a<S>(a.rng8, g, 0, kk, lane, &dummy8, &a0);
-                load_a<S>(a.rng8, g, 1, kk, lane, &dummy8, &a1);
-                acc_lo[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bl16, acc_lo[0], 0, 0, 0);
-                acc_hi[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, bh16, acc_hi[0], 0, 0, 0);
-                acc_lo[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bl16, acc_lo[1], 0, 0, 0);
-                acc_hi[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, bh16, acc_hi[1], 0, 0, 0);
+                if (kk + PD < NK) {
+                    load_b<S>(a.lo, a.copy_stride, a.pitch, dr, dc, kk + PD, h, &dummy8, &pbl[d]);
+                    load_b<S>(a.hi, a.copy_stride, a.pitch, dr, dc, kk + PD, h, &dummy8, &pbh[d]);
+                } else if (has_next) {
+                    load_b<S>(a.lo, a.copy_stride, a.pitch, ndr, ndc, kk + PD - NK, h, &dummy8, &pbl[d]);
+                    load_b<S>(a.hi, a.copy_stride, a.pitch, ndr, ndc, kk + PD - NK, h, &dummy8, &pbh[d]);
+                }
             }
         }
 
@@ -810,11 +850,13 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
 
         // One compare per value; appends happen only for values that
         // survive in some lane (a wave-uniform branch).  The queue is
-        // drained after each 32-row tile, so one tile adds at most 1024.
+        // drained after every DRAIN values.
+        constexpr int DRAIN = Queue<S>::DRAIN;
 #pragma unroll
-        for (int tt = 0; tt < 2; tt++) {
+        for (int grp = 0; grp < 32 / DRAIN; grp++) {
 #pragma unroll
-            for (int t = 0; t < 16; t++) {
+            for (int e = 0; e < DRAIN; e++) {
+                const int b = grp * DRAIN + e, tt = b >> 4, t = b & 15;
                 const int32_t v = (acc_hi[tt][t] << 8) + acc_lo[tt][t];
                 const int i = t & 3;
                 const bool sv = v > lim[i];
