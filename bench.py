@@ -71,6 +71,40 @@ def _gpu_leg(steps: int, warmup: int):
     return img, res
 
 
+def _batch_leg(name: str, specs, steps: int):
+    """One batched step over all planes (one search launch per size over every
+    plane, one HIP graph) against one step per image (each image its own
+    context and graph, the per-image step times summed)."""
+    from mars_amd import synth
+    from mars_amd.provider import EncodeParams, HipContext
+
+    imgs = [synth.plane(w, h, seed) for w, h, seed, _ in specs]
+    params = EncodeParams(min_size=MIN_SIZE, max_size=MAX_SIZE, shift=SHIFT)
+    ctx = HipContext(0)
+    ctx.prepare_batch(imgs, params, [SHIFT])
+    tune = ctx.autotune(reps=1)
+    ctx.run(iters=1, use_graph=True)
+    t = ctx.run(iters=steps, use_graph=True)
+    ctx.close()
+    per_ms = 0.0
+    for img in imgs:
+        c = HipContext(0)
+        c.prepare(img, params, [SHIFT])
+        per_ms += c.run(iters=steps, use_graph=True).ms_step
+        c.close()
+    sizes = {}
+    for w, h, _, _ in specs:
+        sizes[f"{w}x{h}"] = sizes.get(f"{w}x{h}", 0) + 1
+    return {"workload": name, "images": len(imgs), "sizes": sizes, "evals_per_batch": t.evals,
+            "ms_per_batch": t.ms_step, "images_per_s": len(imgs) * 1e3 / t.ms_step,
+            "evals_per_s": t.evals / (t.ms_step * 1e-3),
+            "one_step_per_image": {"ms_per_batch": per_ms,
+                                   "images_per_s": len(imgs) * 1e3 / per_ms},
+            "speedup_vs_one_step_per_image": per_ms / t.ms_step,
+            "autotuned_shapes": {str(2 << i): [int(tune.target_blocks[i]), int(tune.min_tiles[i])]
+                                 for i in range(6) if tune.ms[i] > 0}}
+
+
 def _cpu_leg(img, budget_s: float = 12.0):
     """The oracle (C restatement of the reference CPU search) on the host cores,
     on a bounded sample of the same workload: ranges drawn from every size."""
@@ -139,6 +173,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-image legs")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one eager MFMA step + a 256 MiB copy kernel (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -164,6 +199,15 @@ def main() -> int:
     diverge = sum(1 for a, b in zip(sum(mf["winners"], []), sum(fp["winners"], [])) if a != b)
     n_ranges = sum(len(t) for t in mf["winners"])
 
+    batches = None
+    if not args.no_batch:
+        from mars_amd import synth
+        b512 = [(W, H, 200 + i, 200.0) for i in range(64)]
+        batches = {"batch512": _batch_leg("64 x 512x512 planes (seeds 200..263), quadtree 32->4, "
+                                          "stride 4", b512, 3),
+                   "batch64": _batch_leg("64 planes of uneven sizes (mars_amd.synth.batch_specs, "
+                                         "seeds 100..163), quadtree 32->4, stride 4",
+                                         synth.batch_specs(), 3)}
     cpu = None if args.no_cpu else _cpu_leg(img)
     line = {
         "metric": "R×D×iso evals/s, split-INT8 MFMA vs FP32 kernel; batched img/s @512²",
@@ -182,6 +226,8 @@ def main() -> int:
                                "8 isometries, exhaustive, whole RD node set per step",
                    "evals_per_step": evals, "ranges": n_ranges},
         "images_per_s": 1e3 / mf["ms_step"],
+        "batched_images_per_s_512": batches["batch512"]["images_per_s"] if batches else None,
+        "batches": batches,
         "fp32_kernel": {"value": fp_value, "unit": "evals/s", "ms_per_step": fp["ms_step"],
                         "ranges_differing_from_mfma_winner": diverge},
         "speedup_vs_fp32": value / fp_value,

@@ -14,6 +14,7 @@
  *   mars_wrap_gray         color::wrap_gray_stream (color.rs:382-384, 778-790)
  *   mars_hip_encode_plane  encmars --method <hip> (encmars.rs:780-796): the
  *                          provider is the GPU search table of a context.
+ *   mars_hip_encode_prepared  the same for one plane of a prepared batch.
  *
  * Status: 0 ok, < 0 error (MARS_HIP_E_* codes of mars_hip.h).
  */
@@ -97,6 +98,17 @@ void mars_free(void *p);
 int mars_hip_encode_plane(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t h,
                           const mars_encode_params *p, const mars_encode_options *o,
                           uint8_t **out, size_t *out_len, mars_encode_stats *stats);
+
+/*
+ * Encode plane `plane` of the batch prepared in ctx (mars_hip_prepare_batch)
+ * from its search table: the walks of mars_encode_plane served by
+ * mars_hip_search_plane.  The search fields of *p must equal the prepared
+ * params, and the strides the walk requests must have been prepared.
+ * Read-only on the context: several planes may be encoded concurrently.
+ */
+int mars_hip_encode_prepared(mars_hip_ctx *ctx, uint32_t plane, const mars_encode_params *p,
+                             const mars_encode_options *o, uint8_t **out, size_t *out_len,
+                             mars_encode_stats *stats);
 
 #ifdef __cplusplus
 }

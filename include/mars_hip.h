@@ -75,6 +75,20 @@ typedef struct {
     uint64_t exact_fits_by_size[6]; /* candidates that took the verbatim binary64 fit */
 } mars_hip_timing;
 
+/* One plane of a batch: w x h u8 pixels, row-major. */
+typedef struct {
+    const uint8_t *px;
+    uint32_t w, h;
+} mars_hip_plane;
+
+/* Search chunking shapes chosen by mars_hip_autotune, per range size
+ * 2, 4, 8, 16, 32, 64 (index log2(size) - 1). */
+typedef struct {
+    uint32_t target_blocks[6]; /* blocks per search launch the chunking aims at */
+    uint32_t min_tiles[6];     /* least position tiles per block */
+    double ms[6];              /* search time per size with the chosen shape */
+} mars_hip_tuning;
+
 /* Open a context on HIP device `device` (0-based).  Creates its own stream. */
 int mars_hip_open(int device, mars_hip_ctx **out);
 void mars_hip_close(mars_hip_ctx *ctx);
@@ -92,6 +106,18 @@ int mars_hip_prepare(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t 
                      const mars_hip_params *p, const uint32_t *shifts, uint32_t n_shifts);
 
 /*
+ * The factory body for a batch of planes (the same params and strides for
+ * all): one pool per plane and one search table per (plane, size, stride),
+ * built and searched by one step whose search launches each cover one range
+ * size over every plane of the batch.  encode_image_with_search calls its
+ * factory once per plane (encode.rs:1054-1076); a batch front-loads those
+ * calls for many images.  Plane i is looked up with
+ * mars_hip_search_plane(ctx, i, ...).  Copies its inputs.
+ */
+int mars_hip_prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *planes, uint32_t n_planes,
+                           const mars_hip_params *p, const uint32_t *shifts, uint32_t n_shifts);
+
+/*
  * SearchProvider::search (encode.rs:557-602): the winner for the range block
  * (row, col, size) at domain stride `shift`.  k must be 1 (the default
  * search_candidates of encode.rs:572-578 returns the winner only).  *n_out is
@@ -101,6 +127,12 @@ int mars_hip_prepare(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t 
 int mars_hip_search(const mars_hip_ctx *ctx, uint32_t row, uint32_t col, uint32_t size,
                     uint32_t shift, uint32_t k, mars_hip_candidate *out, uint32_t *n_out,
                     uint64_t *evals);
+
+/* SearchProvider::search of plane `plane` of a prepared batch (plane 0 is
+ * what mars_hip_search serves). */
+int mars_hip_search_plane(const mars_hip_ctx *ctx, uint32_t plane, uint32_t row, uint32_t col,
+                          uint32_t size, uint32_t shift, uint32_t k, mars_hip_candidate *out,
+                          uint32_t *n_out, uint64_t *evals);
 
 /* Last error message of this context ("" if none). */
 const char *mars_hip_last_error(const mars_hip_ctx *ctx);
@@ -117,8 +149,18 @@ int mars_hip_set_kernel(mars_hip_ctx *ctx, int kernel);
  */
 int mars_hip_run(mars_hip_ctx *ctx, int iters, int use_graph, mars_hip_timing *timing);
 
-/* Number of prepared (size, shift) tables and the nodes of table i. */
+/*
+ * Pick the search chunking shape per range size by timing the prepared step
+ * over a fixed space of shapes (reps timed steps each, median); results do
+ * not depend on the shape.  Keeps the best shapes for later runs, refreshes
+ * the lookup table and reports the choice in *out (may be null).
+ */
+int mars_hip_autotune(mars_hip_ctx *ctx, uint32_t reps, mars_hip_tuning *out);
+
+/* Number of prepared (plane, size, shift) tables and the nodes of table i. */
 int mars_hip_table_count(const mars_hip_ctx *ctx, uint32_t *n_tables);
+/* The batch plane table i belongs to. */
+int mars_hip_table_plane(const mars_hip_ctx *ctx, uint32_t i, uint32_t *plane);
 int mars_hip_table_info(const mars_hip_ctx *ctx, uint32_t i, uint32_t *size, uint32_t *shift,
                         uint32_t *n_ranges, uint32_t *n_positions);
 /* Copy table i's winners (n_ranges entries, found[j] = 0 for no candidate). */

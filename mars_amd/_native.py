@@ -16,8 +16,9 @@ KERNEL_MFMA_I8, KERNEL_FP32 = 0, 1
 
 # Every symbol include/mars_hip.h declares.
 EXPORTS = (
-    "mars_hip_open", "mars_hip_close", "mars_hip_prepare", "mars_hip_search",
-    "mars_hip_last_error", "mars_hip_set_kernel", "mars_hip_run", "mars_hip_table_count",
+    "mars_hip_open", "mars_hip_close", "mars_hip_prepare", "mars_hip_prepare_batch",
+    "mars_hip_search", "mars_hip_search_plane", "mars_hip_last_error", "mars_hip_set_kernel",
+    "mars_hip_run", "mars_hip_autotune", "mars_hip_table_count", "mars_hip_table_plane",
     "mars_hip_table_info", "mars_hip_table_read", "mars_hip_fit_f64", "mars_hip_copy_probe",
 )
 
@@ -43,6 +44,15 @@ class Timing(C.Structure):
                 ("exact_fits_by_size", C.c_uint64 * 6)]
 
 
+class Plane(C.Structure):
+    _fields_ = [("px", C.POINTER(C.c_uint8)), ("w", C.c_uint32), ("h", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("target_blocks", C.c_uint32 * 6), ("min_tiles", C.c_uint32 * 6),
+                ("ms", C.c_double * 6)]
+
+
 _lib = None
 
 
@@ -59,8 +69,13 @@ def lib() -> C.CDLL:
     L.mars_hip_close.argtypes = [C.c_void_p]
     L.mars_hip_close.restype = None
     L.mars_hip_prepare.argtypes = [C.c_void_p, P(C.c_uint8), u32, u32, P(Params), P(u32), u32]
+    L.mars_hip_prepare_batch.argtypes = [C.c_void_p, P(Plane), u32, P(Params), P(u32), u32]
     L.mars_hip_search.argtypes = [C.c_void_p, u32, u32, u32, u32, u32, P(CCandidate), P(u32),
                                   P(u64)]
+    L.mars_hip_search_plane.argtypes = [C.c_void_p, u32, u32, u32, u32, u32, u32, P(CCandidate),
+                                        P(u32), P(u64)]
+    L.mars_hip_autotune.argtypes = [C.c_void_p, u32, P(Tuning)]
+    L.mars_hip_table_plane.argtypes = [C.c_void_p, u32, P(u32)]
     L.mars_hip_last_error.argtypes = [C.c_void_p]
     L.mars_hip_last_error.restype = C.c_char_p
     L.mars_hip_set_kernel.argtypes = [C.c_void_p, i32]

@@ -79,6 +79,10 @@ def _lib():
                                             P(CEncodeParams), P(CEncodeOptions),
                                             P(P(C.c_uint8)), P(C.c_size_t), P(CEncodeStats)]
         L.mars_hip_encode_plane.restype = C.c_int
+        L.mars_hip_encode_prepared.argtypes = [C.c_void_p, C.c_uint32, P(CEncodeParams),
+                                               P(CEncodeOptions), P(P(C.c_uint8)),
+                                               P(C.c_size_t), P(CEncodeStats)]
+        L.mars_hip_encode_prepared.restype = C.c_int
         L.mars_wrap_gray.argtypes = [P(C.c_uint8), C.c_size_t, P(P(C.c_uint8)), P(C.c_size_t)]
         L.mars_wrap_gray.restype = C.c_int
         L.mars_free.argtypes = [C.c_void_p]
@@ -149,6 +153,24 @@ def hip_encode_plane(ctx: HipContext, image: np.ndarray, params: EncodeParams,
                                  C.byref(_params(params)), C.byref(_options(options)),
                                  C.byref(buf), C.byref(n), C.byref(st))
     ctx._check(rc, "mars_hip_encode_plane")
+    return _take(L, buf, n), _stats(st)
+
+
+def hip_encode_prepared(ctx: HipContext, plane: int, params: EncodeParams,
+                        options: EncodeOptions) -> Tuple[bytes, EncodeStats]:
+    """Encode plane `plane` of the batch prepared in ctx from its search table.
+
+    Runs without the GIL (ctypes releases it), so several planes can be
+    encoded from a thread pool at once.
+    """
+    L = _lib()
+    buf = C.POINTER(C.c_uint8)()
+    n = C.c_size_t()
+    st = CEncodeStats()
+    rc = L.mars_hip_encode_prepared(ctx._ctx, plane, C.byref(_params(params)),
+                                    C.byref(_options(options)), C.byref(buf), C.byref(n),
+                                    C.byref(st))
+    ctx._check(rc, "mars_hip_encode_prepared")
     return _take(L, buf, n), _stats(st)
 
 

@@ -1,10 +1,19 @@
 // Host side of the C ABI declared in include/mars_hip.h.
 //
-// One context owns one HIP stream and the device copy of one plane: its pool
-// (contracted plane, split i8 planes, per-size window sums) and one search
-// table per (range size, domain stride).  prepare() enumerates the range
-// nodes a quadtree walk can request, uploads the plane, runs the step and
-// downloads the winners; search() is a lookup into that table.
+// One context owns one HIP stream (plus side streams for concurrent
+// per-size searches) and the device copy of a batch of planes: each plane's
+// pool (contracted plane, split i8 planes, summed-area tables) and one search
+// table per (plane, range size, domain stride).  All device buffers of a
+// prepared batch live in one arena.  prepare_batch() enumerates the range
+// nodes a quadtree walk can request on every plane, uploads the planes, lays
+// out per-plane and per-table descriptors plus flat work lists, runs the
+// step and downloads the winners; search() is a lookup into that table.
+//
+// A step is: one launch each for the pool and range preparation of every
+// plane and table, then per (size, stride) one search launch over all
+// planes' tables and one merge launch, the per-size chains on side streams
+// (forked after the preparation, joined at the end), the whole step
+// replayable from one captured HIP graph.
 
 #include <hip/hip_runtime.h>
 
@@ -15,6 +24,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -24,20 +34,48 @@
 
 namespace {
 
-using mars::MergeArgs;
+using mars::Common;
 using mars::Partial;
-using mars::SearchArgs;
+using mars::PlaneDesc;
+using mars::TableDesc;
+using mars::Work2;
+using mars::Work3;
 
 constexpr uint32_t kMaxSize = 32;
-constexpr uint32_t kTargetBlocks = 2048; // >> 256 CUs: fill the chip per launch
 constexpr uint32_t kCounters = 16;
+constexpr int kSideStreams = 4;
+constexpr uint32_t kMaxPlanes = 65536;
+constexpr size_t kAlign = 256;
+
+// Search chunking shape per range size: position chunks per range group
+// are chosen so a launch has about `target_blocks` blocks, with at least
+// `min_tiles` position tiles per chunk.  Results do not depend on it.
+struct Shape {
+    uint32_t target_blocks = 2048; // >> 256 CUs: fill the chip per launch
+    uint32_t min_tiles = 8;        // >= 2 tiles per wave
+};
+constexpr uint32_t kShapeTargets[] = {1024, 2048, 4096, 8192, 16384};
+constexpr uint32_t kShapeMinTiles[] = {4, 8, 16};
+constexpr uint32_t kMaxTarget = 16384, kMinTiles = 4; // bounds of the autotuner's space
+
+struct Plane {
+    uint32_t w = 0, h = 0, wc = 0, hc = 0, pitch = 0;
+    size_t copy_stride = 0;
+    std::vector<uint8_t> host; // the caller's pixels, for the host-side encoder
+    uint8_t *d_px = nullptr;
+    int32_t *d_d = nullptr;
+    float *d_df = nullptr;
+    int8_t *d_lo = nullptr, *d_hi = nullptr;
+    int64_t *d_sat1 = nullptr, *d_sat2 = nullptr;
+};
 
 struct Table {
-    uint32_t size = 0, shift = 0;
+    uint32_t plane = 0, size = 0, shift = 0;
     std::vector<uint32_t> rows, cols;
     std::unordered_map<uint64_t, uint32_t> index;
     uint32_t nr = 0, nr_pad = 0, P = 0, npr = 0, npc = 0, kmem = 0;
-    uint32_t nchunks_mfma = 1, tpc_mfma = 1, nchunks_fp32 = 1, tpc_fp32 = 1;
+    uint32_t nch_cap[2] = {1, 1};                   // partial capacity per range
+    uint32_t nchunks[2] = {1, 1}, tpc[2] = {1, 1}; // current chunking
     uint32_t *d_rows = nullptr, *d_cols = nullptr;
     int8_t *d_rng8 = nullptr;
     float *d_rngf = nullptr;
@@ -52,34 +90,46 @@ struct Table {
     std::vector<uint8_t> h_found;
 };
 
-} // namespace
+// Slice of a work list in the device work buffer.
+struct Slice {
+    uint32_t off = 0, n = 0;
+};
 
-constexpr int kSideStreams = 4;
+// All tables of one (size, stride): one search launch and one merge launch.
+struct Group {
+    uint32_t size = 0, shift = 0;
+    std::vector<uint32_t> tables; // longest searches first
+    Slice search[2], merge[2];    // per kernel
+};
+
+} // namespace
 
 struct mars_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // Per-size search chains run on side streams forked from `stream` after
-    // the pool build (all at the default priority).
-    hipStream_t side[kSideStreams] = {};
+    hipStream_t side[kSideStreams] = {}; // default priority, like `stream`
     hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
     mutable std::mutex err_mu;
     std::string err;
     int kernel = MARS_HIP_KERNEL_MFMA_I8;
     bool prepared = false;
-    uint32_t w = 0, h = 0, wc = 0, hc = 0, pitch = 0;
-    size_t copy_stride = 0;
     mars_hip_params params{};
-    uint8_t *d_px = nullptr;
-    int32_t *d_d = nullptr;
-    float *d_df = nullptr;
-    int8_t *d_lo = nullptr, *d_hi = nullptr;
-    int64_t *d_sat1 = nullptr, *d_sat2 = nullptr;
+    std::vector<Plane> planes;
+    std::vector<Table> tables;
+    std::vector<Group> groups;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, uint32_t> table_of; // (plane, size, shift)
+    Shape shape[6];                                                        // per log2(size) - 1
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    PlaneDesc *d_planes = nullptr;
+    TableDesc *d_tabs = nullptr;
+    Work2 *d_work2 = nullptr;
+    Work3 *d_work3 = nullptr;
+    size_t work2_cap = 0, work3_cap = 0;
+    Slice wl_contract, wl_sat_rows, wl_sat_cols, wl_possums, wl_ranges, wl_rstats, wl_seed;
     unsigned long long *d_counters = nullptr;
     double *d_yscale = nullptr;
-    int stage2 = 0;
-    double alfa_c = 0, alfa_unit = 0;
-    std::vector<Table> tables;
+    Common common{};
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_kernel = -1;
@@ -104,18 +154,28 @@ int fail(mars_hip_ctx *ctx, int code, const std::string &msg)
                         std::string(#expr) + ": " + hipGetErrorString(e_));                    \
     } while (0)
 
+#define RC_TRY(expr)                                                                           \
+    do {                                                                                       \
+        int rc_ = (expr);                                                                      \
+        if (rc_) return rc_;                                                                   \
+    } while (0)
+
 uint32_t ceil_log2(uint32_t q)
 {
     if (q <= 1) return 0;
     return 32 - __builtin_clz(q - 1);
 }
 
+uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+typedef std::map<uint32_t, std::vector<std::pair<uint32_t, uint32_t>>> NodeMap;
+
 // The nodes a walk can search (encode.rs:1330-1362 walk, 1865-1919 walk_rd):
 // forced nodes (larger than max_size or crossing the border) split without a
 // search; a size-1 node never searches; every other node is searched and
 // splits further while size > min_size.
 void collect(uint32_t row, uint32_t col, uint32_t size, uint32_t w, uint32_t h,
-             const mars_hip_params &p, std::map<uint32_t, std::vector<std::pair<uint32_t, uint32_t>>> &out)
+             const mars_hip_params &p, NodeMap &out)
 {
     if (row >= h || col >= w) return;
     bool forced = size > p.max_size || row + size > h || col + size > w;
@@ -138,171 +198,291 @@ void collect(uint32_t row, uint32_t col, uint32_t size, uint32_t w, uint32_t h,
     }
 }
 
-void free_tables(mars_hip_ctx *ctx)
+void drop_graph(mars_hip_ctx *ctx)
 {
-    for (Table &t : ctx->tables) {
-        (void)hipFree(t.d_rows);
-        (void)hipFree(t.d_cols);
-        (void)hipFree(t.d_rng8);
-        (void)hipFree(t.d_rngf);
-        (void)hipFree(t.d_rstat);
-        (void)hipFree(t.d_gthr);
-        (void)hipFree(t.d_posinfo);
-        (void)hipFree(t.d_posd);
-        (void)hipFree(t.d_partial);
-        (void)hipFree(t.d_out);
-        (void)hipFree(t.d_found);
-    }
-    ctx->tables.clear();
-    (void)hipFree(ctx->d_px);
-    (void)hipFree(ctx->d_d);
-    (void)hipFree(ctx->d_df);
-    (void)hipFree(ctx->d_lo);
-    (void)hipFree(ctx->d_hi);
-    (void)hipFree(ctx->d_yscale);
-    (void)hipFree(ctx->d_sat1);
-    (void)hipFree(ctx->d_sat2);
-    ctx->d_yscale = nullptr;
-    ctx->d_sat1 = nullptr;
-    ctx->d_sat2 = nullptr;
-    ctx->d_px = nullptr;
-    ctx->d_d = nullptr;
-    ctx->d_df = nullptr;
-    ctx->d_lo = nullptr;
-    ctx->d_hi = nullptr;
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     ctx->graph_exec = nullptr;
     ctx->graph = nullptr;
     ctx->graph_kernel = -1;
+}
+
+void free_batch(mars_hip_ctx *ctx)
+{
+    drop_graph(ctx);
+    ctx->planes.clear();
+    ctx->tables.clear();
+    ctx->groups.clear();
+    ctx->table_of.clear();
+    (void)hipFree(ctx->arena);
+    (void)hipFree(ctx->d_work2);
+    (void)hipFree(ctx->d_work3);
+    (void)hipFree(ctx->d_yscale);
+    ctx->arena = nullptr;
+    ctx->arena_bytes = 0;
+    ctx->d_planes = nullptr;
+    ctx->d_tabs = nullptr;
+    ctx->d_work2 = nullptr;
+    ctx->d_work3 = nullptr;
+    ctx->work2_cap = ctx->work3_cap = 0;
+    ctx->d_yscale = nullptr;
     ctx->prepared = false;
 }
 
-void chunking(uint32_t ntiles, uint32_t ngroups, uint32_t *nchunks, uint32_t *tpc)
+// Position chunks for a table of `ntiles` tiles in a launch that covers
+// `ngroups` range groups in total.
+void chunking(uint32_t ntiles, uint32_t ngroups, uint32_t target, uint32_t min_tiles,
+              uint32_t *nchunks, uint32_t *tpc)
 {
     if (ntiles == 0) {
         *nchunks = 1;
         *tpc = 1;
         return;
     }
-    uint32_t want = std::max<uint32_t>(1, kTargetBlocks / std::max<uint32_t>(1, ngroups));
-    uint32_t maxc = std::max<uint32_t>(1, ntiles / 8); // >= 2 tiles per wave
+    uint32_t want = std::max<uint32_t>(1, target / std::max<uint32_t>(1, ngroups));
+    uint32_t maxc = std::max<uint32_t>(1, ntiles / min_tiles);
     uint32_t c = std::min(want, maxc);
-    uint32_t t = (ntiles + c - 1) / c;
+    uint32_t t = cdiv(ntiles, c);
     *tpc = t;
-    *nchunks = (ntiles + t - 1) / t;
+    *nchunks = cdiv(ntiles, t);
 }
 
-SearchArgs search_args(const mars_hip_ctx *ctx, const Table &t, int kernel)
-{
-    SearchArgs a{};
-    a.rng8 = t.d_rng8;
-    a.rngf = t.d_rngf;
-    a.rstat = t.d_rstat;
-    a.gthr = t.d_gthr;
-    a.posinfo = t.d_posinfo;
-    a.posd = t.d_posd;
-    a.lo = ctx->d_lo;
-    a.hi = ctx->d_hi;
-    a.copy_stride = ctx->copy_stride;
-    a.pitch = ctx->pitch;
-    a.dplanef = ctx->d_df;
-    a.wc = ctx->wc;
-    a.P = t.P;
-    a.npc = t.npc;
-    a.shift = t.shift;
-    a.nr = t.nr;
-    a.max_alfa = ctx->params.max_alfa;
-    a.bits_alfa = ctx->params.bits_alfa;
-    a.bits_beta = ctx->params.bits_beta;
-    a.partial = t.d_partial;
-    a.counters = ctx->d_counters;
-    a.counter_slot = ceil_log2(t.size);
-    a.stage2 = ctx->stage2;
-    a.alfa_c = ctx->alfa_c;
-    a.alfa_unit = ctx->alfa_unit;
-    a.qa_max = (double)((1u << ctx->params.bits_alfa) - 1);
-    a.qb_max = (double)((1u << ctx->params.bits_beta) - 1);
-    a.yscale = ctx->d_yscale;
-    if (kernel == MARS_HIP_KERNEL_MFMA_I8) {
-        a.ngroups = t.nr_pad / 8;
-        a.nchunks = t.nchunks_mfma;
-        a.tiles_per_chunk = t.tpc_mfma;
-    } else {
-        a.ngroups = t.nr_pad / 4;
-        a.nchunks = t.nchunks_fp32;
-        a.tiles_per_chunk = t.tpc_fp32;
+// Tiles and range groups of a table for kernel k (MFMA: 32 positions x 8
+// ranges; FP32: 64 positions x 4 ranges).
+uint32_t tiles_of(const Table &t, int k) { return cdiv(t.P, k == 0 ? 32 : 64); }
+uint32_t groups_of(const Table &t, int k) { return t.nr_pad / (k == 0 ? 8 : 4); }
+
+// Bump allocator over one device allocation.
+struct ArenaPlan {
+    struct Item {
+        void **dst;
+        size_t bytes;
+    };
+    std::vector<Item> items;
+    size_t total = 0;
+    template <class X> void add(X **dst, size_t bytes)
+    {
+        bytes = std::max<size_t>(bytes, 1);
+        items.push_back({(void **)dst, bytes});
+        total += (bytes + kAlign - 1) / kAlign * kAlign;
     }
-    return a;
+    void bind(char *base)
+    {
+        size_t off = 0;
+        for (const Item &it : items) {
+            *it.dst = base + off;
+            off += (it.bytes + kAlign - 1) / kAlign * kAlign;
+        }
+    }
+};
+
+// Chunking of every table from the current shapes, then descriptors and work
+// lists built and uploaded.  Invalidates the captured graph.
+int layout(mars_hip_ctx *ctx)
+{
+    drop_graph(ctx);
+    // chunking per group: the launch covers all tables of the group
+    for (Group &g : ctx->groups) {
+        const Shape &sh = ctx->shape[ceil_log2(g.size) - 1];
+        for (int k = 0; k < 2; k++) {
+            uint32_t ng = 0;
+            for (uint32_t ti : g.tables) ng += groups_of(ctx->tables[ti], k);
+            for (uint32_t ti : g.tables) {
+                Table &t = ctx->tables[ti];
+                chunking(tiles_of(t, k), ng, sh.target_blocks, sh.min_tiles, &t.nchunks[k],
+                         &t.tpc[k]);
+                if (t.nchunks[k] > t.nch_cap[k]) { // outside the allocated shape space
+                    t.tpc[k] = cdiv(tiles_of(t, k), t.nch_cap[k]);
+                    t.nchunks[k] = cdiv(tiles_of(t, k), t.tpc[k]);
+                }
+            }
+        }
+    }
+    // descriptors
+    std::vector<PlaneDesc> pd(ctx->planes.size());
+    for (size_t i = 0; i < pd.size(); i++) {
+        const Plane &p = ctx->planes[i];
+        pd[i] = PlaneDesc{p.d_px, p.d_d, p.d_df, p.d_lo, p.d_hi, p.d_sat1, p.d_sat2,
+                          p.w,    p.h,   p.wc,   p.hc,   p.pitch, (uint32_t)p.copy_stride};
+    }
+    std::vector<TableDesc> td(ctx->tables.size());
+    for (size_t i = 0; i < td.size(); i++) {
+        const Table &t = ctx->tables[i];
+        const Plane &p = ctx->planes[t.plane];
+        TableDesc &d = td[i];
+        d.px = p.d_px;
+        d.d = p.d_d;
+        d.df = p.d_df;
+        d.lo = p.d_lo;
+        d.hi = p.d_hi;
+        d.sat1 = p.d_sat1;
+        d.sat2 = p.d_sat2;
+        d.w = p.w;
+        d.wc = p.wc;
+        d.pitch = p.pitch;
+        d.copy_stride = (uint32_t)p.copy_stride;
+        d.rows = t.d_rows;
+        d.cols = t.d_cols;
+        d.rng8 = t.d_rng8;
+        d.rngf = t.d_rngf;
+        d.rstat = t.d_rstat;
+        d.gthr = t.d_gthr;
+        d.posinfo = t.d_posinfo;
+        d.posd = t.d_posd;
+        d.partial = t.d_partial;
+        d.out = t.d_out;
+        d.found = t.d_found;
+        d.size = t.size;
+        d.shift = t.shift;
+        d.P = t.P;
+        d.npc = t.npc;
+        d.nr = t.nr;
+        d.nr_pad = t.nr_pad;
+        d.kmem = t.kmem;
+        for (int k = 0; k < 2; k++) {
+            d.nchunks[k] = t.nchunks[k];
+            d.tiles_per_chunk[k] = t.tpc[k];
+        }
+    }
+    // work lists
+    std::vector<Work2> w2;
+    std::vector<Work3> w3;
+    auto begin2 = [&](Slice &s) { s.off = (uint32_t)w2.size(); };
+    auto end2 = [&](Slice &s) { s.n = (uint32_t)w2.size() - s.off; };
+    begin2(ctx->wl_contract);
+    for (uint32_t i = 0; i < ctx->planes.size(); i++) {
+        const Plane &p = ctx->planes[i];
+        for (uint32_t b = 0; b < cdiv((uint64_t)p.wc * p.hc, 256); b++) w2.push_back(make_uint2(i, b));
+    }
+    if (w2.size() == ctx->wl_contract.off) w2.push_back(make_uint2(0, 0)); // counters reset
+    end2(ctx->wl_contract);
+    begin2(ctx->wl_sat_rows);
+    for (uint32_t i = 0; i < ctx->planes.size(); i++)
+        for (uint32_t r = 0; r <= ctx->planes[i].hc; r++) w2.push_back(make_uint2(i, r));
+    end2(ctx->wl_sat_rows);
+    begin2(ctx->wl_sat_cols);
+    for (uint32_t i = 0; i < ctx->planes.size(); i++)
+        for (uint32_t b = 0; b < cdiv(ctx->planes[i].wc + 1, 16); b++) w2.push_back(make_uint2(i, b));
+    end2(ctx->wl_sat_cols);
+    begin2(ctx->wl_possums);
+    for (uint32_t i = 0; i < ctx->tables.size(); i++)
+        for (uint32_t b = 0; b < cdiv(ctx->tables[i].P, 256); b++) w2.push_back(make_uint2(i, b));
+    end2(ctx->wl_possums);
+    begin2(ctx->wl_ranges);
+    for (uint32_t i = 0; i < ctx->tables.size(); i++) {
+        const Table &t = ctx->tables[i];
+        if (t.P == 0) continue;
+        for (uint32_t b = 0; b < cdiv((uint64_t)t.nr_pad * 8 * t.kmem / 4, 256); b++)
+            w2.push_back(make_uint2(i, b));
+    }
+    end2(ctx->wl_ranges);
+    begin2(ctx->wl_rstats);
+    for (uint32_t i = 0; i < ctx->tables.size(); i++) {
+        const Table &t = ctx->tables[i];
+        if (t.P == 0) continue;
+        for (uint32_t b = 0; b < cdiv(t.nr_pad, 4); b++) w2.push_back(make_uint2(i, b));
+    }
+    end2(ctx->wl_rstats);
+    begin2(ctx->wl_seed);
+    for (uint32_t i = 0; i < ctx->tables.size(); i++) {
+        const Table &t = ctx->tables[i];
+        if (t.P == 0) continue;
+        for (uint32_t r = 0; r < t.nr; r++) w2.push_back(make_uint2(i, r));
+    }
+    end2(ctx->wl_seed);
+    for (Group &g : ctx->groups) {
+        for (int k = 0; k < 2; k++) {
+            g.search[k].off = (uint32_t)w3.size();
+            for (uint32_t ti : g.tables) {
+                const Table &t = ctx->tables[ti];
+                if (t.P == 0) continue;
+                for (uint32_t gr = 0; gr < groups_of(t, k); gr++)
+                    for (uint32_t ch = 0; ch < t.nchunks[k]; ch++) w3.push_back(make_uint3(ti, ch, gr));
+            }
+            g.search[k].n = (uint32_t)w3.size() - g.search[k].off;
+            begin2(g.merge[k]);
+            for (uint32_t ti : g.tables) {
+                const Table &t = ctx->tables[ti];
+                const uint32_t nb = k == 0 ? cdiv(t.nr, 128) : t.nr;
+                for (uint32_t b = 0; b < nb; b++) w2.push_back(make_uint2(ti, b));
+            }
+            end2(g.merge[k]);
+        }
+    }
+    // every launch must stay below 2^32 threads
+    if (w2.size() >= (1u << 24) || w3.size() >= (1u << 24))
+        return fail(ctx, MARS_HIP_E_UNSUPPORTED, "batch too large for one step");
+    if (w2.size() > ctx->work2_cap) {
+        (void)hipFree(ctx->d_work2);
+        ctx->d_work2 = nullptr;
+        HIP_TRY(ctx, hipMalloc((void **)&ctx->d_work2, sizeof(Work2) * w2.size()));
+        ctx->work2_cap = w2.size();
+    }
+    if (std::max<size_t>(w3.size(), 1) > ctx->work3_cap) {
+        (void)hipFree(ctx->d_work3);
+        ctx->d_work3 = nullptr;
+        HIP_TRY(ctx, hipMalloc((void **)&ctx->d_work3, sizeof(Work3) * std::max<size_t>(w3.size(), 1)));
+        ctx->work3_cap = std::max<size_t>(w3.size(), 1);
+    }
+    HIP_TRY(ctx, hipMemcpy(ctx->d_work2, w2.data(), sizeof(Work2) * w2.size(), hipMemcpyHostToDevice));
+    if (!w3.empty())
+        HIP_TRY(ctx, hipMemcpy(ctx->d_work3, w3.data(), sizeof(Work3) * w3.size(), hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(ctx->d_planes, pd.data(), sizeof(PlaneDesc) * pd.size(), hipMemcpyHostToDevice));
+    if (!td.empty())
+        HIP_TRY(ctx, hipMemcpy(ctx->d_tabs, td.data(), sizeof(TableDesc) * td.size(), hipMemcpyHostToDevice));
+    return MARS_HIP_OK;
 }
 
-MergeArgs merge_args(const mars_hip_ctx *ctx, const Table &t, int kernel)
+// Pool and range preparation of every plane and table.
+void enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st)
 {
-    MergeArgs m{};
-    m.partial = t.d_partial;
-    m.t1_exact = kernel == MARS_HIP_KERNEL_MFMA_I8 ? 1 : 0;
-    m.nchunks = kernel == MARS_HIP_KERNEL_MFMA_I8 ? t.nchunks_mfma : t.nchunks_fp32;
-    m.nr = t.nr;
-    m.P = t.P;
-    m.npc = t.npc;
-    m.shift = t.shift;
-    m.size = t.size;
-    m.rows = t.d_rows;
-    m.cols = t.d_cols;
-    m.px = ctx->d_px;
-    m.w = ctx->w;
-    m.d = ctx->d_d;
-    m.wc = ctx->wc;
-    m.posinfo = t.d_posinfo;
-    m.rstat = t.d_rstat;
-    m.max_alfa = ctx->params.max_alfa;
-    m.bits_alfa = ctx->params.bits_alfa;
-    m.bits_beta = ctx->params.bits_beta;
-    m.out = t.d_out;
-    m.found = t.d_found;
-    return m;
-}
-
-void enqueue_table(mars_hip_ctx *ctx, Table &t, int kernel, hipStream_t st)
-{
-    mars::launch_possums(ctx->d_sat1, ctx->d_sat2, ctx->wc, t.size, t.shift, t.npc, t.P,
-                         t.d_posinfo, t.d_posd, st);
-    mars::launch_ranges(ctx->d_px, ctx->w, t.d_rows, t.d_cols, t.nr, t.nr_pad, t.size, t.kmem,
-                        t.d_rng8, kernel == MARS_HIP_KERNEL_FP32 ? t.d_rngf : nullptr, t.d_rstat,
-                        t.d_gthr, st);
+    const Work2 *w2 = ctx->d_work2;
+    mars::launch_contract(ctx->d_planes, w2 + ctx->wl_contract.off, ctx->wl_contract.n,
+                          ctx->d_counters, kCounters, st);
+    mars::launch_sat(ctx->d_planes, w2 + ctx->wl_sat_rows.off, ctx->wl_sat_rows.n,
+                     w2 + ctx->wl_sat_cols.off, ctx->wl_sat_cols.n, st);
+    mars::launch_possums(ctx->d_tabs, w2 + ctx->wl_possums.off, ctx->wl_possums.n, st);
+    mars::launch_ranges(ctx->d_tabs, w2 + ctx->wl_ranges.off, ctx->wl_ranges.n,
+                        kernel == MARS_HIP_KERNEL_FP32, st);
+    mars::launch_rstats(ctx->d_tabs, w2 + ctx->wl_rstats.off, ctx->wl_rstats.n, st);
     if (kernel == MARS_HIP_KERNEL_MFMA_I8)
-        mars::launch_seed(t.d_rng8, t.d_rstat, ctx->d_d, ctx->wc, t.size, t.kmem, t.npc, t.P,
-                          t.d_posinfo, t.nr, ctx->params.max_alfa, ctx->params.bits_alfa,
-                          ctx->params.bits_beta, t.d_gthr, st);
+        mars::launch_seed(ctx->d_tabs, w2 + ctx->wl_seed.off, ctx->wl_seed.n, ctx->common, st);
 }
 
-// Enqueue one whole step.  With ev (3 + n_tables events: before the pool,
-// after pool/range prep, after each table's search, after the merges) the
-// step runs serially on the context stream so each launch can be timed;
-// without, each table's chain (prep, search, merge) runs on its own side
-// stream forked after the pool build and joined at the end.
+int enqueue_search(mars_hip_ctx *ctx, const Group &g, int kernel, hipStream_t st)
+{
+    const int k = kernel == MARS_HIP_KERNEL_MFMA_I8 ? 0 : 1;
+    if (mars::launch_search(kernel, g.size, ctx->d_tabs, ctx->d_work3 + g.search[k].off,
+                            g.search[k].n, ctx->common, st) != 0)
+        return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
+    return MARS_HIP_OK;
+}
+
+void enqueue_merge(mars_hip_ctx *ctx, const Group &g, int kernel, hipStream_t st)
+{
+    const int k = kernel == MARS_HIP_KERNEL_MFMA_I8 ? 0 : 1;
+    mars::launch_merge(ctx->d_tabs, ctx->d_work2 + g.merge[k].off, g.merge[k].n, k == 0,
+                       ctx->common, st);
+}
+
+// Enqueue one whole step.  With ev (3 + n_groups events: before the pool,
+// after the preparation, after each group's search, after the merges) the
+// step runs serially on the context stream so each phase can be timed;
+// without, each group's search and merge run on a side stream forked after
+// the preparation and joined at the end.
 int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
 {
     hipStream_t st = ctx->stream;
     if (ev) HIP_TRY(ctx, hipEventRecord(ev[0], st));
-    mars::launch_contract(ctx->d_px, ctx->w, ctx->wc, ctx->hc, ctx->pitch, ctx->copy_stride,
-                          ctx->d_d, ctx->d_df, ctx->d_lo, ctx->d_hi, ctx->d_counters, kCounters,
-                          st);
-    mars::launch_sat(ctx->d_d, ctx->wc, ctx->hc, ctx->d_sat1, ctx->d_sat2, st);
+    enqueue_prep(ctx, kernel, st);
     if (!ev) {
         HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, st));
-        const int nside = std::min<int>(kSideStreams, (int)ctx->tables.size());
+        const int nside = std::min<int>(kSideStreams, (int)ctx->groups.size());
         for (int j = 0; j < nside; j++) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[j], ctx->fork_ev, 0));
-        for (size_t i = 0; i < ctx->tables.size(); i++) {
-            Table &t = ctx->tables[i];
+        for (size_t i = 0; i < ctx->groups.size(); i++) {
             hipStream_t ss = ctx->side[i % kSideStreams];
-            enqueue_table(ctx, t, kernel, ss);
-            if (t.P > 0 && t.nr > 0) {
-                SearchArgs a = search_args(ctx, t, kernel);
-                if (mars::launch_search(kernel, t.size, a, ss) != 0)
-                    return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
-            }
-            mars::launch_merge(merge_args(ctx, t, kernel), ss);
+            RC_TRY(enqueue_search(ctx, ctx->groups[i], kernel, ss));
+            enqueue_merge(ctx, ctx->groups[i], kernel, ss);
         }
         for (int j = 0; j < nside; j++) {
             HIP_TRY(ctx, hipEventRecord(ctx->join_ev[j], ctx->side[j]));
@@ -311,19 +491,13 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
         HIP_TRY(ctx, hipGetLastError());
         return MARS_HIP_OK;
     }
-    for (Table &t : ctx->tables) enqueue_table(ctx, t, kernel, st);
     HIP_TRY(ctx, hipEventRecord(ev[1], st));
-    for (size_t i = 0; i < ctx->tables.size(); i++) {
-        Table &t = ctx->tables[i];
-        if (t.P > 0 && t.nr > 0) {
-            SearchArgs a = search_args(ctx, t, kernel);
-            if (mars::launch_search(kernel, t.size, a, st) != 0)
-                return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
-        }
+    for (size_t i = 0; i < ctx->groups.size(); i++) {
+        RC_TRY(enqueue_search(ctx, ctx->groups[i], kernel, st));
         HIP_TRY(ctx, hipEventRecord(ev[2 + i], st));
     }
-    for (Table &t : ctx->tables) mars::launch_merge(merge_args(ctx, t, kernel), st);
-    HIP_TRY(ctx, hipEventRecord(ev[2 + ctx->tables.size()], st));
+    for (const Group &g : ctx->groups) enqueue_merge(ctx, g, kernel, st);
+    HIP_TRY(ctx, hipEventRecord(ev[2 + ctx->groups.size()], st));
     HIP_TRY(ctx, hipGetLastError());
     return MARS_HIP_OK;
 }
@@ -344,6 +518,261 @@ int download(mars_hip_ctx *ctx)
 }
 
 uint64_t table_evals(const Table &t) { return (uint64_t)t.P * 8; }
+
+// One eager step with events: per-phase times (ms) and counters.
+int timed_step(mars_hip_ctx *ctx, mars_hip_timing *tm, std::vector<double> *group_ms)
+{
+    const size_t ng = ctx->groups.size();
+    std::vector<hipEvent_t> ev(3 + ng);
+    for (auto &e : ev) HIP_TRY(ctx, hipEventCreate(&e));
+    int rc = enqueue_step(ctx, ctx->kernel, ev.data());
+    if (rc == 0) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, MARS_HIP_E_DEVICE, hipGetErrorString(e));
+    }
+    float ms = 0;
+    if (rc == 0 && tm) {
+        *tm = mars_hip_timing{};
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        tm->ms_pool = ms;
+    }
+    if (group_ms) group_ms->assign(ng, 0.0);
+    for (size_t i = 0; rc == 0 && i < ng; i++) {
+        (void)hipEventElapsedTime(&ms, ev[1 + i], ev[2 + i]);
+        if (group_ms) (*group_ms)[i] = ms;
+        if (tm) {
+            tm->ms_search += ms;
+            uint32_t lg = ceil_log2(ctx->groups[i].size);
+            if (lg >= 1 && lg <= 6) tm->ms_by_size[lg - 1] += ms;
+        }
+    }
+    if (rc == 0 && tm) {
+        unsigned long long cnt[kCounters] = {};
+        hipError_t e = hipMemcpy(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(ctx, MARS_HIP_E_DEVICE, hipGetErrorString(e));
+        for (int i = 0; i < 6; i++) {
+            tm->survivors_by_size[i] = cnt[i + 1];
+            tm->exact_fits_by_size[i] = cnt[8 + i + 1];
+            tm->survivors += cnt[i + 1];
+        }
+        for (const Table &t : ctx->tables) tm->evals += (uint64_t)t.nr * table_evals(t);
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+bool valid_params(const mars_hip_params *p)
+{
+    return p->bits_alfa >= 1 && p->bits_alfa <= 16 && p->bits_beta >= 1 && p->bits_beta <= 16 &&
+           p->max_alfa > 0.0 && p->min_size >= 1 && p->max_size >= p->min_size;
+}
+
+int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes,
+                  const mars_hip_params *p, const uint32_t *shifts, uint32_t n_shifts)
+{
+    for (uint32_t i = 0; i < n_planes; i++) {
+        if (!in[i].px) return fail(ctx, MARS_HIP_E_ARG, "null plane");
+        if (in[i].w == 0 || in[i].h == 0 || in[i].w > 16384 || in[i].h > 16384)
+            return fail(ctx, MARS_HIP_E_ARG, "bad plane size");
+    }
+    if (!valid_params(p)) return fail(ctx, MARS_HIP_E_ARG, "bad params");
+    for (uint32_t i = 0; i < n_shifts; i++)
+        if (shifts[i] == 0) return fail(ctx, MARS_HIP_E_ARG, "shift must be > 0");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    free_batch(ctx);
+    ctx->params = *p;
+
+    std::vector<uint32_t> sh(shifts, shifts + n_shifts);
+    std::sort(sh.begin(), sh.end());
+    sh.erase(std::unique(sh.begin(), sh.end()), sh.end());
+
+    // host geometry of planes and tables
+    ctx->planes.resize(n_planes);
+    for (uint32_t pi = 0; pi < n_planes; pi++) {
+        Plane &pl = ctx->planes[pi];
+        pl.w = in[pi].w;
+        pl.h = in[pi].h;
+        pl.wc = pl.w / 2;
+        pl.hc = pl.h / 2;
+        pl.pitch = ((pl.wc + 15) / 16) * 16 + 16;
+        pl.copy_stride = (size_t)pl.pitch * std::max<uint32_t>(pl.hc, 1);
+        pl.host.assign(in[pi].px, in[pi].px + (size_t)pl.w * pl.h);
+        NodeMap nodes;
+        collect(0, 0, 1u << ceil_log2(std::max(pl.w, pl.h)), pl.w, pl.h, *p, nodes);
+        for (auto &kv : nodes)
+            if (kv.first > kMaxSize)
+                return fail(ctx, MARS_HIP_E_UNSUPPORTED, "range sizes above 32 are not supported");
+        for (uint32_t shift : sh) {
+            for (auto it = nodes.rbegin(); it != nodes.rend(); ++it) {
+                Table t;
+                t.plane = pi;
+                t.size = it->first;
+                t.shift = shift;
+                for (auto &rc : it->second) {
+                    t.index[((uint64_t)rc.first << 32) | rc.second] = (uint32_t)t.rows.size();
+                    t.rows.push_back(rc.first);
+                    t.cols.push_back(rc.second);
+                }
+                t.nr = (uint32_t)t.rows.size();
+                t.nr_pad = (t.nr + 7) / 8 * 8;
+                const uint32_t s = t.size, K = s * s;
+                t.kmem = K < 16 ? 16 : K;
+                if (pl.h >= 2 * s && pl.w >= 2 * s) {
+                    t.npr = (pl.h - 2 * s) / shift + 1;
+                    t.npc = (pl.w - 2 * s) / shift + 1;
+                }
+                t.P = t.npr * t.npc;
+                if (t.P > (0xffffffffu / 8)) return fail(ctx, MARS_HIP_E_UNSUPPORTED, "too many positions");
+                ctx->table_of[std::make_tuple(pi, t.size, shift)] = (uint32_t)ctx->tables.size();
+                ctx->tables.push_back(std::move(t));
+            }
+        }
+    }
+    // groups (size descending, then stride), longest searches first
+    {
+        std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> by;
+        for (uint32_t i = 0; i < ctx->tables.size(); i++)
+            by[{ctx->tables[i].size, ctx->tables[i].shift}].push_back(i);
+        for (auto it = by.rbegin(); it != by.rend(); ++it) {
+            Group g;
+            g.size = it->first.first;
+            g.shift = it->first.second;
+            g.tables = it->second;
+            std::stable_sort(g.tables.begin(), g.tables.end(), [&](uint32_t a, uint32_t b) {
+                return ctx->tables[a].P > ctx->tables[b].P;
+            });
+            ctx->groups.push_back(std::move(g));
+        }
+    }
+    // partial capacity: the most chunks any shape of the autotuner's space gives
+    for (Group &g : ctx->groups)
+        for (int k = 0; k < 2; k++) {
+            uint32_t ng = 0;
+            for (uint32_t ti : g.tables) ng += groups_of(ctx->tables[ti], k);
+            for (uint32_t ti : g.tables) {
+                Table &t = ctx->tables[ti];
+                uint32_t tpc;
+                chunking(tiles_of(t, k), ng, kMaxTarget, kMinTiles, &t.nch_cap[k], &tpc);
+            }
+        }
+
+    // one arena for every buffer of the batch
+    ArenaPlan plan;
+    plan.add(&ctx->d_planes, sizeof(PlaneDesc) * std::max<size_t>(ctx->planes.size(), 1));
+    plan.add(&ctx->d_tabs, sizeof(TableDesc) * std::max<size_t>(ctx->tables.size(), 1));
+    for (Plane &pl : ctx->planes) {
+        const size_t nc = (size_t)pl.wc * pl.hc, nsat = (size_t)(pl.wc + 1) * (pl.hc + 1);
+        plan.add(&pl.d_px, (size_t)pl.w * pl.h);
+        plan.add(&pl.d_d, nc * sizeof(int32_t));
+        plan.add(&pl.d_df, nc * sizeof(float));
+        plan.add(&pl.d_lo, 16 * pl.copy_stride);
+        plan.add(&pl.d_hi, 16 * pl.copy_stride);
+        plan.add(&pl.d_sat1, nsat * sizeof(int64_t));
+        plan.add(&pl.d_sat2, nsat * sizeof(int64_t));
+    }
+    for (Table &t : ctx->tables) {
+        const size_t K = (size_t)t.size * t.size;
+        plan.add(&t.d_rows, 4 * (size_t)t.nr);
+        plan.add(&t.d_cols, 4 * (size_t)t.nr);
+        plan.add(&t.d_rng8, (size_t)t.nr_pad * 8 * t.kmem);
+        plan.add(&t.d_rngf, (size_t)t.nr_pad / 4 * K * 32 * sizeof(float));
+        plan.add(&t.d_rstat, sizeof(int32_t) * 2 * t.nr_pad);
+        plan.add(&t.d_gthr, sizeof(uint32_t) * t.nr_pad);
+        plan.add(&t.d_posinfo, sizeof(uint4) * (size_t)t.P);
+        plan.add(&t.d_posd, 2 * sizeof(double) * (size_t)t.P);
+        plan.add(&t.d_partial, sizeof(Partial) * (size_t)t.nr_pad * std::max(t.nch_cap[0], t.nch_cap[1]));
+        plan.add(&t.d_out, sizeof(mars_hip_candidate) * (size_t)t.nr);
+        plan.add(&t.d_found, (size_t)t.nr);
+    }
+    HIP_TRY(ctx, hipMalloc(&ctx->arena, plan.total));
+    ctx->arena_bytes = plan.total;
+    plan.bind((char *)ctx->arena);
+    HIP_TRY(ctx, hipMemsetAsync(ctx->arena, 0, plan.total, ctx->stream));
+    for (uint32_t pi = 0; pi < n_planes; pi++) {
+        Plane &pl = ctx->planes[pi];
+        HIP_TRY(ctx, hipMemcpyAsync(pl.d_px, in[pi].px, (size_t)pl.w * pl.h, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    }
+    for (Table &t : ctx->tables) {
+        HIP_TRY(ctx, hipMemcpyAsync(t.d_rows, t.rows.data(), 4 * (size_t)t.nr, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(t.d_cols, t.cols.data(), 4 * (size_t)t.nr, hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+
+    // Stage-2 constants: exact only when max_alfa is a power of two (then the
+    // alfa scaling of encode.rs:264 is exact in binary64).
+    {
+        int e = 0;
+        double mant = std::frexp(p->max_alfa, &e);
+        Common &c = ctx->common;
+        c = Common{};
+        c.max_alfa = p->max_alfa;
+        c.bits_alfa = p->bits_alfa;
+        c.bits_beta = p->bits_beta;
+        c.stage2 = (mant == 0.5 && p->bits_alfa <= 8 && p->bits_beta <= 12) ? 1 : 0;
+        c.alfa_c = std::ldexp(1.0, (int)p->bits_alfa + 2) / p->max_alfa;
+        c.alfa_unit = p->max_alfa / std::ldexp(1.0, (int)p->bits_alfa);
+        c.qa_max = (double)((1u << p->bits_alfa) - 1);
+        c.qb_max = (double)((1u << p->bits_beta) - 1);
+        const uint32_t nq = 1u << p->bits_alfa;
+        std::vector<double> ys(nq);
+        for (uint32_t q = 0; q < nq; q++) {
+            double a2 = (double)q / (double)(1u << p->bits_alfa) * p->max_alfa;
+            ys[q] = c.qb_max / ((1.0 + std::fabs(a2)) * 255.0);
+        }
+        HIP_TRY(ctx, hipMalloc((void **)&ctx->d_yscale, sizeof(double) * nq));
+        HIP_TRY(ctx, hipMemcpy(ctx->d_yscale, ys.data(), sizeof(double) * nq, hipMemcpyHostToDevice));
+        c.yscale = ctx->d_yscale;
+        c.counters = ctx->d_counters;
+    }
+    RC_TRY(layout(ctx));
+    RC_TRY(enqueue_step(ctx, ctx->kernel, nullptr));
+    RC_TRY(download(ctx));
+    ctx->prepared = true;
+    return MARS_HIP_OK;
+}
+
+int lookup(const mars_hip_ctx *ctx, uint32_t plane, uint32_t row, uint32_t col, uint32_t size,
+           uint32_t shift, uint32_t k, mars_hip_candidate *out, uint32_t *n_out, uint64_t *evals)
+{
+    mars_hip_ctx *mctx = const_cast<mars_hip_ctx *>(ctx);
+    if (!ctx || !out || !n_out || !evals) return fail(mctx, MARS_HIP_E_ARG, "null argument");
+    if (k != 1) return fail(mctx, MARS_HIP_E_UNSUPPORTED, "only k == 1 is supported");
+    if (!ctx->prepared) return fail(mctx, MARS_HIP_E_NOT_PREPARED, "prepare() first");
+    auto ti = ctx->table_of.find(std::make_tuple(plane, size, shift));
+    if (ti != ctx->table_of.end()) {
+        const Table &t = ctx->tables[ti->second];
+        auto it = t.index.find(((uint64_t)row << 32) | col);
+        if (it != t.index.end()) {
+            const uint32_t i = it->second;
+            *evals = table_evals(t);
+            if (t.h_found[i]) {
+                *out = t.h_out[i];
+                *n_out = 1;
+            } else {
+                *n_out = 0;
+            }
+            return MARS_HIP_OK;
+        }
+    }
+    char buf[192];
+    snprintf(buf, sizeof buf, "plane %u range (%u,%u) size %u shift %u was not prepared", plane,
+             row, col, size, shift);
+    return fail(mctx, MARS_HIP_E_NOT_IN_TABLE, buf);
+}
+
+struct PlaneRef {
+    const mars_hip_ctx *ctx;
+    uint32_t plane;
+};
+
+int plane_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
+                    mars_hip_candidate *out, uint32_t *n_out, uint64_t *evals)
+{
+    const PlaneRef *r = (const PlaneRef *)user;
+    return lookup(r->ctx, r->plane, row, col, size, shift, 1, out, n_out, evals);
+}
 
 } // namespace
 
@@ -380,7 +809,7 @@ void mars_hip_close(mars_hip_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int j = 0; j < kSideStreams; j++)
         if (ctx->side[j]) (void)hipStreamSynchronize(ctx->side[j]);
-    free_tables(ctx);
+    free_batch(ctx);
     (void)hipFree(ctx->d_counters);
     for (int j = 0; j < kSideStreams; j++) {
         if (ctx->side[j]) (void)hipStreamDestroy(ctx->side[j]);
@@ -411,112 +840,17 @@ int mars_hip_prepare(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t 
                      const mars_hip_params *p, const uint32_t *shifts, uint32_t n_shifts)
 {
     if (!ctx || !px || !p || !shifts || n_shifts == 0) return fail(ctx, MARS_HIP_E_ARG, "null argument");
-    if (w == 0 || h == 0 || w > 16384 || h > 16384) return fail(ctx, MARS_HIP_E_ARG, "bad plane size");
-    if (p->bits_alfa < 1 || p->bits_alfa > 16 || p->bits_beta < 1 || p->bits_beta > 16 ||
-        !(p->max_alfa > 0.0) || p->min_size < 1 || p->max_size < p->min_size)
-        return fail(ctx, MARS_HIP_E_ARG, "bad params");
-    for (uint32_t i = 0; i < n_shifts; i++)
-        if (shifts[i] == 0) return fail(ctx, MARS_HIP_E_ARG, "shift must be > 0");
-    HIP_TRY(ctx, hipSetDevice(ctx->device));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    free_tables(ctx);
-    ctx->params = *p;
-    ctx->w = w;
-    ctx->h = h;
-    ctx->wc = w / 2;
-    ctx->hc = h / 2;
-    ctx->pitch = ((ctx->wc + 15) / 16) * 16 + 16;
-    ctx->copy_stride = (size_t)ctx->pitch * std::max<uint32_t>(ctx->hc, 1);
+    const mars_hip_plane one{px, w, h};
+    return prepare_batch(ctx, &one, 1, p, shifts, n_shifts);
+}
 
-    std::map<uint32_t, std::vector<std::pair<uint32_t, uint32_t>>> nodes;
-    uint32_t vsize = 1u << ceil_log2(std::max(w, h));
-    collect(0, 0, vsize, w, h, *p, nodes);
-    for (auto &kv : nodes)
-        if (kv.first > kMaxSize)
-            return fail(ctx, MARS_HIP_E_UNSUPPORTED, "range sizes above 32 are not supported");
-
-    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_px, (size_t)w * h));
-    HIP_TRY(ctx, hipMemcpy(ctx->d_px, px, (size_t)w * h, hipMemcpyHostToDevice));
-    size_t nc = std::max<size_t>((size_t)ctx->wc * ctx->hc, 1);
-    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_d, nc * sizeof(int32_t)));
-    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_df, nc * sizeof(float)));
-    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_lo, 16 * ctx->copy_stride));
-    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_hi, 16 * ctx->copy_stride));
-    HIP_TRY(ctx, hipMemset(ctx->d_lo, 0, 16 * ctx->copy_stride));
-    HIP_TRY(ctx, hipMemset(ctx->d_hi, 0, 16 * ctx->copy_stride));
-    size_t nsat = (size_t)(ctx->wc + 1) * (ctx->hc + 1);
-    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_sat1, nsat * sizeof(int64_t)));
-    HIP_TRY(ctx, hipMalloc((void **)&ctx->d_sat2, nsat * sizeof(int64_t)));
-
-    // Stage-2 constants: exact only when max_alfa is a power of two (then the
-    // alfa scaling of encode.rs:264 is exact in binary64).
-    {
-        int e = 0;
-        double mant = std::frexp(p->max_alfa, &e);
-        ctx->stage2 = (mant == 0.5 && p->bits_alfa <= 8 && p->bits_beta <= 12) ? 1 : 0;
-        ctx->alfa_c = std::ldexp(1.0, (int)p->bits_alfa + 2) / p->max_alfa;
-        ctx->alfa_unit = p->max_alfa / std::ldexp(1.0, (int)p->bits_alfa);
-        uint32_t nq = 1u << p->bits_alfa;
-        std::vector<double> ys(nq);
-        double qbm = (double)((1u << p->bits_beta) - 1);
-        for (uint32_t q = 0; q < nq; q++) {
-            double a2 = (double)q / (double)(1u << p->bits_alfa) * p->max_alfa;
-            ys[q] = qbm / ((1.0 + std::fabs(a2)) * 255.0);
-        }
-        HIP_TRY(ctx, hipMalloc((void **)&ctx->d_yscale, sizeof(double) * nq));
-        HIP_TRY(ctx, hipMemcpy(ctx->d_yscale, ys.data(), sizeof(double) * nq, hipMemcpyHostToDevice));
-    }
-
-    std::vector<uint32_t> sh(shifts, shifts + n_shifts);
-    std::sort(sh.begin(), sh.end());
-    sh.erase(std::unique(sh.begin(), sh.end()), sh.end());
-    for (uint32_t shift : sh) {
-        for (auto it = nodes.rbegin(); it != nodes.rend(); ++it) {
-            Table t;
-            t.size = it->first;
-            t.shift = shift;
-            for (auto &rc : it->second) {
-                t.index[((uint64_t)rc.first << 32) | rc.second] = (uint32_t)t.rows.size();
-                t.rows.push_back(rc.first);
-                t.cols.push_back(rc.second);
-            }
-            t.nr = (uint32_t)t.rows.size();
-            t.nr_pad = (t.nr + 7) / 8 * 8;
-            uint32_t s = t.size, K = s * s;
-            t.kmem = K < 16 ? 16 : K;
-            if (h >= 2 * s && w >= 2 * s) {
-                t.npr = (h - 2 * s) / shift + 1;
-                t.npc = (w - 2 * s) / shift + 1;
-            }
-            t.P = t.npr * t.npc;
-            if (t.P > (0xffffffffu / 8)) return fail(ctx, MARS_HIP_E_UNSUPPORTED, "too many positions");
-            chunking((t.P + 31) / 32, t.nr_pad / 8, &t.nchunks_mfma, &t.tpc_mfma);
-            chunking((t.P + 63) / 64, t.nr_pad / 4, &t.nchunks_fp32, &t.tpc_fp32);
-            uint32_t nch = std::max(t.nchunks_mfma, t.nchunks_fp32);
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_rows, sizeof(uint32_t) * std::max<uint32_t>(t.nr, 1)));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_cols, sizeof(uint32_t) * std::max<uint32_t>(t.nr, 1)));
-            if (t.nr) {
-                HIP_TRY(ctx, hipMemcpy(t.d_rows, t.rows.data(), 4 * t.nr, hipMemcpyHostToDevice));
-                HIP_TRY(ctx, hipMemcpy(t.d_cols, t.cols.data(), 4 * t.nr, hipMemcpyHostToDevice));
-            }
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_rng8, (size_t)t.nr_pad * 8 * t.kmem));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_rngf, (size_t)t.nr_pad / 4 * K * 32 * sizeof(float)));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_rstat, sizeof(int32_t) * 2 * t.nr_pad));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_gthr, sizeof(uint32_t) * t.nr_pad));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_posinfo, sizeof(uint4) * std::max<uint32_t>(t.P, 1)));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_posd, 2 * sizeof(double) * std::max<uint32_t>(t.P, 1)));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_partial, sizeof(Partial) * (size_t)t.nr_pad * nch));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_out, sizeof(mars_hip_candidate) * std::max<uint32_t>(t.nr, 1)));
-            HIP_TRY(ctx, hipMalloc((void **)&t.d_found, std::max<uint32_t>(t.nr, 1)));
-            ctx->tables.push_back(std::move(t));
-        }
-    }
-    int rc = enqueue_step(ctx, ctx->kernel, nullptr);
-    if (rc) return rc;
-    rc = download(ctx);
-    if (rc) return rc;
-    ctx->prepared = true;
-    return MARS_HIP_OK;
+int mars_hip_prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *planes, uint32_t n_planes,
+                           const mars_hip_params *p, const uint32_t *shifts, uint32_t n_shifts)
+{
+    if (!ctx || !planes || !p || !shifts || n_shifts == 0 || n_planes == 0)
+        return fail(ctx, MARS_HIP_E_ARG, "null argument");
+    if (n_planes > kMaxPlanes) return fail(ctx, MARS_HIP_E_ARG, "too many planes");
+    return prepare_batch(ctx, planes, n_planes, p, shifts, n_shifts);
 }
 
 int mars_hip_run(mars_hip_ctx *ctx, int iters, int use_graph, mars_hip_timing *timing)
@@ -524,40 +858,13 @@ int mars_hip_run(mars_hip_ctx *ctx, int iters, int use_graph, mars_hip_timing *t
     if (!ctx || iters < 1) return fail(ctx, MARS_HIP_E_ARG, "bad argument");
     if (!ctx->prepared) return fail(ctx, MARS_HIP_E_NOT_PREPARED, "prepare() first");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const size_t nt = ctx->tables.size();
-    std::vector<hipEvent_t> ev(3 + nt);
-    for (auto &e : ev) HIP_TRY(ctx, hipEventCreate(&e));
     mars_hip_timing tm{};
-    // Per-phase breakdown from one eager step.
-    int rc = enqueue_step(ctx, ctx->kernel, ev.data());
-    if (rc) return rc;
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    float ms = 0;
-    HIP_TRY(ctx, hipEventElapsedTime(&ms, ev[0], ev[1]));
-    tm.ms_pool = ms;
-    for (size_t i = 0; i < nt; i++) {
-        HIP_TRY(ctx, hipEventElapsedTime(&ms, i == 0 ? ev[1] : ev[1 + i], ev[2 + i]));
-        tm.ms_search += ms;
-        uint32_t lg = ceil_log2(ctx->tables[i].size);
-        if (lg >= 1 && lg <= 6) tm.ms_by_size[lg - 1] += ms;
-    }
-    unsigned long long cnt[kCounters] = {};
-    HIP_TRY(ctx, hipMemcpy(cnt, ctx->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
-    for (int i = 0; i < 6; i++) {
-        tm.survivors_by_size[i] = cnt[i + 1];
-        tm.exact_fits_by_size[i] = cnt[8 + i + 1];
-        tm.survivors += cnt[i + 1];
-    }
-    for (const Table &t : ctx->tables) tm.evals += (uint64_t)t.nr * table_evals(t);
+    RC_TRY(timed_step(ctx, &tm, nullptr)); // per-phase breakdown from one eager step
 
-    // Timed steps.
     if (use_graph && (ctx->graph_exec == nullptr || ctx->graph_kernel != ctx->kernel)) {
-        if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-        if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-        ctx->graph_exec = nullptr;
-        ctx->graph = nullptr;
+        drop_graph(ctx);
         HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_step(ctx, ctx->kernel, nullptr);
+        int rc = enqueue_step(ctx, ctx->kernel, nullptr);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (rc) return rc;
@@ -566,23 +873,76 @@ int mars_hip_run(mars_hip_ctx *ctx, int iters, int use_graph, mars_hip_timing *t
         HIP_TRY(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
         ctx->graph_kernel = ctx->kernel;
     }
-    HIP_TRY(ctx, hipEventRecord(ev[0], ctx->stream));
+    hipEvent_t e0, e1;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    HIP_TRY(ctx, hipEventCreate(&e1));
+    HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
     for (int it = 0; it < iters; it++) {
         if (use_graph) {
             HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
         } else {
-            rc = enqueue_step(ctx, ctx->kernel, nullptr);
-            if (rc) return rc;
+            RC_TRY(enqueue_step(ctx, ctx->kernel, nullptr));
         }
     }
-    HIP_TRY(ctx, hipEventRecord(ev[1], ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    HIP_TRY(ctx, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    float ms = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
     tm.ms_step = ms / iters;
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    rc = download(ctx);
-    if (rc) return rc;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    RC_TRY(download(ctx));
     if (timing) *timing = tm;
+    return MARS_HIP_OK;
+}
+
+int mars_hip_autotune(mars_hip_ctx *ctx, uint32_t reps, mars_hip_tuning *out)
+{
+    if (!ctx || reps < 1) return fail(ctx, MARS_HIP_E_ARG, "bad argument");
+    if (!ctx->prepared) return fail(ctx, MARS_HIP_E_NOT_PREPARED, "prepare() first");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    // Every candidate shape is applied to all sizes at once; each size keeps
+    // the shape whose summed search time (its groups, median of reps) is
+    // lowest.
+    double best[6];
+    Shape chosen[6];
+    for (int i = 0; i < 6; i++) {
+        best[i] = 1e30;
+        chosen[i] = ctx->shape[i];
+    }
+    for (uint32_t target : kShapeTargets)
+        for (uint32_t mt : kShapeMinTiles) {
+            for (int i = 0; i < 6; i++) ctx->shape[i] = Shape{target, mt};
+            RC_TRY(layout(ctx));
+            std::vector<std::vector<double>> samples(6);
+            for (uint32_t r = 0; r < reps; r++) {
+                std::vector<double> gms;
+                RC_TRY(timed_step(ctx, nullptr, &gms));
+                double by[6] = {};
+                for (size_t g = 0; g < gms.size(); g++) by[ceil_log2(ctx->groups[g].size) - 1] += gms[g];
+                for (int i = 0; i < 6; i++) samples[i].push_back(by[i]);
+            }
+            for (int i = 0; i < 6; i++) {
+                std::sort(samples[i].begin(), samples[i].end());
+                const double med = samples[i][samples[i].size() / 2];
+                if (med < best[i]) {
+                    best[i] = med;
+                    chosen[i] = Shape{target, mt};
+                }
+            }
+        }
+    for (int i = 0; i < 6; i++) ctx->shape[i] = chosen[i];
+    RC_TRY(layout(ctx));
+    RC_TRY(enqueue_step(ctx, ctx->kernel, nullptr));
+    RC_TRY(download(ctx));
+    if (out) {
+        *out = mars_hip_tuning{};
+        for (int i = 0; i < 6; i++) {
+            out->target_blocks[i] = chosen[i].target_blocks;
+            out->min_tiles[i] = chosen[i].min_tiles;
+            out->ms[i] = best[i] < 1e29 ? best[i] : 0.0;
+        }
+    }
     return MARS_HIP_OK;
 }
 
@@ -590,28 +950,14 @@ int mars_hip_search(const mars_hip_ctx *ctx, uint32_t row, uint32_t col, uint32_
                     uint32_t shift, uint32_t k, mars_hip_candidate *out, uint32_t *n_out,
                     uint64_t *evals)
 {
-    mars_hip_ctx *mctx = const_cast<mars_hip_ctx *>(ctx);
-    if (!ctx || !out || !n_out || !evals) return fail(mctx, MARS_HIP_E_ARG, "null argument");
-    if (k != 1) return fail(mctx, MARS_HIP_E_UNSUPPORTED, "only k == 1 is supported");
-    if (!ctx->prepared) return fail(mctx, MARS_HIP_E_NOT_PREPARED, "prepare() first");
-    for (const Table &t : ctx->tables) {
-        if (t.size != size || t.shift != shift) continue;
-        auto it = t.index.find(((uint64_t)row << 32) | col);
-        if (it == t.index.end()) break;
-        uint32_t i = it->second;
-        *evals = table_evals(t);
-        if (t.h_found[i]) {
-            *out = t.h_out[i];
-            *n_out = 1;
-        } else {
-            *n_out = 0;
-        }
-        return MARS_HIP_OK;
-    }
-    char buf[160];
-    snprintf(buf, sizeof buf, "range (%u,%u) size %u shift %u was not prepared", row, col, size,
-             shift);
-    return fail(mctx, MARS_HIP_E_NOT_IN_TABLE, buf);
+    return lookup(ctx, 0, row, col, size, shift, k, out, n_out, evals);
+}
+
+int mars_hip_search_plane(const mars_hip_ctx *ctx, uint32_t plane, uint32_t row, uint32_t col,
+                          uint32_t size, uint32_t shift, uint32_t k, mars_hip_candidate *out,
+                          uint32_t *n_out, uint64_t *evals)
+{
+    return lookup(ctx, plane, row, col, size, shift, k, out, n_out, evals);
 }
 
 int mars_hip_table_count(const mars_hip_ctx *ctx, uint32_t *n_tables)
@@ -630,6 +976,13 @@ int mars_hip_table_info(const mars_hip_ctx *ctx, uint32_t i, uint32_t *size, uin
     if (shift) *shift = t.shift;
     if (n_ranges) *n_ranges = t.nr;
     if (n_positions) *n_positions = t.P;
+    return MARS_HIP_OK;
+}
+
+int mars_hip_table_plane(const mars_hip_ctx *ctx, uint32_t i, uint32_t *plane)
+{
+    if (!ctx || !plane || i >= ctx->tables.size()) return MARS_HIP_E_ARG;
+    *plane = ctx->tables[i].plane;
     return MARS_HIP_OK;
 }
 
@@ -699,14 +1052,6 @@ int mars_hip_copy_probe(mars_hip_ctx *ctx, size_t bytes, double *ms)
     return MARS_HIP_OK;
 }
 
-static int table_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size,
-                           uint32_t shift, mars_hip_candidate *out, uint32_t *n_out,
-                           uint64_t *evals)
-{
-    return mars_hip_search((const mars_hip_ctx *)user, row, col, size, shift, 1, out, n_out,
-                           evals);
-}
-
 int mars_hip_encode_plane(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t h,
                           const mars_encode_params *p, const mars_encode_options *o,
                           uint8_t **out, size_t *out_len, mars_encode_stats *stats)
@@ -723,11 +1068,29 @@ int mars_hip_encode_plane(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint
     // adaptive density is on (encode.rs:432-437, RD path only).
     uint32_t shifts[2] = {p->shift, 2 * p->shift};
     const uint32_t ns = (o->adaptive_density && p->has_lambda) ? 2 : 1;
-    int rc = mars_hip_prepare(ctx, px, w, h, &hp, shifts, ns);
-    if (rc) return rc;
-    rc = mars_encode_plane(px, w, h, p, o, table_search_cb, ctx, table_search_cb, ctx, out,
-                           out_len, stats);
-    if (rc && ctx->err.empty()) return fail(ctx, rc, "encode failed");
+    RC_TRY(mars_hip_prepare(ctx, px, w, h, &hp, shifts, ns));
+    return mars_hip_encode_prepared(ctx, 0, p, o, out, out_len, stats);
+}
+
+int mars_hip_encode_prepared(mars_hip_ctx *ctx, uint32_t plane, const mars_encode_params *p,
+                             const mars_encode_options *o, uint8_t **out, size_t *out_len,
+                             mars_encode_stats *stats)
+{
+    if (!ctx || !p || !o) return fail(ctx, MARS_HIP_E_ARG, "null argument");
+    if (!ctx->prepared) return fail(ctx, MARS_HIP_E_NOT_PREPARED, "prepare() first");
+    if (plane >= ctx->planes.size()) return fail(ctx, MARS_HIP_E_ARG, "no such plane");
+    const mars_hip_params &hp = ctx->params;
+    if (p->min_size != hp.min_size || p->max_size != hp.max_size || p->bits_alfa != hp.bits_alfa ||
+        p->bits_beta != hp.bits_beta || p->max_alfa != hp.max_alfa)
+        return fail(ctx, MARS_HIP_E_ARG, "encode params differ from the prepared search params");
+    const Plane &pl = ctx->planes[plane];
+    PlaneRef ref{ctx, plane};
+    int rc = mars_encode_plane(pl.host.data(), pl.w, pl.h, p, o, plane_search_cb, &ref,
+                               plane_search_cb, &ref, out, out_len, stats);
+    if (rc) {
+        std::lock_guard<std::mutex> lk(ctx->err_mu);
+        if (ctx->err.empty()) ctx->err = "encode failed";
+    }
     return rc;
 }
 

@@ -1,5 +1,11 @@
 // Shared declarations between the device code (kernels.hip) and the host
 // side of the C ABI (api.hip).
+//
+// A step covers a batch of planes.  Each plane has a pool (contracted plane,
+// split i8 planes, summed-area tables); each (plane, range size, stride)
+// has a search table.  Kernels take arrays of plane or table descriptors in
+// device memory plus a flat work list, so one launch covers every plane (and
+// every table of a size) of the batch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,67 +23,74 @@ struct Partial {
     int32_t t1;   // exact t1_x4 of the winner (MFMA kernel), 0 otherwise
 };
 
-struct SearchArgs {
-    const int8_t *rng8;      // [nr_pad][8][kmem] r - 128
-    const float *rngf;       // [nr_pad/4][K][32] r (FP32 baseline)
-    const int32_t *rstat;    // [nr_pad][2] t0, t2
-    uint32_t *gthr;          // [nr_pad] pruning threshold (float bits, SSE units)
-    const uint4 *posinfo;    // [P] {dr | dc << 16, S1, S2, (float)(n*S2 - S1^2)}
-    const double *posd;      // [P][2] Di = n*S2 - S1^2, 1/Di (binary64, stage 2)
-    const int8_t *lo, *hi;   // split planes, 16 shifted copies each
-    size_t copy_stride;      // bytes between copies
-    uint32_t pitch;          // bytes per contracted row in a copy
-    const float *dplanef;    // contracted plane as f32 (FP32 baseline)
-    uint32_t wc;             // contracted width
-    uint32_t P, npc, shift;  // positions, positions per row, stride
-    uint32_t nr, ngroups, nchunks, tiles_per_chunk;
-    double max_alfa;
-    uint32_t bits_alfa, bits_beta;
-    Partial *partial;        // [nr_pad][nchunks]
-    unsigned long long *counters;
-    uint32_t counter_slot;   // log2(size): counters[slot] stage-1 survivors, [8+slot] stage 3
-    // Stage-2 constants (valid when stage2 != 0: max_alfa is a power of two).
-    int stage2;
-    double alfa_c;           // 2^(bits_alfa+2) / max_alfa: x = alfa_c * Ni / Di
-    double alfa_unit;        // max_alfa / 2^bits_alfa: alfa2 = qalfa * alfa_unit
-    double qa_max, qb_max;   // 2^bits_alfa - 1, 2^bits_beta - 1
-    const double *yscale;    // [2^bits_alfa] (2^bits_beta-1) / ((1 + alfa2) * 255)
+// One plane's pool.
+struct PlaneDesc {
+    const uint8_t *px;
+    int32_t *d;          // contracted plane (i32)
+    float *df;           // contracted plane (f32, FP32 baseline)
+    int8_t *lo, *hi;     // split planes, 16 byte-shifted copies each
+    int64_t *sat1, *sat2;
+    uint32_t w, h, wc, hc, pitch, copy_stride;
 };
 
-struct MergeArgs {
-    const Partial *partial;
-    int t1_exact;
-    uint32_t nchunks, nr, P, npc, shift, size;
-    const uint32_t *rows, *cols;
+// One (plane, range size, stride) search table.
+struct TableDesc {
+    // the plane
     const uint8_t *px;
-    uint32_t w;
     const int32_t *d;
-    uint32_t wc;
-    const uint4 *posinfo;
-    const int32_t *rstat;
-    double max_alfa;
-    uint32_t bits_alfa, bits_beta;
+    const float *df;
+    const int8_t *lo, *hi;
+    const int64_t *sat1, *sat2;
+    uint32_t w, wc, pitch, copy_stride;
+    // the table
+    const uint32_t *rows, *cols;
+    int8_t *rng8;        // [nr_pad][8][kmem] r - 128
+    float *rngf;         // [nr_pad/4][K][32] r (FP32 baseline)
+    int32_t *rstat;      // [nr_pad][2] t0, t2
+    uint32_t *gthr;      // [nr_pad] pruning threshold (float bits, SSE units)
+    uint4 *posinfo;      // [P] {dr | dc << 16, S1, S2, (float)(n*S2 - S1^2)}
+    double *posd;        // [P][2] Di = n*S2 - S1^2, 1/Di (binary64, stage 2)
+    Partial *partial;    // [nr_pad][nchunks]
     mars_hip_candidate *out;
     uint8_t *found;
+    uint32_t size, shift, P, npc, nr, nr_pad, kmem;
+    // Position chunking of the search: [0] MFMA kernel (32-position
+    // tiles), [1] FP32 baseline (64-position tiles).  partial holds
+    // nr_pad x nchunks[kernel] entries.
+    uint32_t nchunks[2], tiles_per_chunk[2];
 };
 
-int launch_search(int kernel, uint32_t size, const SearchArgs &a, hipStream_t st);
-void launch_contract(const uint8_t *px, uint32_t w, uint32_t wc, uint32_t hc, uint32_t pitch,
-                     size_t copy_stride, int32_t *d, float *df, int8_t *lo, int8_t *hi,
+// Step-wide constants.
+struct Common {
+    double max_alfa;
+    uint32_t bits_alfa, bits_beta;
+    // Stage-2 constants (valid when stage2 != 0: max_alfa is a power of two).
+    int stage2;
+    double alfa_c;        // 2^(bits_alfa+2) / max_alfa: x = alfa_c * Ni / Di
+    double alfa_unit;     // max_alfa / 2^bits_alfa: alfa2 = qalfa * alfa_unit
+    double qa_max, qb_max; // 2^bits_alfa - 1, 2^bits_beta - 1
+    const double *yscale; // [2^bits_alfa] (2^bits_beta-1) / ((1 + alfa2) * 255)
+    unsigned long long *counters; // [16]: [log2 s] stage-1 survivors, [8 + log2 s] stage-3 fits
+};
+
+// Work items: x = descriptor index, y/z = kernel-specific block coordinates.
+typedef uint2 Work2;
+typedef uint3 Work3;
+
+void launch_contract(const PlaneDesc *planes, const Work2 *work, uint32_t nwork,
                      unsigned long long *counters, uint32_t n_counters, hipStream_t st);
-void launch_sat(const int32_t *d, uint32_t wc, uint32_t hc, int64_t *sat1, int64_t *sat2,
-                hipStream_t st);
-void launch_possums(const int64_t *sat1, const int64_t *sat2, uint32_t wc, uint32_t size,
-                    uint32_t shift, uint32_t npc, uint32_t P, uint4 *posinfo, double *posd,
-                    hipStream_t st);
-void launch_seed(const int8_t *rng8, const int32_t *rstat, const int32_t *d, uint32_t wc,
-                 uint32_t size, uint32_t kmem, uint32_t npc, uint32_t P, const uint4 *posinfo,
-                 uint32_t nr, double max_alfa, uint32_t bits_alfa, uint32_t bits_beta,
-                 uint32_t *gthr, hipStream_t st);
-void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const uint32_t *cols,
-                   uint32_t nr, uint32_t nr_pad, uint32_t size, uint32_t kmem, int8_t *rng8,
-                   float *rngf, int32_t *rstat, uint32_t *gthr, hipStream_t st);
-void launch_merge(const MergeArgs &a, hipStream_t st);
+void launch_sat(const PlaneDesc *planes, const Work2 *rows_work, uint32_t n_rows_work,
+                const Work2 *cols_work, uint32_t n_cols_work, hipStream_t st);
+void launch_possums(const TableDesc *t, const Work2 *work, uint32_t nwork, hipStream_t st);
+void launch_ranges(const TableDesc *t, const Work2 *work, uint32_t nwork, bool fp32,
+                   hipStream_t st);
+void launch_rstats(const TableDesc *t, const Work2 *work, uint32_t nwork, hipStream_t st);
+void launch_seed(const TableDesc *t, const Work2 *work, uint32_t nwork, const Common &c,
+                 hipStream_t st);
+int launch_search(int kernel, uint32_t size, const TableDesc *t, const Work3 *work,
+                  uint32_t nwork, const Common &c, hipStream_t st);
+void launch_merge(const TableDesc *t, const Work2 *work, uint32_t nwork, bool t1_exact,
+                  const Common &c, hipStream_t st);
 void launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t st);
 void launch_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
                          uint32_t bits_beta, uint32_t *qa, uint32_t *qb, double *rms,

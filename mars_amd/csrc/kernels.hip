@@ -36,6 +36,123 @@ namespace mars {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
+// Global-memory pointers.  Pointers read out of a descriptor in device
+// memory are generic to the compiler, which then emits flat_* accesses (they
+// also count against the LDS wait counter); the device views below carry
+// address space 1 so every descriptor access compiles to global_*.
+#define GLB __attribute__((address_space(1)))
+
+template <class X> __device__ __forceinline__ GLB X *glb(X *p) { return (GLB X *)p; }
+
+// Class types (uint4, Partial, mars_hip_candidate) cannot be copied to or
+// from address space 1 in C++, so the views hold them as plain vectors and
+// words and these helpers move them.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 ld_u4(const GLB u32x4 *p, size_t i)
+{
+    const u32x4 v = p[i];
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ Partial ld_partial(const GLB Partial *p, size_t i)
+{
+    Partial e;
+    e.rms = p[i].rms;
+    e.key = p[i].key;
+    e.t1 = p[i].t1;
+    return e;
+}
+__device__ __forceinline__ void st_partial(GLB Partial *p, size_t i, double rms, uint32_t key,
+                                           int32_t t1)
+{
+    p[i].rms = rms;
+    p[i].key = key;
+    p[i].t1 = t1;
+}
+__device__ __forceinline__ void st_cand(GLB mars_hip_candidate *p, size_t i,
+                                        const mars_hip_candidate &c)
+{
+    static_assert(sizeof(mars_hip_candidate) % 8 == 0, "candidate is whole words");
+    GLB uint64_t *d = (GLB uint64_t *)(p + i);
+    const uint64_t *w = (const uint64_t *)&c;
+#pragma unroll
+    for (size_t k = 0; k < sizeof(mars_hip_candidate) / 8; k++) d[k] = w[k];
+}
+
+struct PlaneG {
+    const GLB uint8_t *px;
+    GLB int32_t *d;
+    GLB float *df;
+    GLB int8_t *lo, *hi;
+    GLB int64_t *sat1, *sat2;
+    uint32_t w, h, wc, hc, pitch, copy_stride;
+};
+
+__device__ __forceinline__ PlaneG view(const PlaneDesc &p)
+{
+    return PlaneG{glb(p.px), glb(p.d),  glb(p.df), glb(p.lo), glb(p.hi),    glb(p.sat1), glb(p.sat2),
+                  p.w,       p.h,       p.wc,      p.hc,      p.pitch,      p.copy_stride};
+}
+
+struct TableG {
+    const GLB uint8_t *px;
+    const GLB int32_t *d;
+    const GLB float *df;
+    const GLB int8_t *lo, *hi;
+    const GLB int64_t *sat1, *sat2;
+    uint32_t w, wc, pitch, copy_stride;
+    const GLB uint32_t *rows, *cols;
+    GLB int8_t *rng8;
+    GLB float *rngf;
+    GLB int32_t *rstat;
+    GLB uint32_t *gthr;
+    GLB u32x4 *posinfo;
+    GLB double *posd;
+    GLB Partial *partial;
+    GLB mars_hip_candidate *out;
+    GLB uint8_t *found;
+    uint32_t size, shift, P, npc, nr, nr_pad, kmem;
+    uint32_t nchunks[2], tiles_per_chunk[2];
+};
+
+__device__ __forceinline__ TableG view(const TableDesc &t)
+{
+    TableG g;
+    g.px = glb(t.px);
+    g.d = glb(t.d);
+    g.df = glb(t.df);
+    g.lo = glb(t.lo);
+    g.hi = glb(t.hi);
+    g.sat1 = glb(t.sat1);
+    g.sat2 = glb(t.sat2);
+    g.w = t.w;
+    g.wc = t.wc;
+    g.pitch = t.pitch;
+    g.copy_stride = t.copy_stride;
+    g.rows = glb(t.rows);
+    g.cols = glb(t.cols);
+    g.rng8 = glb(t.rng8);
+    g.rngf = glb(t.rngf);
+    g.rstat = glb(t.rstat);
+    g.gthr = glb(t.gthr);
+    g.posinfo = (GLB u32x4 *)t.posinfo;
+    g.posd = glb(t.posd);
+    g.partial = glb(t.partial);
+    g.out = glb(t.out);
+    g.found = glb(t.found);
+    g.size = t.size;
+    g.shift = t.shift;
+    g.P = t.P;
+    g.npc = t.npc;
+    g.nr = t.nr;
+    g.nr_pad = t.nr_pad;
+    g.kmem = t.kmem;
+    g.nchunks[0] = t.nchunks[0];
+    g.nchunks[1] = t.nchunks[1];
+    g.tiles_per_chunk[0] = t.tiles_per_chunk[0];
+    g.tiles_per_chunk[1] = t.tiles_per_chunk[1];
+    return g;
+}
+
 // ---------------------------------------------------------------- fits
 
 // encode.rs:446-448 quantise_f64
@@ -164,44 +281,47 @@ __device__ __forceinline__ void iso_map(uint32_t k, uint32_t u, uint32_t v, uint
 
 // ---------------------------------------------------------------- pool
 
-// One thread per contracted sample.  Writes D (i32), D as f32, and the
-// i8 planes lo = (D & 255) - 128, hi = D >> 8 into copies c = 0..15 at
-// x = j - c (copy c holds the row shifted left by c bytes).
-__global__ void k_contract(const uint8_t *__restrict__ px, uint32_t w, uint32_t wc, uint32_t hc,
-                           uint32_t pitch, size_t copy_stride, int32_t *__restrict__ d,
-                           float *__restrict__ df, int8_t *__restrict__ lo,
-                           int8_t *__restrict__ hi, unsigned long long *__restrict__ counters,
-                           uint32_t n_counters)
+// Work item (plane, block): one thread per contracted sample.  Writes D
+// (i32), D as f32, and the i8 planes lo = (D & 255) - 128, hi = D >> 8 into
+// copies c = 0..15 at x = j - c (copy c holds the row shifted left by c
+// bytes).  Block 0 of the launch also clears the step's counters.
+__global__ void k_contract(const PlaneDesc *__restrict__ planes, const Work2 *__restrict__ work,
+                           unsigned long long *__restrict__ counters, uint32_t n_counters)
 {
-    uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < n_counters) counters[idx] = 0;
-    if (idx >= wc * hc) return;
-    uint32_t i = idx / wc, j = idx - i * wc;
-    const uint8_t *p0 = px + (size_t)(2 * i) * w + 2 * j;
-    int32_t v = (int32_t)p0[0] + (int32_t)p0[1] + (int32_t)p0[w] + (int32_t)p0[w + 1];
-    d[idx] = v;
-    df[idx] = (float)v;
+    if (blockIdx.x == 0 && threadIdx.x < n_counters) glb(counters)[threadIdx.x] = 0;
+    const Work2 wk = work[blockIdx.x];
+    const PlaneG pl = view(planes[wk.x]);
+    const uint32_t idx = wk.y * blockDim.x + threadIdx.x;
+    if (idx >= pl.wc * pl.hc) return;
+    const uint32_t i = idx / pl.wc, j = idx - i * pl.wc;
+    const GLB uint8_t *p0 = pl.px + (size_t)(2 * i) * pl.w + 2 * j;
+    int32_t v = (int32_t)p0[0] + (int32_t)p0[1] + (int32_t)p0[pl.w] + (int32_t)p0[pl.w + 1];
+    pl.d[idx] = v;
+    pl.df[idx] = (float)v;
     int8_t l = (int8_t)((v & 255) - 128);
     int8_t hh = (int8_t)(v >> 8);
 #pragma unroll
     for (uint32_t c = 0; c < 16; c++) {
         if (j >= c) {
-            size_t o = c * copy_stride + (size_t)i * pitch + (j - c);
-            lo[o] = l;
-            hi[o] = hh;
+            size_t o = c * (size_t)pl.copy_stride + (size_t)i * pl.pitch + (j - c);
+            pl.lo[o] = l;
+            pl.hi[o] = hh;
         }
     }
 }
 
 // Summed-area tables of D and D^2: sat[(i)(wc+1) + j] = sum over rows < i,
-// columns < j.  Pass 1 (one block per contracted row, plus block hc for the
-// zero row): inclusive row prefix sums by a block-wide scan.  Pass 2 (one
-// thread per column): running sums down the columns.
-__global__ void k_sat_rows(const int32_t *__restrict__ d, uint32_t wc, uint32_t hc,
-                           int64_t *__restrict__ sat1, int64_t *__restrict__ sat2)
+// columns < j.  Pass 1, work item (plane, row i; i = hc is the zero row):
+// inclusive row prefix sums by a block-wide scan.  Pass 2, work item (plane,
+// block of 16 columns): running sums down the columns.
+__global__ void k_sat_rows(const PlaneDesc *__restrict__ planes, const Work2 *__restrict__ work)
 {
     __shared__ int64_t s1[256], s2[256];
-    const uint32_t i = blockIdx.x, t = threadIdx.x, W1 = wc + 1;
+    const Work2 wk = work[blockIdx.x];
+    const PlaneG pl = view(planes[wk.x]);
+    const uint32_t wc = pl.wc, hc = pl.hc;
+    GLB int64_t *sat1 = pl.sat1, *sat2 = pl.sat2;
+    const uint32_t i = wk.y, t = threadIdx.x, W1 = wc + 1;
     if (i == hc) { // the zero row 0
         for (uint32_t j = t; j < W1; j += 256) {
             sat1[j] = 0;
@@ -209,7 +329,7 @@ __global__ void k_sat_rows(const int32_t *__restrict__ d, uint32_t wc, uint32_t 
         }
         return;
     }
-    int64_t *o1 = sat1 + (size_t)(i + 1) * W1, *o2 = sat2 + (size_t)(i + 1) * W1;
+    GLB int64_t *o1 = sat1 + (size_t)(i + 1) * W1, *o2 = sat2 + (size_t)(i + 1) * W1;
     if (t == 0) {
         o1[0] = 0;
         o2[0] = 0;
@@ -217,7 +337,7 @@ __global__ void k_sat_rows(const int32_t *__restrict__ d, uint32_t wc, uint32_t 
     int64_t c1 = 0, c2 = 0;
     for (uint32_t base = 0; base < wc; base += 256) {
         uint32_t j = base + t;
-        int64_t x = j < wc ? d[(size_t)i * wc + j] : 0;
+        int64_t x = j < wc ? pl.d[(size_t)i * wc + j] : 0;
         s1[t] = x;
         s2[t] = x * x;
         __syncthreads();
@@ -241,12 +361,15 @@ __global__ void k_sat_rows(const int32_t *__restrict__ d, uint32_t wc, uint32_t 
 // Column pass: a block covers 16 columns x 16 row segments; each thread
 // scans its segment, the segment totals are combined in LDS, and a second
 // sweep adds the preceding segments' total.
-__global__ void k_sat_cols(uint32_t wc, uint32_t hc, int64_t *__restrict__ sat1,
-                           int64_t *__restrict__ sat2)
+__global__ void k_sat_cols(const PlaneDesc *__restrict__ planes, const Work2 *__restrict__ work)
 {
     __shared__ int64_t tot1[16][16], tot2[16][16];
+    const Work2 wk = work[blockIdx.x];
+    const PlaneG pl = view(planes[wk.x]);
+    const uint32_t wc = pl.wc, hc = pl.hc;
+    GLB int64_t *sat1 = pl.sat1, *sat2 = pl.sat2;
     const uint32_t cl = threadIdx.x & 15, seg = threadIdx.x >> 4, W1 = wc + 1;
-    const uint32_t j = blockIdx.x * 16 + cl;
+    const uint32_t j = wk.y * 16 + cl;
     const uint32_t L = (hc + 15) / 16, r0 = 1 + seg * L;
     uint32_t r1 = r0 + L;
     if (r1 > hc + 1) r1 = hc + 1;
@@ -273,50 +396,52 @@ __global__ void k_sat_cols(uint32_t wc, uint32_t hc, int64_t *__restrict__ sat1,
         }
 }
 
-// One thread per domain position of a table: window sums from the SATs,
-// packed as {dr | dc << 16, S1, S2, (float)Di} plus {Di, 1/Di} in binary64.
-__global__ void k_possums(const int64_t *__restrict__ sat1, const int64_t *__restrict__ sat2,
-                          uint32_t wc, uint32_t size, uint32_t shift, uint32_t npc, uint32_t P,
-                          uint4 *__restrict__ posinfo, double *__restrict__ posd)
+// Work item (table, block of 256 domain positions): window sums from the
+// SATs, packed as {dr | dc << 16, S1, S2, (float)Di} plus {Di, 1/Di} in
+// binary64.
+__global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    const uint32_t W1 = wc + 1;
-    uint32_t pi = p / npc, pj = p - pi * npc;
-    uint32_t dr = (pi * shift) >> 1, dc = (pj * shift) >> 1;
-    size_t a = (size_t)dr * W1 + dc, b = (size_t)(dr + size) * W1 + dc;
-    int64_t s1 = sat1[b + size] - sat1[a + size] - sat1[b] + sat1[a];
-    int64_t s2 = sat2[b + size] - sat2[a + size] - sat2[b] + sat2[a];
-    int64_t n = (int64_t)size * size;
-    int64_t di = n * s2 - s1 * s1;
-    posinfo[p] = make_uint4(dr | (dc << 16), (uint32_t)s1, (uint32_t)s2, __float_as_uint((float)di));
-    posd[2 * p] = (double)di;
-    posd[2 * p + 1] = di > 0 ? 1.0 / (double)di : 0.0;
+    const Work2 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t p = wk.y * blockDim.x + threadIdx.x;
+    if (p >= T.P) return;
+    const uint32_t W1 = T.wc + 1, size = T.size;
+    const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
+    const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
+    const size_t a = (size_t)dr * W1 + dc, b = (size_t)(dr + size) * W1 + dc;
+    const int64_t s1 = T.sat1[b + size] - T.sat1[a + size] - T.sat1[b] + T.sat1[a];
+    const int64_t s2 = T.sat2[b + size] - T.sat2[a + size] - T.sat2[b] + T.sat2[a];
+    const int64_t n = (int64_t)size * size;
+    const int64_t di = n * s2 - s1 * s1;
+    T.posinfo[p] = (u32x4){dr | (dc << 16), (uint32_t)s1, (uint32_t)s2, __float_as_uint((float)di)};
+    T.posd[2 * p] = (double)di;
+    T.posd[2 * p + 1] = di > 0 ? 1.0 / (double)di : 0.0;
 }
 
-// Threshold seed: one 256-thread block per range fits 64 spread candidates
-// (8 positions x 8 isometries; 32 = 4 x 8 at S >= 32), each candidate's
-// correlation split over 256/candidates threads.  The smallest inflated SSE becomes the
-// range's starting threshold.
-__global__ void k_seed(const int8_t *__restrict__ rng8, const int32_t *__restrict__ rstat,
-                       const int32_t *__restrict__ d, uint32_t wc, uint32_t size, uint32_t kmem,
-                       uint32_t npc, uint32_t P, const uint4 *__restrict__ posinfo,
-                       double max_alfa, uint32_t bits_alfa, uint32_t bits_beta,
-                       uint32_t *__restrict__ gthr)
+// Threshold seed, work item (table, range): a 256-thread block fits 64
+// spread candidates (8 positions x 8 isometries; 32 = 4 x 8 at S >= 32), each
+// candidate's correlation split over 256/candidates threads.  The smallest
+// inflated SSE becomes the range's starting threshold.
+__global__ void k_seed(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work,
+                       Common cm)
 {
     __shared__ uint32_t s_min;
     __shared__ int32_t s_t1[256];
-    const uint32_t K = size * size;
+    const Work2 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t size = T.size, K = size * size, P = T.P;
     const uint32_t ncand = size >= 32 ? 32 : 64, q = 256 / ncand, npos = ncand / 8;
-    const uint32_t r = blockIdx.x, t = threadIdx.x, c = t % ncand, sl = t / ncand;
+    const uint32_t r = wk.y, t = threadIdx.x, c = t % ncand, sl = t / ncand;
     if (t == 0) s_min = 0x7f800000u;
     s_t1[t] = 0;
     __syncthreads();
     const uint32_t j = c >> 3, iso = c & 7;
     const uint32_t p = P >= npos ? (uint32_t)(((uint64_t)j * (P - 1)) / (npos - 1)) : j % P;
-    const uint4 pin = posinfo[p];
+    const uint4 pin = ld_u4(T.posinfo, p);
     const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
-    const int8_t *rk = rng8 + ((size_t)r * 8 + iso) * kmem;
+    const GLB int8_t *rk = T.rng8 + ((size_t)r * 8 + iso) * T.kmem;
+    const GLB int32_t *d = T.d;
+    const uint32_t wc = T.wc;
     const uint32_t k0 = sl * K / q, k1 = (sl + 1) * K / q;
     int32_t t1 = 0;
 #pragma unroll 8
@@ -330,29 +455,31 @@ __global__ void k_seed(const int8_t *__restrict__ rng8, const int32_t *__restric
         const int64_t n = (int64_t)K;
         uint32_t qa, qb;
         double rms, sum;
-        fit_f64(n, (int32_t)pin.y, (int32_t)pin.z, rstat[2 * r], s_t1[c], rstat[2 * r + 1],
-                max_alfa, bits_alfa, bits_beta, &qa, &qb, &rms, &sum);
+        fit_f64(n, (int32_t)pin.y, (int32_t)pin.z, T.rstat[2 * r], s_t1[c], T.rstat[2 * r + 1],
+                cm.max_alfa, cm.bits_alfa, cm.bits_beta, &qa, &qb, &rms, &sum);
         double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
         atomicMin(&s_min, __float_as_uint((float)(tau * (1.0 + 1e-7))));
     }
     __syncthreads();
-    if (t == 0) gthr[r] = s_min;
+    if (t == 0) T.gthr[r] = s_min;
 }
 
-// Range operands: one thread per 4 bytes of rng8, layout [range][iso][kmem]
-// (i8 r-128, zero padded to kmem); rngf (FP32 baseline only, may be null)
-// layout [range/4][k][iso*4 + range%4] (f32 r).  lg_size/lg_kmem: log2.
-__global__ void k_ranges(const uint8_t *__restrict__ px, uint32_t w,
-                         const uint32_t *__restrict__ rows, const uint32_t *__restrict__ cols,
-                         uint32_t nr, uint32_t nr_pad, uint32_t lg_size, uint32_t lg_kmem,
-                         int8_t *__restrict__ rng8, float *__restrict__ rngf)
+// Range operands, work item (table, block): one thread per 4 bytes of
+// rng8, layout [range][iso][kmem] (i8 r-128, zero padded to kmem); with
+// `fp32` also rngf, layout [range/4][k][iso*4 + range%4] (f32 r).
+__global__ void k_ranges(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work,
+                         int fp32)
 {
-    const uint32_t e = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
-    const uint32_t size = 1u << lg_size, K = size * size, kmem = 1u << lg_kmem;
-    if (e >= nr_pad * 8 * kmem) return;
+    const Work2 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t e = (wk.y * blockDim.x + threadIdx.x) * 4;
+    const uint32_t lg_size = __builtin_ctz(T.size), lg_kmem = __builtin_ctz(T.kmem);
+    const uint32_t size = T.size, K = size * size, kmem = T.kmem;
+    if (e >= T.nr_pad * 8 * kmem) return;
     const uint32_t r = e >> (lg_kmem + 3), iso = (e >> lg_kmem) & 7, k0 = e & (kmem - 1);
-    const bool valid = r < nr;
-    const uint32_t row = valid ? rows[r] : 0, col = valid ? cols[r] : 0;
+    const bool valid = r < T.nr;
+    const uint32_t row = valid ? T.rows[r] : 0, col = valid ? T.cols[r] : 0;
+    GLB float *rngf = fp32 ? T.rngf : nullptr;
     uint32_t packed = 0;
 #pragma unroll
     for (uint32_t b = 0; b < 4; b++) {
@@ -361,31 +488,30 @@ __global__ void k_ranges(const uint8_t *__restrict__ px, uint32_t w,
             const uint32_t u = k >> lg_size, v = k & (size - 1);
             uint32_t i, j;
             iso_map(iso, u, v, size, &i, &j);
-            const uint32_t pix = px[(row + i) * w + col + j];
+            const uint32_t pix = T.px[(row + i) * T.w + col + j];
             packed |= ((pix - 128u) & 0xffu) << (8 * b);
             if (rngf) rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = (float)pix;
         } else if (rngf && k < K) {
             rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = 0.0f;
         }
     }
-    *(uint32_t *)(rng8 + e) = packed;
+    *(uint32_t *)(T.rng8 + e) = packed;
 }
 
-// Range sums t0 = sum r, t2 = sum r^2: one wave per (padded) range.  Also
-// resets the range's pruning threshold.
-__global__ void k_rstats(const uint8_t *__restrict__ px, uint32_t w,
-                         const uint32_t *__restrict__ rows, const uint32_t *__restrict__ cols,
-                         uint32_t nr, uint32_t nr_pad, uint32_t lg_size,
-                         int32_t *__restrict__ rstat, uint32_t *__restrict__ gthr)
+// Range sums t0 = sum r, t2 = sum r^2, work item (table, block of 4 ranges):
+// one wave per (padded) range.  Also resets the range's pruning threshold.
+__global__ void k_rstats(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work)
 {
-    const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (r >= nr_pad) return;
-    const uint32_t size = 1u << lg_size, K = size * size;
+    const Work2 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t r = wk.y * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (r >= T.nr_pad) return;
+    const uint32_t lg_size = __builtin_ctz(T.size), size = T.size, K = size * size;
     int32_t s1 = 0, s2 = 0;
-    if (r < nr) {
-        const uint32_t row = rows[r], col = cols[r];
+    if (r < T.nr) {
+        const uint32_t row = T.rows[r], col = T.cols[r];
         for (uint32_t e = lane; e < K; e += 64) {
-            const int32_t pix = px[(row + (e >> lg_size)) * w + col + (e & (size - 1))];
+            const int32_t pix = T.px[(row + (e >> lg_size)) * T.w + col + (e & (size - 1))];
             s1 += pix;
             s2 += pix * pix;
         }
@@ -396,9 +522,9 @@ __global__ void k_rstats(const uint8_t *__restrict__ px, uint32_t w,
         s2 += __shfl_xor(s2, off);
     }
     if (lane == 0) {
-        rstat[2 * r] = s1;
-        rstat[2 * r + 1] = s2;
-        gthr[r] = 0x7f800000u; // +inf: no threshold yet
+        T.rstat[2 * r] = s1;
+        T.rstat[2 * r + 1] = s2;
+        T.gthr[r] = 0x7f800000u; // +inf: no threshold yet
     }
 }
 
@@ -415,19 +541,19 @@ template <int S> struct Geo {
 
 // Load 4/8/16 aligned bytes of a split plane: plane[row][col .. col+n).
 // All 16 copies of a plane span < 4 GiB, so offsets are 32-bit.
-__device__ __forceinline__ uint32_t ld4(const int8_t *base, size_t cs, uint32_t pitch,
+__device__ __forceinline__ uint32_t ld4(const GLB int8_t *base, size_t cs, uint32_t pitch,
                                         uint32_t row, uint32_t col)
 {
     uint32_t c = col & 3;
     return *(const uint32_t *)(base + (c * (uint32_t)cs + row * pitch + (col - c)));
 }
-__device__ __forceinline__ uint2 ld8(const int8_t *base, size_t cs, uint32_t pitch,
+__device__ __forceinline__ uint2 ld8(const GLB int8_t *base, size_t cs, uint32_t pitch,
                                      uint32_t row, uint32_t col)
 {
     uint32_t c = col & 7;
     return *(const uint2 *)(base + (c * (uint32_t)cs + row * pitch + (col - c)));
 }
-__device__ __forceinline__ v4i ld16(const int8_t *base, size_t cs, uint32_t pitch,
+__device__ __forceinline__ v4i ld16(const GLB int8_t *base, size_t cs, uint32_t pitch,
                                     uint32_t row, uint32_t col)
 {
     uint32_t c = col & 15;
@@ -439,7 +565,7 @@ __device__ __forceinline__ v4i ld16(const int8_t *base, size_t cs, uint32_t pitc
 // the A fragment, so the product does not depend on how the instruction
 // orders K internally.
 template <int S>
-__device__ __forceinline__ void load_b(const int8_t *plane, size_t cs, uint32_t pitch,
+__device__ __forceinline__ void load_b(const GLB int8_t *plane, size_t cs, uint32_t pitch,
                                        uint32_t dr, uint32_t dc, int kk, int h, long *b8, v4i *b16)
 {
     if constexpr (S == 2) {
@@ -474,7 +600,7 @@ __device__ __forceinline__ void load_b(const int8_t *plane, size_t cs, uint32_t 
 }
 
 template <int S>
-__device__ __forceinline__ void load_a(const int8_t *rng8, uint32_t g, int tt, int kk, int lane,
+__device__ __forceinline__ void load_a(const GLB int8_t *rng8, uint32_t g, int tt, int kk, int lane,
                                        long *a8, v4i *a16)
 {
     constexpr int KMEM = Geo<S>::KMEM;
@@ -482,7 +608,7 @@ __device__ __forceinline__ void load_a(const int8_t *rng8, uint32_t g, int tt, i
     int m = lane & 31, h = lane >> 5;
     int rl = ((m >> 2) & 1) * 4 + (m & 3);
     int iso = 4 * tt + (m >> 3);
-    const int8_t *p = rng8 + ((size_t)(g * 8 + rl) * 8 + iso) * KMEM + kk * KSTEP + h * (KSTEP / 2);
+    const GLB int8_t *p = rng8 + ((size_t)(g * 8 + rl) * 8 + iso) * KMEM + kk * KSTEP + h * (KSTEP / 2);
     if constexpr (Geo<S>::SMALLK)
         *a8 = *(const long *)p;
     else
@@ -507,7 +633,7 @@ __device__ __forceinline__ uint32_t tau_bits(double sum, int64_t n)
 
 // Stage 2 (see k_search_mfma): true when the candidate may beat or tie the
 // threshold, or sits on a quantisation boundary -- i.e. needs stage 3.
-__device__ __forceinline__ bool stage2_keep(const SearchArgs &a, double nd, double inv_n,
+__device__ __forceinline__ bool stage2_keep(const Common &a, double nd, double inv_n,
                                             int32_t t1, int32_t S1, int64_t t0, double Di,
                                             double invDi, double A, float thr)
 {
@@ -528,7 +654,7 @@ __device__ __forceinline__ bool stage2_keep(const SearchArgs &a, double nd, doub
     const double bs = (t0d - a2 * ((double)S1 * 0.25)) * inv_n;
     double beta = bs;
     if (a2 > 0.0) beta += a2 * 255.0;
-    const double yp = beta * a.yscale[(int)qa] + 0.5;
+    const double yp = beta * glb(a.yscale)[(int)qa] + 0.5;
     double mb = floor(yp);
     const double fr = yp - mb;
     const bool boundary = fr < 1e-9 || fr > 1.0 - 1e-9;
@@ -550,8 +676,8 @@ template <int S> struct Queue {
 
 // Certified split-INT8 search for one range size S.
 //
-// Grid: x = position chunk, y = group of 8 ranges.  Block = 4 waves; each
-// wave takes the chunk's 32-position tiles round robin.  A wave tile is
+// Work item (table, position chunk, group of 8 ranges); block = 4 waves,
+// each wave takes the chunk's 32-position tiles round robin.  A wave tile is
 // 64 GEMM rows (8 ranges x 8 isometries, two 32-row MFMA tiles) x 32
 // positions.  Output layout of v_mfma_i32_32x32: item t of lane l is
 // row 8*(t/4) + 4*(l/32) + t%4, column l%32; rows are arranged so that lane
@@ -586,18 +712,23 @@ template <int S> struct Queue {
 // dropped; the rest (new bests, near ties, boundary cases) take stage 3, the
 // verbatim binary64 fit of encode.rs:247-289, and compete with the wave's
 // best by (rms, key).  Winners tighten tau through LDS; blocks exchange tau
-// through a global array every 8 tiles.  Tau starts from k_seed's sample.
+// through a global array every 16 tiles.  Tau starts from k_seed's sample.
 template <int S>
-__global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
+__global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict__ tabs,
+                                                     const Work3 *__restrict__ work, Common c)
 {
     using G = Geo<S>;
     constexpr int NK = G::NK;
     constexpr int64_t n = (int64_t)S * S;
-    constexpr int LOG2N = S == 2 ? 2 : S == 4 ? 4 : S == 8 ? 6 : S == 16 ? 8 : 10;
+    constexpr int LOG2S = S == 2 ? 1 : S == 4 ? 2 : S == 8 ? 3 : S == 16 ? 4 : 5;
+    constexpr int LOG2N = 2 * LOG2S;
     constexpr bool INT_LIMIT = S <= 8; // every stage-1 quantity fits 32 bits
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5, jj = lane & 31;
-    const uint32_t g = blockIdx.y, chunk = blockIdx.x;
+    const Work3 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t chunk = wk.y, g = wk.z;
+    const uint32_t nchunks = T.nchunks[0], tiles_per_chunk = T.tiles_per_chunk[0];
     const double nd = (double)n, inv_n = 1.0 / (double)n;
 
     __shared__ uint32_t s_thr[8];
@@ -614,9 +745,9 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
     if (threadIdx.x < 2) s_count[threadIdx.x] = 0;
     if (threadIdx.x < 8) {
         uint32_t r = g * 8 + threadIdx.x;
-        bool v = r < a.nr;
-        s_thr[threadIdx.x] = v ? a.gthr[r] : 0x7f800000u;
-        int64_t t0 = v ? a.rstat[2 * r] : 0, t2 = v ? a.rstat[2 * r + 1] : 0;
+        bool v = r < T.nr;
+        s_thr[threadIdx.x] = v ? T.gthr[r] : 0x7f800000u;
+        int64_t t0 = v ? T.rstat[2 * r] : 0, t2 = v ? T.rstat[2 * r + 1] : 0;
         s_T0[threadIdx.x] = t0;
         s_T2[threadIdx.x] = t2;
         s_A[threadIdx.x] = (double)(n * t2 - t0 * t0);
@@ -636,10 +767,10 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         uint32_t r = g * 8 + 4 * h + i;
-        rvalid[i] = r < a.nr;
+        rvalid[i] = r < T.nr;
         uint32_t rr = rvalid[i] ? r : 0;
-        T0[i] = a.rstat[2 * rr];
-        const int64_t t2 = a.rstat[2 * rr + 1];
+        T0[i] = T.rstat[2 * rr];
+        const int64_t t2 = T.rstat[2 * rr + 1];
         A[i] = (double)(n * t2 - T0[i] * T0[i]);
         Alo[i] = (float)A[i] * (1.0f - 3e-7f); // <= A after both roundings
         srp[i] = (int32_t)(T0[i] - 128 * n);
@@ -651,10 +782,10 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
 #pragma unroll
         for (int tt = 0; tt < 2; tt++)
 #pragma unroll
-            for (int kk = 0; kk < NK; kk++) load_a<S>(a.rng8, g, tt, kk, lane, &a8[tt][kk], &a16[tt][kk]);
+            for (int kk = 0; kk < NK; kk++) load_a<S>(T.rng8, g, tt, kk, lane, &a8[tt][kk], &a16[tt][kk]);
     } else {
         // The group's 8 ranges x 8 isometries are 64 contiguous rows of rng8.
-        const int8_t *src = a.rng8 + (size_t)g * 64 * G::KMEM;
+        const GLB int8_t *src = T.rng8 + (size_t)g * 64 * G::KMEM;
         constexpr int CPR = G::KMEM / 16; // 16-byte chunks per row
         for (int c = threadIdx.x; c < 64 * CPR; c += 256) {
             const int row = c / CPR, col = c - row * CPR;
@@ -666,9 +797,9 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
     const int am = lane & 31;
     const int arow0 = (((am >> 2) & 1) * 4 + (am & 3)) * 8 + (am >> 3);
 
-    const uint32_t ntiles = (a.P + 31) / 32;
-    const uint32_t t_begin = chunk * a.tiles_per_chunk;
-    uint32_t t_end = t_begin + a.tiles_per_chunk;
+    const uint32_t ntiles = (T.P + 31) / 32;
+    const uint32_t t_begin = chunk * tiles_per_chunk;
+    uint32_t t_end = t_begin + tiles_per_chunk;
     if (t_end > ntiles) t_end = ntiles;
     unsigned long long qtotal = 0, stage3 = 0; // stage-1 survivors (wave), stage-3 fits (lane)
     uint32_t tcount = 0;
@@ -685,15 +816,15 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
         bool keep = false, win = false;
         double rms = 0.0, sum = 0.0;
         if (act) {
-            const uint4 pin = a.posinfo[p];
+            const uint4 pin = ld_u4(T.posinfo, p);
             const int32_t S1 = (int32_t)pin.y, S2 = (int32_t)pin.z;
-            const double Di = a.posd[2 * p], invDi = a.posd[2 * p + 1];
+            const double Di = T.posd[2 * p], invDi = T.posd[2 * p + 1];
             const int64_t t0 = s_T0[slot], t2 = s_T2[slot];
-            keep = !a.stage2 || stage2_keep(a, nd, inv_n, t1, S1, t0, Di, invDi, s_A[slot],
+            keep = !c.stage2 || stage2_keep(c, nd, inv_n, t1, S1, t0, Di, invDi, s_A[slot],
                                             __uint_as_float(s_thr[slot]));
             if (keep) {
                 uint32_t qa, qb;
-                fit_f64(n, S1, S2, t0, t1, t2, a.max_alfa, a.bits_alfa, a.bits_beta, &qa, &qb,
+                fit_f64(n, S1, S2, t0, t1, t2, c.max_alfa, c.bits_alfa, c.bits_beta, &qa, &qb,
                         &rms, &sum);
                 win = better(rms, key, wb[slot].rms, wb[slot].key);
             }
@@ -726,7 +857,7 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
     };
     auto load_pin = [&](uint32_t tile) {
         const uint32_t p = tile * 32 + jj;
-        return a.posinfo[p < a.P ? p : a.P - 1];
+        return ld_u4(T.posinfo, p < T.P ? p : T.P - 1);
     };
     auto fetch_b = [&](const uint4 pin, Frag &f) {
         f.pin = pin;
@@ -734,8 +865,8 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
             const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
 #pragma unroll
             for (int kk = 0; kk < NK; kk++) {
-                load_b<S>(a.lo, a.copy_stride, a.pitch, dr, dc, kk, h, &f.l8[kk], &f.l16[kk]);
-                load_b<S>(a.hi, a.copy_stride, a.pitch, dr, dc, kk, h, &f.h8[kk], &f.h16[kk]);
+                load_b<S>(T.lo, T.copy_stride, T.pitch, dr, dc, kk, h, &f.l8[kk], &f.l16[kk]);
+                load_b<S>(T.hi, T.copy_stride, T.pitch, dr, dc, kk, h, &f.h8[kk], &f.h16[kk]);
             }
         }
     };
@@ -751,23 +882,23 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
         long dummy8 = 0;
 #pragma unroll
         for (int d = 0; d < PD; d++) {
-            load_b<S>(a.lo, a.copy_stride, a.pitch, dr0, dc0, d, h, &dummy8, &pbl[d]);
-            load_b<S>(a.hi, a.copy_stride, a.pitch, dr0, dc0, d, h, &dummy8, &pbh[d]);
+            load_b<S>(T.lo, T.copy_stride, T.pitch, dr0, dc0, d, h, &dummy8, &pbl[d]);
+            load_b<S>(T.hi, T.copy_stride, T.pitch, dr0, dc0, d, h, &dummy8, &pbh[d]);
         }
     }
-    const bool exchange = a.nchunks > 1;
+    const bool exchange = nchunks > 1;
 
     for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4, tcount++) {
         // Exchange thresholds with the other blocks of this range group.
         if (exchange && (tcount & 15) == 15 && lane < 8) {
             uint32_t r = g * 8 + lane;
-            if (r < a.nr) atomicMin(&s_thr[lane], atomicMin(&a.gthr[r], s_thr[lane]));
+            if (r < T.nr) atomicMin(&s_thr[lane], atomicMin((uint32_t *)&T.gthr[r], s_thr[lane]));
         }
         const Frag cur = nxt;
         if (tile + 4 < t_end) fetch_b(pin_ahead, nxt);
         if (tile + 8 < t_end) pin_ahead = load_pin(tile + 8);
         const uint32_t p = tile * 32 + jj;
-        const bool pv = p < a.P;
+        const bool pv = p < T.P;
         const uint4 pin = cur.pin;
         const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
         const int32_t S1p = (int32_t)pin.y;
@@ -808,11 +939,11 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
                 // Refill the slot with the fragment PD steps ahead.
                 long dummy8 = 0;
                 if (kk + PD < NK) {
-                    load_b<S>(a.lo, a.copy_stride, a.pitch, dr, dc, kk + PD, h, &dummy8, &pbl[d]);
-                    load_b<S>(a.hi, a.copy_stride, a.pitch, dr, dc, kk + PD, h, &dummy8, &pbh[d]);
+                    load_b<S>(T.lo, T.copy_stride, T.pitch, dr, dc, kk + PD, h, &dummy8, &pbl[d]);
+                    load_b<S>(T.hi, T.copy_stride, T.pitch, dr, dc, kk + PD, h, &dummy8, &pbh[d]);
                 } else if (has_next) {
-                    load_b<S>(a.lo, a.copy_stride, a.pitch, ndr, ndc, kk + PD - NK, h, &dummy8, &pbl[d]);
-                    load_b<S>(a.hi, a.copy_stride, a.pitch, ndr, ndc, kk + PD - NK, h, &dummy8, &pbh[d]);
+                    load_b<S>(T.lo, T.copy_stride, T.pitch, ndr, ndc, kk + PD - NK, h, &dummy8, &pbl[d]);
+                    load_b<S>(T.hi, T.copy_stride, T.pitch, ndr, ndc, kk + PD - NK, h, &dummy8, &pbh[d]);
                 }
             }
         }
@@ -888,23 +1019,20 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
     if (lane == 0) atomicAdd(&s_count[0], qtotal);
     atomicAdd(&s_count[1], stage3);
     __syncthreads();
-    if (threadIdx.x == 0 && a.counters) {
-        atomicAdd(&a.counters[a.counter_slot], s_count[0]);
-        atomicAdd(&a.counters[8 + a.counter_slot], s_count[1]);
+    if (threadIdx.x == 0 && c.counters) {
+        atomicAdd((unsigned long long *)&glb(c.counters)[LOG2S], s_count[0]);
+        atomicAdd((unsigned long long *)&glb(c.counters)[8 + LOG2S], s_count[1]);
     }
 
     if (threadIdx.x < 8) {
         const uint32_t sl = threadIdx.x, r = g * 8 + sl;
-        if (r < a.nr) {
+        if (r < T.nr) {
             BestRec b = s_wb[0][sl];
             for (int wv = 1; wv < 4; wv++) {
                 const BestRec &e = s_wb[wv][sl];
                 if (better(e.rms, e.key, b.rms, b.key)) b = e;
             }
-            Partial &out = a.partial[(size_t)r * a.nchunks + chunk];
-            out.rms = b.rms;
-            out.key = b.key;
-            out.t1 = b.t1;
+            st_partial(T.partial, (size_t)r * nchunks + chunk, b.rms, b.key, b.t1);
         }
     }
 }
@@ -919,11 +1047,14 @@ __global__ __launch_bounds__(256) void k_search_mfma(SearchArgs a)
 // reference wherever binary32 rounding moves a quantisation decision or an
 // argmin near-tie (and t1 is inexact above 2^24).
 template <int S>
-__global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
+__global__ __launch_bounds__(256) void k_search_fp32(const TableDesc *__restrict__ tabs,
+                                                     const Work3 *__restrict__ work, Common c)
 {
     constexpr int K = S * S;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t g = blockIdx.y, chunk = blockIdx.x;
+    const Work3 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t chunk = wk.y, g = wk.z;
     __shared__ BestRec s_best[4][64][4];
 
     float T0[4], T2[4];
@@ -931,10 +1062,10 @@ __global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         uint32_t r = g * 4 + i;
-        rvalid[i] = r < a.nr;
+        rvalid[i] = r < T.nr;
         uint32_t rr = rvalid[i] ? r : 0;
-        T0[i] = (float)a.rstat[2 * rr];
-        T2[i] = (float)a.rstat[2 * rr + 1];
+        T0[i] = (float)T.rstat[2 * rr];
+        T2[i] = (float)T.rstat[2 * rr + 1];
     }
     float brms[4];
     uint32_t bkey[4];
@@ -944,27 +1075,27 @@ __global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
         bkey[i] = 0xffffffffu;
     }
     const float s0 = (float)K;
-    const uint32_t ntiles = (a.P + 63) / 64;
-    const uint32_t t_begin = chunk * a.tiles_per_chunk;
-    uint32_t t_end = t_begin + a.tiles_per_chunk;
+    const uint32_t ntiles = (T.P + 63) / 64;
+    const uint32_t t_begin = chunk * T.tiles_per_chunk[1];
+    uint32_t t_end = t_begin + T.tiles_per_chunk[1];
     if (t_end > ntiles) t_end = ntiles;
-    const float *Ar = a.rngf + (size_t)g * K * 32;
+    const GLB float *Ar = T.rngf + (size_t)g * K * 32;
 
     for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4) {
         const uint32_t p = tile * 64 + lane;
-        const bool pv = p < a.P;
-        const uint32_t pc = pv ? p : a.P - 1;
-        const uint4 pin = a.posinfo[pc];
+        const bool pv = p < T.P;
+        const uint32_t pc = pv ? p : T.P - 1;
+        const uint4 pin = ld_u4(T.posinfo, pc);
         const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
-        const float *D = a.dplanef + (size_t)dr * a.wc + dc;
+        const GLB float *D = T.df + (size_t)dr * T.wc + dc;
         float acc[32];
 #pragma unroll
         for (int r = 0; r < 32; r++) acc[r] = 0.0f;
         for (int u = 0; u < S; u++) {
 #pragma unroll 8
             for (int v = 0; v < S; v++) {
-                float d = D[(size_t)u * a.wc + v];
-                const float *ar = Ar + (size_t)(u * S + v) * 32;
+                float d = D[(size_t)u * T.wc + v];
+                const GLB float *ar = Ar + (size_t)(u * S + v) * 32;
 #pragma unroll
                 for (int r = 0; r < 32; r++) acc[r] = __builtin_fmaf(ar[r], d, acc[r]);
             }
@@ -977,8 +1108,8 @@ __global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
             const uint32_t iso = r >> 2;
             uint32_t qa, qb;
             float rms;
-            fit_f32(s0, S1, S2, T0[i], acc[r], T2[i], (float)a.max_alfa, a.bits_alfa,
-                    a.bits_beta, &qa, &qb, &rms);
+            fit_f32(s0, S1, S2, T0[i], acc[r], T2[i], (float)c.max_alfa, c.bits_alfa,
+                    c.bits_beta, &qa, &qb, &rms);
             uint32_t key = p * 8 + iso;
             if (rms < brms[i] || (rms == brms[i] && key < bkey[i])) {
                 brms[i] = rms;
@@ -994,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
     __syncthreads();
     if (threadIdx.x < 4) {
         uint32_t i = threadIdx.x, r = g * 4 + i;
-        if (r < a.nr) {
+        if (r < T.nr) {
             double b = __longlong_as_double(0x7ff0000000000000ll);
             uint32_t k = 0xffffffffu;
             for (int wv = 0; wv < 4; wv++)
@@ -1005,56 +1136,58 @@ __global__ __launch_bounds__(256) void k_search_fp32(SearchArgs a)
                         k = e.key;
                     }
                 }
-            Partial &out = a.partial[(size_t)r * a.nchunks + chunk];
-            out.rms = b;
-            out.key = k;
-            out.t1 = 0;
+            st_partial(T.partial, (size_t)r * T.nchunks[1] + chunk, b, k, 0);
         }
     }
 }
 
 // ---------------------------------------------------------------- merge
 
-__device__ __forceinline__ void write_winner(const MergeArgs &a, uint32_t r, uint32_t k,
-                                             int64_t t1)
+__device__ __forceinline__ void write_winner(const TableG &T, const Common &cm, uint32_t r,
+                                             uint32_t k, int64_t t1)
 {
     mars_hip_candidate c = {};
-    const uint32_t p = k >> 3, iso = k & 7, S = a.size;
-    const uint4 pin = a.posinfo[p];
-    const uint32_t pi = p / a.npc, pj = p - pi * a.npc;
-    c.dom_row = pi * a.shift;
-    c.dom_col = pj * a.shift;
+    const uint32_t p = k >> 3, iso = k & 7, S = T.size;
+    const uint4 pin = ld_u4(T.posinfo, p);
+    const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
+    c.dom_row = pi * T.shift;
+    c.dom_col = pj * T.shift;
     c.isometry = iso;
     c.s0 = (int64_t)S * S;
     c.s1_x4 = (int32_t)pin.y;
     c.s2_x16 = (int32_t)pin.z;
-    c.t0 = a.rstat[2 * r];
+    c.t0 = T.rstat[2 * r];
     c.t1_x4 = t1;
-    c.t2 = a.rstat[2 * r + 1];
+    c.t2 = T.rstat[2 * r + 1];
     uint32_t qa, qb;
     double rms, sum;
-    fit_f64(c.s0, c.s1_x4, c.s2_x16, c.t0, c.t1_x4, c.t2, a.max_alfa, a.bits_alfa, a.bits_beta,
+    fit_f64(c.s0, c.s1_x4, c.s2_x16, c.t0, c.t1_x4, c.t2, cm.max_alfa, cm.bits_alfa, cm.bits_beta,
             &qa, &qb, &rms, &sum);
     c.qalfa = qa;
     c.qbeta = qb;
     c.rms = rms;
-    a.out[r] = c;
-    a.found[r] = 1;
+    st_cand(T.out, r, c);
+    T.found[r] = 1;
 }
 
-// One thread per range, for partials that carry the winner's exact t1 (the
-// MFMA search): min over position chunks by (rms, key), then the winner's
-// RawMoments and (qalfa, qbeta, rms) through the binary64 fit.
-__global__ void k_merge_exact(MergeArgs a)
+// Work item (table, block of 128 ranges), one thread per range, for partials
+// that carry the winner's exact t1 (the MFMA search): min over position
+// chunks by (rms, key), then the winner's RawMoments and (qalfa, qbeta, rms)
+// through the binary64 fit.
+__global__ void k_merge_exact(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work,
+                              Common cm)
 {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.nr) return;
+    const Work2 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t r = wk.y * blockDim.x + threadIdx.x;
+    if (r >= T.nr) return;
+    const uint32_t nchunks = T.nchunks[0];
     double b = __longlong_as_double(0x7ff0000000000000ll);
     uint32_t k = 0xffffffffu;
     int32_t t1 = 0;
-    if (a.P > 0)
-        for (uint32_t ch = 0; ch < a.nchunks; ch++) {
-            const Partial e = a.partial[(size_t)r * a.nchunks + ch];
+    if (T.P > 0)
+        for (uint32_t ch = 0; ch < nchunks; ch++) {
+            const Partial e = ld_partial(T.partial, (size_t)r * nchunks + ch);
             if (better(e.rms, e.key, b, k)) {
                 b = e.rms;
                 k = e.key;
@@ -1062,31 +1195,34 @@ __global__ void k_merge_exact(MergeArgs a)
             }
         }
     if (k == 0xffffffffu) {
-        a.out[r] = mars_hip_candidate{};
-        a.found[r] = 0;
+        st_cand(T.out, r, mars_hip_candidate{});
+        T.found[r] = 0;
         return;
     }
-    write_winner(a, r, k, t1);
+    write_winner(T, cm, r, k, t1);
 }
 
-// One 64-thread block per range, for partials without an exact t1 (the FP32
-// baseline): min over position chunks by (rms, key), the winner's t1 by a
-// direct window correlation, then its RawMoments and (qalfa, qbeta, rms)
-// through the binary64 fit, so the table holds the reference's Candidate
-// (encode.rs:234-244) field for field.
-__global__ void k_merge(MergeArgs a)
+// Work item (table, range), one 64-thread block, for partials without an
+// exact t1 (the FP32 baseline): min over position chunks by (rms, key), the
+// winner's t1 by a direct window correlation, then its RawMoments and
+// (qalfa, qbeta, rms) through the binary64 fit, so the table holds the
+// reference's Candidate (encode.rs:234-244) field for field.
+__global__ void k_merge(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work,
+                        Common cm)
 {
     __shared__ double s_rms[64];
     __shared__ uint32_t s_key[64];
     __shared__ int32_t s_t1[64];
     __shared__ long long s_acc;
-    const uint32_t r = blockIdx.x, t = threadIdx.x;
+    const Work2 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t r = wk.y, t = threadIdx.x, nchunks = T.nchunks[1];
     double b = __longlong_as_double(0x7ff0000000000000ll);
     uint32_t k = 0xffffffffu;
     int32_t t1 = 0;
-    if (a.P > 0)
-        for (uint32_t ch = t; ch < a.nchunks; ch += 64) {
-            const Partial &e = a.partial[(size_t)r * a.nchunks + ch];
+    if (T.P > 0)
+        for (uint32_t ch = t; ch < nchunks; ch += 64) {
+            const Partial e = ld_partial(T.partial, (size_t)r * nchunks + ch);
             if (better(e.rms, e.key, b, k)) {
                 b = e.rms;
                 k = e.key;
@@ -1112,28 +1248,28 @@ __global__ void k_merge(MergeArgs a)
     k = s_key[0];
     if (k == 0xffffffffu) {
         if (t == 0) {
-            a.out[r] = mars_hip_candidate{};
-            a.found[r] = 0;
+            st_cand(T.out, r, mars_hip_candidate{});
+            T.found[r] = 0;
         }
         return;
     }
-    const uint32_t p = k >> 3, iso = k & 7, S = a.size;
-    const uint4 pin = a.posinfo[p];
+    const uint32_t p = k >> 3, iso = k & 7, S = T.size;
+    const uint4 pin = ld_u4(T.posinfo, p);
     const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
     {
-        const uint32_t row = a.rows[r], col = a.cols[r];
+        const uint32_t row = T.rows[r], col = T.cols[r];
         long long part = 0;
         for (uint32_t e = t; e < S * S; e += 64) {
             uint32_t u = e / S, v = e - u * S, i, j;
             iso_map(iso, u, v, S, &i, &j);
-            part += (long long)a.px[(size_t)(row + i) * a.w + col + j] *
-                    (long long)a.d[(size_t)(dr + u) * a.wc + dc + v];
+            part += (long long)T.px[(size_t)(row + i) * T.w + col + j] *
+                    (long long)T.d[(size_t)(dr + u) * T.wc + dc + v];
         }
         atomicAdd((unsigned long long *)&s_acc, (unsigned long long)part);
         __syncthreads();
     }
     if (t != 0) return;
-    write_winner(a, r, k, (int64_t)s_acc);
+    write_winner(T, cm, r, k, (int64_t)s_acc);
 }
 
 // Plain copy of n 16-byte words: the known-traffic reference for the HBM
@@ -1164,95 +1300,86 @@ __global__ void k_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, ui
 
 // ---------------------------------------------------------------- launchers
 
-template <int S> static void launch_mfma(const SearchArgs &a, hipStream_t st)
+template <int S>
+static void launch_mfma(const TableDesc *t, const Work3 *work, uint32_t n, const Common &c,
+                        hipStream_t st)
 {
-    dim3 grid(a.nchunks, a.ngroups);
-    hipLaunchKernelGGL(k_search_mfma<S>, grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_search_mfma<S>, dim3(n), dim3(256), 0, st, t, work, c);
 }
-template <int S> static void launch_fp32(const SearchArgs &a, hipStream_t st)
+template <int S>
+static void launch_fp32(const TableDesc *t, const Work3 *work, uint32_t n, const Common &c,
+                        hipStream_t st)
 {
-    dim3 grid(a.nchunks, a.ngroups);
-    hipLaunchKernelGGL(k_search_fp32<S>, grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_search_fp32<S>, dim3(n), dim3(256), 0, st, t, work, c);
 }
 
-int launch_search(int kernel, uint32_t size, const SearchArgs &a, hipStream_t st)
+int launch_search(int kernel, uint32_t size, const TableDesc *t, const Work3 *work,
+                  uint32_t n, const Common &c, hipStream_t st)
 {
+    if (n == 0) return 0;
     if (kernel == 0) {
         switch (size) {
-        case 2: launch_mfma<2>(a, st); return 0;
-        case 4: launch_mfma<4>(a, st); return 0;
-        case 8: launch_mfma<8>(a, st); return 0;
-        case 16: launch_mfma<16>(a, st); return 0;
-        case 32: launch_mfma<32>(a, st); return 0;
+        case 2: launch_mfma<2>(t, work, n, c, st); return 0;
+        case 4: launch_mfma<4>(t, work, n, c, st); return 0;
+        case 8: launch_mfma<8>(t, work, n, c, st); return 0;
+        case 16: launch_mfma<16>(t, work, n, c, st); return 0;
+        case 32: launch_mfma<32>(t, work, n, c, st); return 0;
         }
     } else {
         switch (size) {
-        case 2: launch_fp32<2>(a, st); return 0;
-        case 4: launch_fp32<4>(a, st); return 0;
-        case 8: launch_fp32<8>(a, st); return 0;
-        case 16: launch_fp32<16>(a, st); return 0;
-        case 32: launch_fp32<32>(a, st); return 0;
+        case 2: launch_fp32<2>(t, work, n, c, st); return 0;
+        case 4: launch_fp32<4>(t, work, n, c, st); return 0;
+        case 8: launch_fp32<8>(t, work, n, c, st); return 0;
+        case 16: launch_fp32<16>(t, work, n, c, st); return 0;
+        case 32: launch_fp32<32>(t, work, n, c, st); return 0;
         }
     }
     return -1;
 }
 
-void launch_contract(const uint8_t *px, uint32_t w, uint32_t wc, uint32_t hc, uint32_t pitch,
-                     size_t copy_stride, int32_t *d, float *df, int8_t *lo, int8_t *hi,
+void launch_contract(const PlaneDesc *planes, const Work2 *work, uint32_t n,
                      unsigned long long *counters, uint32_t n_counters, hipStream_t st)
 {
-    uint32_t n = wc * hc;
-    if (n < n_counters) n = n_counters;
-    hipLaunchKernelGGL(k_contract, dim3((n + 255) / 256), dim3(256), 0, st, px, w, wc, hc, pitch,
-                       copy_stride, d, df, lo, hi, counters, n_counters);
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_contract, dim3(n), dim3(256), 0, st, planes, work, counters, n_counters);
 }
 
-void launch_sat(const int32_t *d, uint32_t wc, uint32_t hc, int64_t *sat1, int64_t *sat2,
-                hipStream_t st)
+void launch_sat(const PlaneDesc *planes, const Work2 *rows_work, uint32_t n_rows,
+                const Work2 *cols_work, uint32_t n_cols, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sat_rows, dim3(hc + 1), dim3(256), 0, st, d, wc, hc, sat1, sat2);
-    hipLaunchKernelGGL(k_sat_cols, dim3((wc + 1 + 15) / 16), dim3(256), 0, st, wc, hc, sat1,
-                       sat2);
+    if (n_rows) hipLaunchKernelGGL(k_sat_rows, dim3(n_rows), dim3(256), 0, st, planes, rows_work);
+    if (n_cols) hipLaunchKernelGGL(k_sat_cols, dim3(n_cols), dim3(256), 0, st, planes, cols_work);
 }
 
-void launch_possums(const int64_t *sat1, const int64_t *sat2, uint32_t wc, uint32_t size,
-                    uint32_t shift, uint32_t npc, uint32_t P, uint4 *posinfo, double *posd,
-                    hipStream_t st)
+void launch_possums(const TableDesc *t, const Work2 *work, uint32_t n, hipStream_t st)
 {
-    if (P == 0) return;
-    hipLaunchKernelGGL(k_possums, dim3((P + 255) / 256), dim3(256), 0, st, sat1, sat2, wc, size,
-                       shift, npc, P, posinfo, posd);
+    if (n) hipLaunchKernelGGL(k_possums, dim3(n), dim3(256), 0, st, t, work);
 }
 
-void launch_seed(const int8_t *rng8, const int32_t *rstat, const int32_t *d, uint32_t wc,
-                 uint32_t size, uint32_t kmem, uint32_t npc, uint32_t P, const uint4 *posinfo,
-                 uint32_t nr, double max_alfa, uint32_t bits_alfa, uint32_t bits_beta,
-                 uint32_t *gthr, hipStream_t st)
+void launch_seed(const TableDesc *t, const Work2 *work, uint32_t n, const Common &c,
+                 hipStream_t st)
 {
-    if (P == 0 || nr == 0) return;
-    hipLaunchKernelGGL(k_seed, dim3(nr), dim3(256), 0, st, rng8, rstat, d, wc, size, kmem, npc, P,
-                       posinfo, max_alfa, bits_alfa, bits_beta, gthr);
+    if (n) hipLaunchKernelGGL(k_seed, dim3(n), dim3(256), 0, st, t, work, c);
 }
 
-void launch_ranges(const uint8_t *px, uint32_t w, const uint32_t *rows, const uint32_t *cols,
-                   uint32_t nr, uint32_t nr_pad, uint32_t size, uint32_t kmem, int8_t *rng8,
-                   float *rngf, int32_t *rstat, uint32_t *gthr, hipStream_t st)
+void launch_ranges(const TableDesc *t, const Work2 *work, uint32_t n, bool fp32, hipStream_t st)
 {
-    const uint32_t lg_size = __builtin_ctz(size), lg_kmem = __builtin_ctz(kmem);
-    const uint32_t nthreads = nr_pad * 8 * kmem / 4;
-    hipLaunchKernelGGL(k_ranges, dim3((nthreads + 255) / 256), dim3(256), 0, st, px, w, rows, cols,
-                       nr, nr_pad, lg_size, lg_kmem, rng8, rngf);
-    hipLaunchKernelGGL(k_rstats, dim3((nr_pad + 3) / 4), dim3(256), 0, st, px, w, rows, cols, nr,
-                       nr_pad, lg_size, rstat, gthr);
+    if (n) hipLaunchKernelGGL(k_ranges, dim3(n), dim3(256), 0, st, t, work, fp32 ? 1 : 0);
 }
 
-void launch_merge(const MergeArgs &a, hipStream_t st)
+void launch_rstats(const TableDesc *t, const Work2 *work, uint32_t n, hipStream_t st)
 {
-    if (a.nr == 0) return;
-    if (a.t1_exact)
-        hipLaunchKernelGGL(k_merge_exact, dim3((a.nr + 127) / 128), dim3(128), 0, st, a);
+    if (n) hipLaunchKernelGGL(k_rstats, dim3(n), dim3(256), 0, st, t, work);
+}
+
+void launch_merge(const TableDesc *t, const Work2 *work, uint32_t n, bool t1_exact,
+                  const Common &c, hipStream_t st)
+{
+    if (n == 0) return;
+    if (t1_exact)
+        hipLaunchKernelGGL(k_merge_exact, dim3(n), dim3(128), 0, st, t, work, c);
     else
-        hipLaunchKernelGGL(k_merge, dim3(a.nr), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(k_merge, dim3(n), dim3(64), 0, st, t, work, c);
 }
 
 void launch_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,

@@ -123,26 +123,52 @@ class HipContext:
             self._ctx, img.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, C.byref(p), sh,
             len(shifts)), "mars_hip_prepare")
 
+    def prepare_batch(self, images: Sequence[np.ndarray], params: EncodeParams,
+                      shifts: Sequence[int]) -> None:
+        """One pool and search table per plane, searched by one batched step."""
+        imgs = [np.ascontiguousarray(im, dtype=np.uint8) for im in images]
+        if not imgs or any(im.ndim != 2 for im in imgs):
+            raise ValueError("images must be a non-empty list of 2-D u8 planes")
+        planes = (N.Plane * len(imgs))()
+        for i, im in enumerate(imgs):
+            planes[i].px = im.ctypes.data_as(C.POINTER(C.c_uint8))
+            planes[i].h, planes[i].w = im.shape
+        p = N.Params(params.min_size, params.max_size, params.shift, params.bits_alfa,
+                     params.bits_beta, 0, params.max_alfa)
+        sh = (C.c_uint32 * len(shifts))(*shifts)
+        self._check(self._L.mars_hip_prepare_batch(self._ctx, planes, len(imgs), C.byref(p), sh,
+                                                   len(shifts)), "mars_hip_prepare_batch")
+
+    def autotune(self, reps: int = 3) -> N.Tuning:
+        t = N.Tuning()
+        self._check(self._L.mars_hip_autotune(self._ctx, reps, C.byref(t)), "mars_hip_autotune")
+        return t
+
     def run(self, iters: int = 1, use_graph: bool = True) -> N.Timing:
         t = N.Timing()
         self._check(self._L.mars_hip_run(self._ctx, iters, int(use_graph), C.byref(t)),
                     "mars_hip_run")
         return t
 
-    def search_raw(self, row: int, col: int, size: int, shift: int):
+    def search_raw(self, row: int, col: int, size: int, shift: int, plane: int = 0):
         out = N.CCandidate()
         n = C.c_uint32()
         ev = C.c_uint64()
-        rc = self._L.mars_hip_search(self._ctx, row, col, size, shift, 1, C.byref(out),
-                                     C.byref(n), C.byref(ev))
+        rc = self._L.mars_hip_search_plane(self._ctx, plane, row, col, size, shift, 1,
+                                           C.byref(out), C.byref(n), C.byref(ev))
         return rc, (out if n.value else None), ev.value
 
-    def tables(self):
-        """List of (size, shift, rows, cols, candidates[list or None per range], n_positions)."""
+    def tables(self, plane: int = 0):
+        """Tables of one plane: list of (size, shift, rows, cols, candidates[list or None per
+        range], n_positions)."""
         n = C.c_uint32()
         self._check(self._L.mars_hip_table_count(self._ctx, C.byref(n)), "table_count")
         res = []
         for i in range(n.value):
+            pl = C.c_uint32()
+            self._check(self._L.mars_hip_table_plane(self._ctx, i, C.byref(pl)), "table_plane")
+            if pl.value != plane:
+                continue
             s, sh, nr, P = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
             self._check(self._L.mars_hip_table_info(self._ctx, i, C.byref(s), C.byref(sh),
                                                     C.byref(nr), C.byref(P)), "table_info")

@@ -17,7 +17,7 @@ import hashlib
 
 import numpy as np
 
-__all__ = ["plane", "rgb", "sha256", "WORKLOAD_SEEDS"]
+__all__ = ["plane", "rgb", "sha256", "batch_specs", "WORKLOAD_SEEDS"]
 
 # Seeds of the named workloads (256x256 fixed-grid, 512x512 quadtree,
 # 768x512 RGB, the 64-image batch base, the 2048x2048 dense pool).
@@ -100,6 +100,26 @@ def rgb(w: int, h: int, seed: int) -> np.ndarray:
         own = _field(np.random.default_rng(seed * 1000 + 1 + c), w, h) - 128.0
         chans.append(_to_u8(128.0 + (2.0 * base + own) / np.sqrt(5.0)))
     return np.ascontiguousarray(np.stack(chans, axis=-1))
+
+
+# The 64-image batch: sizes and RD lambdas drawn uniformly from these sets.
+BATCH_SIZES = ((256, 256), (512, 512), (768, 512), (1024, 768))
+BATCH_LAMBDAS = (50.0, 200.0, 800.0)
+
+
+def batch_specs(n: int = 64, seed: int = WORKLOAD_SEEDS["batch64"]):
+    """(w, h, plane seed, lambda) of each image of the seeded uneven batch.
+
+    Sizes and lambdas come from one Generator(PCG64(seed)) stream; image i is
+    ``plane(w, h, seed + i)``.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        w, h = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+        lam = BATCH_LAMBDAS[int(rng.integers(len(BATCH_LAMBDAS)))]
+        out.append((w, h, seed + i, lam))
+    return out
 
 
 def sha256(img: np.ndarray) -> str:
