@@ -80,6 +80,7 @@ struct Table {
     int8_t *d_rng8 = nullptr;
     float *d_rngf = nullptr;
     int32_t *d_rstat = nullptr;
+    int32_t *d_rpos = nullptr;
     uint32_t *d_gthr = nullptr;
     uint4 *d_posinfo = nullptr;
     double *d_posd = nullptr;
@@ -326,6 +327,7 @@ int layout(mars_hip_ctx *ctx)
         d.rng8 = t.d_rng8;
         d.rngf = t.d_rngf;
         d.rstat = t.d_rstat;
+        d.rpos = t.d_rpos;
         d.gthr = t.d_gthr;
         d.posinfo = t.d_posinfo;
         d.posd = t.d_posd;
@@ -678,6 +680,7 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         plan.add(&t.d_rng8, (size_t)t.nr_pad * 8 * t.kmem);
         plan.add(&t.d_rngf, (size_t)t.nr_pad / 4 * K * 32 * sizeof(float));
         plan.add(&t.d_rstat, sizeof(int32_t) * 2 * t.nr_pad);
+        plan.add(&t.d_rpos, sizeof(int32_t) * t.nr_pad);
         plan.add(&t.d_gthr, sizeof(uint32_t) * t.nr_pad);
         plan.add(&t.d_posinfo, sizeof(uint4) * (size_t)t.P);
         plan.add(&t.d_posd, 2 * sizeof(double) * (size_t)t.P);

@@ -28,7 +28,7 @@ struct PlaneDesc {
     const uint8_t *px;
     int32_t *d;          // contracted plane (i32)
     float *df;           // contracted plane (f32, FP32 baseline)
-    int8_t *lo, *hi;     // split planes, 16 byte-shifted copies each
+    int8_t *lo, *hi;     // split planes D % 4, D / 4 - 128; 16 byte-shifted copies each
     int64_t *sat1, *sat2;
     uint32_t w, h, wc, hc, pitch, copy_stride;
 };
@@ -47,8 +47,9 @@ struct TableDesc {
     int8_t *rng8;        // [nr_pad][8][kmem] r - 128
     float *rngf;         // [nr_pad/4][K][32] r (FP32 baseline)
     int32_t *rstat;      // [nr_pad][2] t0, t2
+    int32_t *rpos;       // [nr_pad] 3 * sum max(r - 128, 0)
     uint32_t *gthr;      // [nr_pad] pruning threshold (float bits, SSE units)
-    uint4 *posinfo;      // [P] {dr | dc << 16, S1, S2, (float)(n*S2 - S1^2)}
+    uint4 *posinfo;      // [P] {window byte offset, S1, sqrt(n*S2 - S1^2) as f32 rounded down, S2}
     double *posd;        // [P][2] Di = n*S2 - S1^2, 1/Di (binary64, stage 2)
     Partial *partial;    // [nr_pad][nchunks]
     mars_hip_candidate *out;

@@ -53,6 +53,23 @@ __device__ __forceinline__ uint4 ld_u4(const GLB u32x4 *p, size_t i)
     const u32x4 v = p[i];
     return make_uint4(v.x, v.y, v.z, v.w);
 }
+// A fresh read of an LDS word that other waves update (a relaxed atomic
+// load stays a ds_read; a volatile one would become a flat load, which also
+// waits for every global load in flight).
+__device__ __forceinline__ uint32_t lds_load(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The first three words of a record: a 12-byte load, so no destination
+// register of an in-flight load is dead (the compiler would reuse it and
+// wait for the load).
+__device__ __forceinline__ uint3 ld_u3(const GLB u32x4 *p, size_t i)
+{
+    typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+    const u32x3 v = *(const GLB u32x3 *)(p + i);
+    return make_uint3(v.x, v.y, v.z);
+}
 __device__ __forceinline__ Partial ld_partial(const GLB Partial *p, size_t i)
 {
     Partial e;
@@ -104,6 +121,7 @@ struct TableG {
     GLB int8_t *rng8;
     GLB float *rngf;
     GLB int32_t *rstat;
+    GLB int32_t *rpos;
     GLB uint32_t *gthr;
     GLB u32x4 *posinfo;
     GLB double *posd;
@@ -133,6 +151,7 @@ __device__ __forceinline__ TableG view(const TableDesc &t)
     g.rng8 = glb(t.rng8);
     g.rngf = glb(t.rngf);
     g.rstat = glb(t.rstat);
+    g.rpos = glb(t.rpos);
     g.gthr = glb(t.gthr);
     g.posinfo = (GLB u32x4 *)t.posinfo;
     g.posd = glb(t.posd);
@@ -282,7 +301,7 @@ __device__ __forceinline__ void iso_map(uint32_t k, uint32_t u, uint32_t v, uint
 // ---------------------------------------------------------------- pool
 
 // Work item (plane, block): one thread per contracted sample.  Writes D
-// (i32), D as f32, and the i8 planes lo = (D & 255) - 128, hi = D >> 8 into
+// (i32), D as f32, and the i8 planes lo = D % 4, hi = D / 4 - 128 into
 // copies c = 0..15 at x = j - c (copy c holds the row shifted left by c
 // bytes).  Block 0 of the launch also clears the step's counters.
 __global__ void k_contract(const PlaneDesc *__restrict__ planes, const Work2 *__restrict__ work,
@@ -298,8 +317,8 @@ __global__ void k_contract(const PlaneDesc *__restrict__ planes, const Work2 *__
     int32_t v = (int32_t)p0[0] + (int32_t)p0[1] + (int32_t)p0[pl.w] + (int32_t)p0[pl.w + 1];
     pl.d[idx] = v;
     pl.df[idx] = (float)v;
-    int8_t l = (int8_t)((v & 255) - 128);
-    int8_t hh = (int8_t)(v >> 8);
+    const int8_t l = (int8_t)(v & 3);           // Dl = D % 4
+    const int8_t hh = (int8_t)((v >> 2) - 128); // Dh' = D / 4 - 128
 #pragma unroll
     for (uint32_t c = 0; c < 16; c++) {
         if (j >= c) {
@@ -397,8 +416,9 @@ __global__ void k_sat_cols(const PlaneDesc *__restrict__ planes, const Work2 *__
 }
 
 // Work item (table, block of 256 domain positions): window sums from the
-// SATs, packed as {dr | dc << 16, S1, S2, (float)Di} plus {Di, 1/Di} in
-// binary64.
+// SATs, packed as {window byte offset, S1, sqrt(Di) as f32, S2} plus
+// {Di, 1/Di} in binary64.  The search's hot loop reads the first three
+// words only.
 __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work)
 {
     const Work2 wk = work[blockIdx.x];
@@ -413,7 +433,14 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
     const int64_t s2 = T.sat2[b + size] - T.sat2[a + size] - T.sat2[b] + T.sat2[a];
     const int64_t n = (int64_t)size * size;
     const int64_t di = n * s2 - s1 * s1;
-    T.posinfo[p] = (u32x4){dr | (dc << 16), (uint32_t)s1, (uint32_t)s2, __float_as_uint((float)di)};
+    // z: sqrt(Di) rounded down to binary32 (the stage-1 bound's factor).
+    const float sq = __double2float_rd(sqrt((double)di));
+    // x: byte offset of the window origin in the copy of the split planes
+    // where its rows start aligned to the search's load width.
+    const uint32_t cmask = size == 2 ? 0 : size == 4 ? 3 : size == 8 ? 7 : 15;
+    const uint32_t cp = dc & cmask;
+    const uint32_t boff = cp * T.copy_stride + dr * T.pitch + (dc - cp);
+    T.posinfo[p] = (u32x4){boff, (uint32_t)s1, __float_as_uint(sq), (uint32_t)s2};
     T.posd[2 * p] = (double)di;
     T.posd[2 * p + 1] = di > 0 ? 1.0 / (double)di : 0.0;
 }
@@ -438,7 +465,8 @@ __global__ void k_seed(const TableDesc *__restrict__ tabs, const Work2 *__restri
     const uint32_t j = c >> 3, iso = c & 7;
     const uint32_t p = P >= npos ? (uint32_t)(((uint64_t)j * (P - 1)) / (npos - 1)) : j % P;
     const uint4 pin = ld_u4(T.posinfo, p);
-    const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
+    const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
+    const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
     const GLB int8_t *rk = T.rng8 + ((size_t)r * 8 + iso) * T.kmem;
     const GLB int32_t *d = T.d;
     const uint32_t wc = T.wc;
@@ -455,7 +483,7 @@ __global__ void k_seed(const TableDesc *__restrict__ tabs, const Work2 *__restri
         const int64_t n = (int64_t)K;
         uint32_t qa, qb;
         double rms, sum;
-        fit_f64(n, (int32_t)pin.y, (int32_t)pin.z, T.rstat[2 * r], s_t1[c], T.rstat[2 * r + 1],
+        fit_f64(n, (int32_t)pin.y, (int32_t)pin.w, T.rstat[2 * r], s_t1[c], T.rstat[2 * r + 1],
                 cm.max_alfa, cm.bits_alfa, cm.bits_beta, &qa, &qb, &rms, &sum);
         double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
         atomicMin(&s_min, __float_as_uint((float)(tau * (1.0 + 1e-7))));
@@ -498,8 +526,9 @@ __global__ void k_ranges(const TableDesc *__restrict__ tabs, const Work2 *__rest
     *(uint32_t *)(T.rng8 + e) = packed;
 }
 
-// Range sums t0 = sum r, t2 = sum r^2, work item (table, block of 4 ranges):
-// one wave per (padded) range.  Also resets the range's pruning threshold.
+// Range sums t0 = sum r, t2 = sum r^2 and 3 * sum max(r - 128, 0), work
+// item (table, block of 4 ranges): one wave per (padded) range.  Also resets
+// the range's pruning threshold.
 __global__ void k_rstats(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work)
 {
     const Work2 wk = work[blockIdx.x];
@@ -507,23 +536,26 @@ __global__ void k_rstats(const TableDesc *__restrict__ tabs, const Work2 *__rest
     const uint32_t r = wk.y * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (r >= T.nr_pad) return;
     const uint32_t lg_size = __builtin_ctz(T.size), size = T.size, K = size * size;
-    int32_t s1 = 0, s2 = 0;
+    int32_t s1 = 0, s2 = 0, sp = 0;
     if (r < T.nr) {
         const uint32_t row = T.rows[r], col = T.cols[r];
         for (uint32_t e = lane; e < K; e += 64) {
             const int32_t pix = T.px[(row + (e >> lg_size)) * T.w + col + (e & (size - 1))];
             s1 += pix;
             s2 += pix * pix;
+            sp += pix > 128 ? pix - 128 : 0;
         }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         s1 += __shfl_xor(s1, off);
         s2 += __shfl_xor(s2, off);
+        sp += __shfl_xor(sp, off);
     }
     if (lane == 0) {
         T.rstat[2 * r] = s1;
         T.rstat[2 * r + 1] = s2;
+        T.rpos[r] = 3 * sp; // largest sum of r'*Dl over Dl in [0, 3]
         T.gthr[r] = 0x7f800000u; // +inf: no threshold yet
     }
 }
@@ -539,53 +571,35 @@ template <int S> struct Geo {
     static constexpr bool PRELOAD_A = NK <= 2;             // keep A in registers
 };
 
-// Load 4/8/16 aligned bytes of a split plane: plane[row][col .. col+n).
-// All 16 copies of a plane span < 4 GiB, so offsets are 32-bit.
-__device__ __forceinline__ uint32_t ld4(const GLB int8_t *base, size_t cs, uint32_t pitch,
-                                        uint32_t row, uint32_t col)
-{
-    uint32_t c = col & 3;
-    return *(const uint32_t *)(base + (c * (uint32_t)cs + row * pitch + (col - c)));
-}
-__device__ __forceinline__ uint2 ld8(const GLB int8_t *base, size_t cs, uint32_t pitch,
-                                     uint32_t row, uint32_t col)
-{
-    uint32_t c = col & 7;
-    return *(const uint2 *)(base + (c * (uint32_t)cs + row * pitch + (col - c)));
-}
-__device__ __forceinline__ v4i ld16(const GLB int8_t *base, size_t cs, uint32_t pitch,
-                                    uint32_t row, uint32_t col)
-{
-    uint32_t c = col & 15;
-    return *(const v4i *)(base + (c * (uint32_t)cs + row * pitch + (col - c)));
-}
-
-// B fragment (one split plane) of lane half h for K-step kk at window
-// origin (dr, dc).  The k order inside a fragment is the same one used for
-// the A fragment, so the product does not depend on how the instruction
-// orders K internally.
+// B fragment (one split plane) of lane half h for K-step kk of the window
+// whose origin is byte `boff` of the plane's copy layout (the copy in which
+// the window's rows start 4/8/16-byte aligned; all copies of a plane span
+// < 4 GiB).  The k order inside a fragment is the same one used for the A
+// fragment, so the product does not depend on how the instruction orders K
+// internally.
 template <int S>
-__device__ __forceinline__ void load_b(const GLB int8_t *plane, size_t cs, uint32_t pitch,
-                                       uint32_t dr, uint32_t dc, int kk, int h, long *b8, v4i *b16)
+__device__ __forceinline__ void load_b(const GLB int8_t *plane, uint32_t boff, uint32_t pitch,
+                                       int kk, int h, long *b8, v4i *b16)
 {
     if constexpr (S == 2) {
         // K = 4 padded to 16: half 0 holds k 0..3 = (0,0) (0,1) (1,0) (1,1).
         uint32_t x = 0;
         if (h == 0) {
-            const uint8_t *p = (const uint8_t *)plane; // copy 0 = unshifted plane
-            uint32_t a0 = p[dr * pitch + dc], a1 = p[dr * pitch + dc + 1];
-            uint32_t a2 = p[(dr + 1) * pitch + dc], a3 = p[(dr + 1) * pitch + dc + 1];
-            x = a0 | (a1 << 8) | (a2 << 16) | (a3 << 24);
+            const GLB uint8_t *p = (const GLB uint8_t *)plane + boff;
+            x = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[pitch] << 16) |
+                ((uint32_t)p[pitch + 1] << 24);
         }
         *b8 = (long)x;
     } else if constexpr (S == 4) {
-        uint32_t r0 = ld4(plane, cs, pitch, dr + 2 * h, dc);
-        uint32_t r1 = ld4(plane, cs, pitch, dr + 2 * h + 1, dc);
+        const uint32_t o = boff + 2 * h * pitch;
+        const uint32_t r0 = *(const GLB uint32_t *)(plane + o);
+        const uint32_t r1 = *(const GLB uint32_t *)(plane + o + pitch);
         *b8 = (long)((uint64_t)r0 | ((uint64_t)r1 << 32));
     } else if constexpr (S == 8) {
-        uint32_t u = 4 * kk + 2 * h;
-        uint2 r0 = ld8(plane, cs, pitch, dr + u, dc);
-        uint2 r1 = ld8(plane, cs, pitch, dr + u + 1, dc);
+        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+        const uint32_t o = boff + (4 * kk + 2 * h) * pitch;
+        const u32x2 r0 = *(const GLB u32x2 *)(plane + o);
+        const u32x2 r1 = *(const GLB u32x2 *)(plane + o + pitch);
         v4i t;
         t.x = (int)r0.x;
         t.y = (int)r0.y;
@@ -593,9 +607,8 @@ __device__ __forceinline__ void load_b(const GLB int8_t *plane, size_t cs, uint3
         t.w = (int)r1.y;
         *b16 = t;
     } else {
-        uint32_t k0 = 32 * kk + 16 * h;
-        uint32_t u = k0 / S, v0 = k0 % S;
-        *b16 = ld16(plane, cs, pitch, dr + u, dc + v0);
+        const uint32_t k0 = 32 * kk + 16 * h;
+        *b16 = *(const GLB v4i *)(plane + boff + (k0 / S) * pitch + (k0 % S));
     }
 }
 
@@ -610,9 +623,9 @@ __device__ __forceinline__ void load_a(const GLB int8_t *rng8, uint32_t g, int t
     int iso = 4 * tt + (m >> 3);
     const GLB int8_t *p = rng8 + ((size_t)(g * 8 + rl) * 8 + iso) * KMEM + kk * KSTEP + h * (KSTEP / 2);
     if constexpr (Geo<S>::SMALLK)
-        *a8 = *(const long *)p;
+        *a8 = *(const GLB long *)p;
     else
-        *a16 = *(const v4i *)p;
+        *a16 = *(const GLB v4i *)p;
 }
 
 struct BestRec {
@@ -684,8 +697,10 @@ template <int S> struct Queue {
 // half h owns ranges 4h..4h+3 and item t of row tile tt is range t%4,
 // isometry 4*tt + t/4.
 //
-// Correlation.  r' = r - 128 (i8), D = 256*Dh + Dl, Dl' = Dl - 128 (i8):
-//   t1_x4 = sum r*D = 256*sum r'Dh + sum r'Dl' + 128*(sum r' + S1)     (exact)
+// Correlation.  r' = r - 128 (i8), D = 4*(Dh' + 128) + Dl with Dh' = D/4 - 128
+// and Dl = D%4 (both i8):
+//   t1_x4 = sum r*D = 4*hi + lo + 512*sum r' + 128*S1,
+//   hi = sum r'Dh',  lo = sum r'Dl  (exact).
 //
 // Stage 1 (every candidate).  The least-squares residual with free slope
 // and offset is a lower bound of the quantised SSE:
@@ -695,10 +710,12 @@ template <int S> struct Queue {
 // reached, inflated so that anything above it has a strictly larger binary64
 // rms), a candidate can only win if Ni >= sqrt((A - n tau) Di), i.e.
 //   t1_x4 > floor((S1*t0 + sqrt((A - n tau) Di)) / n) - 1.
-// One integer compare per candidate; the limit is computed once per (range,
-// position), rounded so that it never excludes more than the exact bound.
-// Candidates with Ni < 0 are covered too (their qalfa is 0, SSE >= A/n).
-// Survivors are appended to a per-wave LDS queue (wave ballot + mbcnt).
+// The limit is computed once per (range, position) from the tile's four
+// thresholds (one LDS read issued before the MFMAs) and the position's
+// sqrt(Di), rounded so that it never excludes more than the exact bound;
+// then one integer compare per candidate.  Candidates with Ni < 0 are
+// covered too (their qalfa is 0, SSE >= A/n).  Survivors are appended to a
+// per-wave LDS queue (wave ballot + mbcnt).
 //
 // Stage 2 (queue batches of 64, binary64 with exact operands): the
 // candidate's quantised codes and SSE without any IEEE divide or square root.
@@ -731,7 +748,7 @@ __global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict
     const uint32_t nchunks = T.nchunks[0], tiles_per_chunk = T.tiles_per_chunk[0];
     const double nd = (double)n, inv_n = 1.0 / (double)n;
 
-    __shared__ uint32_t s_thr[8];
+    __shared__ __align__(16) uint32_t s_thr[8];
     __shared__ unsigned long long s_count[2];
     __shared__ int64_t s_T0[8], s_T2[8];
     __shared__ double s_A[8];
@@ -759,21 +776,22 @@ __global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict
     }
 
     // Range statistics of the lane's 4 ranges.
-    int64_t T0[4];
-    double A[4];
-    float Alo[4];
-    int32_t srp[4];
-    bool rvalid[4];
+    int32_t T0[4];  // t0 (< 2^24)
+    double A[4];    // A (sizes >= 16)
+    float Alo[4];   // A rounded down to binary32 (sizes <= 8)
+    int32_t off[4]; // t1 = 4*hi + lo + off + 128*S1
+    int32_t rmin[4]; // INT_MIN for a real range, INT_MAX for padding
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        uint32_t r = g * 8 + 4 * h + i;
-        rvalid[i] = r < T.nr;
-        uint32_t rr = rvalid[i] ? r : 0;
-        T0[i] = T.rstat[2 * rr];
-        const int64_t t2 = T.rstat[2 * rr + 1];
-        A[i] = (double)(n * t2 - T0[i] * T0[i]);
+        const uint32_t r = g * 8 + 4 * h + i;
+        const bool rv = r < T.nr;
+        const uint32_t rr = rv ? r : 0;
+        const int64_t t0 = T.rstat[2 * rr], t2 = T.rstat[2 * rr + 1];
+        T0[i] = (int32_t)t0;
+        A[i] = (double)(n * t2 - t0 * t0);
         Alo[i] = (float)A[i] * (1.0f - 3e-7f); // <= A after both roundings
-        srp[i] = (int32_t)(T0[i] - 128 * n);
+        off[i] = (int32_t)(512 * (t0 - 128 * n));
+        rmin[i] = rv ? (int32_t)0x80000000 : 0x7fffffff;
     }
 
     long a8[2][NK];
@@ -787,9 +805,9 @@ __global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict
         // The group's 8 ranges x 8 isometries are 64 contiguous rows of rng8.
         const GLB int8_t *src = T.rng8 + (size_t)g * 64 * G::KMEM;
         constexpr int CPR = G::KMEM / 16; // 16-byte chunks per row
-        for (int c = threadIdx.x; c < 64 * CPR; c += 256) {
-            const int row = c / CPR, col = c - row * CPR;
-            *(v4i *)(s_rng + row * ASTRIDE + col * 16) = *(const v4i *)(src + (size_t)row * G::KMEM + col * 16);
+        for (int cc = threadIdx.x; cc < 64 * CPR; cc += 256) {
+            const int row = cc / CPR, col = cc - row * CPR;
+            *(v4i *)(s_rng + row * ASTRIDE + col * 16) = *(const GLB v4i *)(src + (size_t)row * G::KMEM + col * 16);
         }
     }
     __syncthreads();
@@ -817,7 +835,7 @@ __global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict
         double rms = 0.0, sum = 0.0;
         if (act) {
             const uint4 pin = ld_u4(T.posinfo, p);
-            const int32_t S1 = (int32_t)pin.y, S2 = (int32_t)pin.z;
+            const int32_t S1 = (int32_t)pin.y, S2 = (int32_t)pin.w;
             const double Di = T.posd[2 * p], invDi = T.posd[2 * p + 1];
             const int64_t t0 = s_T0[slot], t2 = s_T2[slot];
             keep = !c.stage2 || stage2_keep(c, nd, inv_n, t1, S1, t0, Di, invDi, s_A[slot],
@@ -851,39 +869,42 @@ __global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict
     // (for sizes whose A fragments live in registers) the B fragments one
     // tile ahead, so neither load's latency sits in front of the MFMAs.
     struct Frag {
-        uint4 pin;
+        uint3 pin; // {window byte offset, S1, sqrt(Di) bits}
         long l8[NK], h8[NK];
         v4i l16[NK], h16[NK];
     };
     auto load_pin = [&](uint32_t tile) {
         const uint32_t p = tile * 32 + jj;
-        return ld_u4(T.posinfo, p < T.P ? p : T.P - 1);
+        return ld_u3(T.posinfo, p < T.P ? p : T.P - 1);
     };
-    auto fetch_b = [&](const uint4 pin, Frag &f) {
+    auto fetch_b = [&](const uint3 pin, Frag &f) {
         f.pin = pin;
         if constexpr (G::PRELOAD_A) {
-            const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
 #pragma unroll
             for (int kk = 0; kk < NK; kk++) {
-                load_b<S>(T.lo, T.copy_stride, T.pitch, dr, dc, kk, h, &f.l8[kk], &f.l16[kk]);
-                load_b<S>(T.hi, T.copy_stride, T.pitch, dr, dc, kk, h, &f.h8[kk], &f.h16[kk]);
+                load_b<S>(T.lo, pin.x, T.pitch, kk, h, &f.l8[kk], &f.l16[kk]);
+                load_b<S>(T.hi, pin.x, T.pitch, kk, h, &f.h8[kk], &f.h16[kk]);
             }
         }
     };
-    Frag nxt;
-    uint4 pin_ahead = make_uint4(0, 0, 0, 0);
-    if (t_begin + wave < t_end) fetch_b(load_pin(t_begin + wave), nxt);
+    Frag nxt = {};
+    uint3 pin_ahead = make_uint3(0, 0, 0);
+    const bool has_tiles = t_begin + wave < t_end;
+    if (has_tiles) fetch_b(load_pin(t_begin + wave), nxt);
     if (t_begin + wave + 4 < t_end) pin_ahead = load_pin(t_begin + wave + 4);
     // Sizes >= 16: window fragments in flight PD K-steps ahead, across tiles.
-    constexpr int PD = NK < 8 ? NK : 8;
+    constexpr int PD = S == 16 ? 4 : NK < 8 ? NK : 8;
     v4i pbl[PD], pbh[PD];
     if constexpr (!G::PRELOAD_A) {
-        const uint32_t dr0 = nxt.pin.x & 0xffff, dc0 = nxt.pin.x >> 16;
         long dummy8 = 0;
 #pragma unroll
         for (int d = 0; d < PD; d++) {
-            load_b<S>(T.lo, T.copy_stride, T.pitch, dr0, dc0, d, h, &dummy8, &pbl[d]);
-            load_b<S>(T.hi, T.copy_stride, T.pitch, dr0, dc0, d, h, &dummy8, &pbh[d]);
+            pbl[d] = (v4i){};
+            pbh[d] = (v4i){};
+            if (has_tiles) {
+                load_b<S>(T.lo, nxt.pin.x, T.pitch, d, h, &dummy8, &pbl[d]);
+                load_b<S>(T.hi, nxt.pin.x, T.pitch, d, h, &dummy8, &pbh[d]);
+            }
         }
     }
     const bool exchange = nchunks > 1;
@@ -892,17 +913,22 @@ __global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict
         // Exchange thresholds with the other blocks of this range group.
         if (exchange && (tcount & 15) == 15 && lane < 8) {
             uint32_t r = g * 8 + lane;
-            if (r < T.nr) atomicMin(&s_thr[lane], atomicMin((uint32_t *)&T.gthr[r], s_thr[lane]));
+            if (r < T.nr) atomicMin(&s_thr[lane], atomicMin((uint32_t *)&T.gthr[r], lds_load(&s_thr[lane])));
         }
+        // The four thresholds of the lane's ranges, read before the MFMAs so
+        // the LDS latency overlaps them.
+        uint32_t thb[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) thb[i] = lds_load(&s_thr[4 * h + i]);
         const Frag cur = nxt;
         if (tile + 4 < t_end) fetch_b(pin_ahead, nxt);
         if (tile + 8 < t_end) pin_ahead = load_pin(tile + 8);
         const uint32_t p = tile * 32 + jj;
-        const bool pv = p < T.P;
-        const uint4 pin = cur.pin;
-        const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
-        const int32_t S1p = (int32_t)pin.y;
-        const float Dif = __uint_as_float(pin.w);
+        const uint32_t boff = cur.pin.x;
+        const int32_t S1p = (int32_t)cur.pin.y;
+        const float sqDi = __uint_as_float(cur.pin.z);
+        // INT_MIN, or INT_MAX past the last position
+        const int32_t pmin = p < T.P ? (int32_t)0x80000000 : 0x7fffffff;
 
         v16i acc_lo[2], acc_hi[2];
 #pragma unroll
@@ -916,67 +942,71 @@ __global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict
 #pragma unroll
                 for (int tt = 0; tt < 2; tt++) {
                     if constexpr (G::SMALLK) {
-                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], cur.l8[kk], acc_lo[tt], 0, 0, 0);
                         acc_hi[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], cur.h8[kk], acc_hi[tt], 0, 0, 0);
+                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], cur.l8[kk], acc_lo[tt], 0, 0, 0);
                     } else {
-                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], cur.l16[kk], acc_lo[tt], 0, 0, 0);
                         acc_hi[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], cur.h16[kk], acc_hi[tt], 0, 0, 0);
+                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], cur.l16[kk], acc_lo[tt], 0, 0, 0);
                     }
                 }
             }
         } else {
             const bool has_next = tile + 4 < t_end;
-            const uint32_t ndr = nxt.pin.x & 0xffff, ndc = nxt.pin.x >> 16;
+            const uint32_t nboff = nxt.pin.x;
+            // Runtime loop over blocks of PD K-steps (only the block is
+            // unrolled, so the loads in flight stay at PD per plane).
+#pragma unroll 1
+            for (int k0 = 0; k0 < NK; k0 += PD) {
 #pragma unroll
-            for (int kk = 0; kk < NK; kk++) {
-                const int d = kk % PD;
-                const v4i a0 = *(const v4i *)(s_rng + arow0 * ASTRIDE + kk * 32 + h * 16);
-                const v4i a1 = *(const v4i *)(s_rng + (arow0 + 4) * ASTRIDE + kk * 32 + h * 16);
-                acc_lo[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbl[d], acc_lo[0], 0, 0, 0);
-                acc_hi[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbh[d], acc_hi[0], 0, 0, 0);
-                acc_lo[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbl[d], acc_lo[1], 0, 0, 0);
-                acc_hi[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbh[d], acc_hi[1], 0, 0, 0);
-                // Refill the slot with the fragment PD steps ahead.
-                long dummy8 = 0;
-                if (kk + PD < NK) {
-                    load_b<S>(T.lo, T.copy_stride, T.pitch, dr, dc, kk + PD, h, &dummy8, &pbl[d]);
-                    load_b<S>(T.hi, T.copy_stride, T.pitch, dr, dc, kk + PD, h, &dummy8, &pbh[d]);
-                } else if (has_next) {
-                    load_b<S>(T.lo, T.copy_stride, T.pitch, ndr, ndc, kk + PD - NK, h, &dummy8, &pbl[d]);
-                    load_b<S>(T.hi, T.copy_stride, T.pitch, ndr, ndc, kk + PD - NK, h, &dummy8, &pbh[d]);
+                for (int d = 0; d < PD; d++) {
+                    const int kk = k0 + d;
+                    const v4i a0 = *(const v4i *)(s_rng + arow0 * ASTRIDE + kk * 32 + h * 16);
+                    const v4i a1 = *(const v4i *)(s_rng + (arow0 + 4) * ASTRIDE + kk * 32 + h * 16);
+                    acc_hi[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbh[d], acc_hi[0], 0, 0, 0);
+                    acc_hi[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbh[d], acc_hi[1], 0, 0, 0);
+                    acc_lo[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbl[d], acc_lo[0], 0, 0, 0);
+                    acc_lo[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbl[d], acc_lo[1], 0, 0, 0);
+                    // Refill the slot with the fragment PD steps ahead.
+                    long dummy8 = 0;
+                    if (kk + PD < NK) {
+                        load_b<S>(T.lo, boff, T.pitch, kk + PD, h, &dummy8, &pbl[d]);
+                        load_b<S>(T.hi, boff, T.pitch, kk + PD, h, &dummy8, &pbh[d]);
+                    } else if (has_next) {
+                        load_b<S>(T.lo, nboff, T.pitch, kk + PD - NK, h, &dummy8, &pbl[d]);
+                        load_b<S>(T.hi, nboff, T.pitch, kk + PD - NK, h, &dummy8, &pbh[d]);
+                    }
                 }
             }
         }
 
-        // Stage-1 limit per (range, position) on the raw correlation.
-        int32_t lim[4], off[4];
+        // Stage-1 limit per (range, position) on v = 4*hi + lo.
+        int32_t lim[4], toff[4];
+        const int32_t o128 = S1p << 7;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            off[i] = 128 * (srp[i] + S1p);
-            const float thr = __uint_as_float(s_thr[4 * h + i]);
-            int32_t L;
+            toff[i] = off[i] + o128;
+            const float thr = __uint_as_float(thb[i]);
+            float amt;
             if constexpr (INT_LIMIT) {
-                const float amt = Alo[i] - (float)n * thr;
-                if (amt > 0.0f) {
-                    const float nt = __builtin_amdgcn_sqrtf(amt * Dif) * (1.0f - 1e-5f);
-                    const uint64_t X = (uint64_t)(uint32_t)S1p * (uint64_t)(uint32_t)T0[i] +
-                                       (uint64_t)(uint32_t)nt;
-                    L = (int32_t)(X >> LOG2N) - 1 - off[i];
-                } else {
-                    L = (int32_t)0x80000000;
-                }
+                amt = Alo[i] - (float)n * thr;
             } else {
                 const double nthr = nd * (double)thr;
-                if (nthr < A[i]) {
-                    const float amt = (float)(A[i] - nthr);
-                    const float nt = __builtin_amdgcn_sqrtf(amt * Dif) * (1.0f - 1e-5f);
-                    const double X = ((double)S1p * (double)T0[i] + (double)nt) * inv_n;
-                    L = (int32_t)floor(X) - 1 - off[i];
-                } else {
-                    L = (int32_t)0x80000000;
-                }
+                amt = nthr < A[i] ? (float)(A[i] - nthr) : 0.0f;
             }
-            lim[i] = (pv && rvalid[i]) ? L : 0x7fffffff;
+            // sqrt within 1 ulp; (1 - 2e-5) keeps nt below sqrt((A - n tau) Di)
+            // after the product's rounding (sqDi is sqrt(Di) rounded down).
+            const float nt = __builtin_amdgcn_sqrtf(fmaxf(amt, 0.0f)) * sqDi * (1.0f - 2e-5f);
+            int32_t L;
+            if constexpr (INT_LIMIT) {
+                // S1*t0 + nt < 2^31 for sizes <= 8, S1 and t0 < 2^24.
+                const uint32_t X = __umul24((uint32_t)S1p, (uint32_t)T0[i]) + (uint32_t)nt;
+                L = (int32_t)(X >> LOG2N) - 1 - toff[i];
+            } else {
+                const double X = ((double)S1p * (double)T0[i] + (double)nt) * inv_n;
+                L = (int32_t)floor(X) - 1 - toff[i];
+            }
+            L = amt > 0.0f ? L : (int32_t)0x80000000;
+            lim[i] = max(L, max(rmin[i], pmin));
         }
 
         // One compare per value; appends happen only for values that
@@ -988,7 +1018,7 @@ __global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict
 #pragma unroll
             for (int e = 0; e < DRAIN; e++) {
                 const int b = grp * DRAIN + e, tt = b >> 4, t = b & 15;
-                const int32_t v = (acc_hi[tt][t] << 8) + acc_lo[tt][t];
+                const int32_t v = 4 * acc_hi[tt][t] + acc_lo[tt][t];
                 const int i = t & 3;
                 const bool sv = v > lim[i];
                 const unsigned long long m = __ballot(sv);
@@ -997,7 +1027,7 @@ __global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict
                         (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
                     if (sv) {
                         const uint32_t slot = 4 * h + i;
-                        q[qn + pre] = make_uint2((uint32_t)(v + off[i]) | (slot << 28),
+                        q[qn + pre] = make_uint2((uint32_t)(v + toff[i]) | (slot << 28),
                                                  p * 8 + 4 * tt + (t >> 2));
                     }
                     qn += (uint32_t)__popcll(m);
@@ -1086,7 +1116,8 @@ __global__ __launch_bounds__(256) void k_search_fp32(const TableDesc *__restrict
         const bool pv = p < T.P;
         const uint32_t pc = pv ? p : T.P - 1;
         const uint4 pin = ld_u4(T.posinfo, pc);
-        const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
+        const uint32_t pi = pc / T.npc, pj = pc - pi * T.npc;
+        const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
         const GLB float *D = T.df + (size_t)dr * T.wc + dc;
         float acc[32];
 #pragma unroll
@@ -1101,7 +1132,7 @@ __global__ __launch_bounds__(256) void k_search_fp32(const TableDesc *__restrict
             }
         }
         if (!pv) continue;
-        const float S1 = (float)(int32_t)pin.y, S2 = (float)(int32_t)pin.z;
+        const float S1 = (float)(int32_t)pin.y, S2 = (float)(int32_t)pin.w;
 #pragma unroll
         for (int r = 0; r < 32; r++) {
             const int i = r & 3;
@@ -1155,7 +1186,7 @@ __device__ __forceinline__ void write_winner(const TableG &T, const Common &cm, 
     c.isometry = iso;
     c.s0 = (int64_t)S * S;
     c.s1_x4 = (int32_t)pin.y;
-    c.s2_x16 = (int32_t)pin.z;
+    c.s2_x16 = (int32_t)pin.w;
     c.t0 = T.rstat[2 * r];
     c.t1_x4 = t1;
     c.t2 = T.rstat[2 * r + 1];
@@ -1254,8 +1285,8 @@ __global__ void k_merge(const TableDesc *__restrict__ tabs, const Work2 *__restr
         return;
     }
     const uint32_t p = k >> 3, iso = k & 7, S = T.size;
-    const uint4 pin = ld_u4(T.posinfo, p);
-    const uint32_t dr = pin.x & 0xffff, dc = pin.x >> 16;
+    const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
+    const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
     {
         const uint32_t row = T.rows[r], col = T.cols[r];
         long long part = 0;
