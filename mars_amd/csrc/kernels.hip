@@ -731,7 +731,7 @@ template <int S> struct Queue {
 // best by (rms, key).  Winners tighten tau through LDS; blocks exchange tau
 // through a global array every 16 tiles.  Tau starts from k_seed's sample.
 template <int S>
-__global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict__ tabs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 8 ? 3 : 2))) void k_search_mfma(const TableDesc *__restrict__ tabs,
                                                      const Work3 *__restrict__ work, Common c)
 {
     using G = Geo<S>;
@@ -1009,29 +1009,44 @@ __global__ __launch_bounds__(256) void k_search_mfma(const TableDesc *__restrict
             lim[i] = max(L, max(rmin[i], pmin));
         }
 
-        // One compare per value; appends happen only for values that
-        // survive in some lane (a wave-uniform branch).  The queue is
-        // drained after every DRAIN values.
+        // One compare per value, issued 8 at a time into SGPR masks that
+        // the scalar unit ORs, so a group costs one branch; only a group
+        // where some lane passes goes value by value and appends (wave
+        // ballot + mbcnt).  The queue is drained after every DRAIN values.
         constexpr int DRAIN = Queue<S>::DRAIN;
+        constexpr int GW = DRAIN < 8 ? DRAIN : 8; // values per compare group
+        static_assert(32 % DRAIN == 0 && DRAIN % GW == 0, "groups tile the 32 values");
+        static_assert(64 * DRAIN + 64 <= Queue<S>::CAP, "a drain interval fits the queue");
 #pragma unroll
         for (int grp = 0; grp < 32 / DRAIN; grp++) {
 #pragma unroll
-            for (int e = 0; e < DRAIN; e++) {
-                const int b = grp * DRAIN + e, tt = b >> 4, t = b & 15;
-                const int32_t v = 4 * acc_hi[tt][t] + acc_lo[tt][t];
-                const int i = t & 3;
-                const bool sv = v > lim[i];
-                const unsigned long long m = __ballot(sv);
-                if (m) {
-                    const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
-                        (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                    if (sv) {
-                        const uint32_t slot = 4 * h + i;
-                        q[qn + pre] = make_uint2((uint32_t)(v + toff[i]) | (slot << 28),
-                                                 p * 8 + 4 * tt + (t >> 2));
+            for (int g0 = 0; g0 < DRAIN; g0 += GW) {
+                int32_t v[GW];
+                unsigned long long any = 0;
+#pragma unroll
+                for (int e = 0; e < GW; e++) {
+                    const int b = grp * DRAIN + g0 + e, tt = b >> 4, t = b & 15;
+                    v[e] = 4 * acc_hi[tt][t] + acc_lo[tt][t];
+                    any |= __ballot(v[e] > lim[t & 3]);
+                }
+                if (any == 0) continue;
+#pragma unroll
+                for (int e = 0; e < GW; e++) {
+                    const int b = grp * DRAIN + g0 + e, tt = b >> 4, t = b & 15;
+                    const int i = t & 3;
+                    const bool sv = v[e] > lim[i];
+                    const unsigned long long m = __ballot(sv);
+                    if (m) {
+                        const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                        if (sv) {
+                            const uint32_t slot = 4 * h + i;
+                            q[qn + pre] = make_uint2((uint32_t)(v[e] + toff[i]) | (slot << 28),
+                                                     p * 8 + 4 * tt + (t >> 2));
+                        }
+                        qn += (uint32_t)__popcll(m);
+                        qtotal += (uint32_t)__popcll(m);
                     }
-                    qn += (uint32_t)__popcll(m);
-                    qtotal += (uint32_t)__popcll(m);
                 }
             }
             // Drain full batches; keep the tail (< 64) for later.
