@@ -731,7 +731,7 @@ template <int S> struct Queue {
 // best by (rms, key).  Winners tighten tau through LDS; blocks exchange tau
 // through a global array every 16 tiles.  Tau starts from k_seed's sample.
 template <int S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 8 ? 3 : 2))) void k_search_mfma(const TableDesc *__restrict__ tabs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 : S <= 8 ? 3 : 2))) void k_search_mfma(const TableDesc *__restrict__ tabs,
                                                      const Work3 *__restrict__ work, Common c)
 {
     using G = Geo<S>;
@@ -930,55 +930,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 8 ? 3 
         // INT_MIN, or INT_MAX past the last position
         const int32_t pmin = p < T.P ? (int32_t)0x80000000 : 0x7fffffff;
 
-        v16i acc_lo[2], acc_hi[2];
-#pragma unroll
-        for (int tt = 0; tt < 2; tt++) {
-            acc_lo[tt] = (v16i){};
-            acc_hi[tt] = (v16i){};
-        }
-        if constexpr (G::PRELOAD_A) {
-#pragma unroll
-            for (int kk = 0; kk < NK; kk++) {
-#pragma unroll
-                for (int tt = 0; tt < 2; tt++) {
-                    if constexpr (G::SMALLK) {
-                        acc_hi[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], cur.h8[kk], acc_hi[tt], 0, 0, 0);
-                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], cur.l8[kk], acc_lo[tt], 0, 0, 0);
-                    } else {
-                        acc_hi[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], cur.h16[kk], acc_hi[tt], 0, 0, 0);
-                        acc_lo[tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], cur.l16[kk], acc_lo[tt], 0, 0, 0);
-                    }
-                }
-            }
-        } else {
-            const bool has_next = tile + 4 < t_end;
-            const uint32_t nboff = nxt.pin.x;
-            // Runtime loop over blocks of PD K-steps (only the block is
-            // unrolled, so the loads in flight stay at PD per plane).
-#pragma unroll 1
-            for (int k0 = 0; k0 < NK; k0 += PD) {
-#pragma unroll
-                for (int d = 0; d < PD; d++) {
-                    const int kk = k0 + d;
-                    const v4i a0 = *(const v4i *)(s_rng + arow0 * ASTRIDE + kk * 32 + h * 16);
-                    const v4i a1 = *(const v4i *)(s_rng + (arow0 + 4) * ASTRIDE + kk * 32 + h * 16);
-                    acc_hi[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbh[d], acc_hi[0], 0, 0, 0);
-                    acc_hi[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbh[d], acc_hi[1], 0, 0, 0);
-                    acc_lo[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbl[d], acc_lo[0], 0, 0, 0);
-                    acc_lo[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbl[d], acc_lo[1], 0, 0, 0);
-                    // Refill the slot with the fragment PD steps ahead.
-                    long dummy8 = 0;
-                    if (kk + PD < NK) {
-                        load_b<S>(T.lo, boff, T.pitch, kk + PD, h, &dummy8, &pbl[d]);
-                        load_b<S>(T.hi, boff, T.pitch, kk + PD, h, &dummy8, &pbh[d]);
-                    } else if (has_next) {
-                        load_b<S>(T.lo, nboff, T.pitch, kk + PD - NK, h, &dummy8, &pbl[d]);
-                        load_b<S>(T.hi, nboff, T.pitch, kk + PD - NK, h, &dummy8, &pbh[d]);
-                    }
-                }
-            }
-        }
-
         // Stage-1 limit per (range, position) on v = 4*hi + lo.
         int32_t lim[4], toff[4];
         const int32_t o128 = S1p << 7;
@@ -1009,55 +960,113 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 8 ? 3 
             lim[i] = max(L, max(rmin[i], pmin));
         }
 
-        // One compare per value, issued 8 at a time into SGPR masks that
-        // the scalar unit ORs, so a group costs one branch; only a group
-        // where some lane passes goes value by value and appends (wave
-        // ballot + mbcnt).  The queue is drained after every DRAIN values.
+        // Epilogue of one row tile: one compare per value, issued 8 at a
+        // time into SGPR masks that the scalar unit ORs, so a group costs one
+        // branch; only a group where some lane passes goes value by value and
+        // appends (wave ballot + mbcnt).  The queue is drained after every
+        // DRAIN values.
         constexpr int DRAIN = Queue<S>::DRAIN;
         constexpr int GW = DRAIN < 8 ? DRAIN : 8; // values per compare group
-        static_assert(32 % DRAIN == 0 && DRAIN % GW == 0, "groups tile the 32 values");
+        static_assert(16 % DRAIN == 0 && DRAIN % GW == 0, "groups tile a row tile's 16 values");
         static_assert(64 * DRAIN + 64 <= Queue<S>::CAP, "a drain interval fits the queue");
+        auto epilogue = [&](const v16i &ah, const v16i &al, const int tt) {
 #pragma unroll
-        for (int grp = 0; grp < 32 / DRAIN; grp++) {
+            for (int grp = 0; grp < 16 / DRAIN; grp++) {
 #pragma unroll
-            for (int g0 = 0; g0 < DRAIN; g0 += GW) {
-                int32_t v[GW];
-                unsigned long long any = 0;
+                for (int g0 = 0; g0 < DRAIN; g0 += GW) {
+                    int32_t v[GW];
+                    unsigned long long any = 0;
 #pragma unroll
-                for (int e = 0; e < GW; e++) {
-                    const int b = grp * DRAIN + g0 + e, tt = b >> 4, t = b & 15;
-                    v[e] = 4 * acc_hi[tt][t] + acc_lo[tt][t];
-                    any |= __ballot(v[e] > lim[t & 3]);
-                }
-                if (any == 0) continue;
+                    for (int e = 0; e < GW; e++) {
+                        const int t = grp * DRAIN + g0 + e;
+                        v[e] = 4 * ah[t] + al[t];
+                        any |= __ballot(v[e] > lim[t & 3]);
+                    }
+                    if (any == 0) continue;
 #pragma unroll
-                for (int e = 0; e < GW; e++) {
-                    const int b = grp * DRAIN + g0 + e, tt = b >> 4, t = b & 15;
-                    const int i = t & 3;
-                    const bool sv = v[e] > lim[i];
-                    const unsigned long long m = __ballot(sv);
-                    if (m) {
-                        const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
-                            (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                        if (sv) {
-                            const uint32_t slot = 4 * h + i;
-                            q[qn + pre] = make_uint2((uint32_t)(v[e] + toff[i]) | (slot << 28),
-                                                     p * 8 + 4 * tt + (t >> 2));
+                    for (int e = 0; e < GW; e++) {
+                        const int t = grp * DRAIN + g0 + e, i = t & 3;
+                        const bool sv = v[e] > lim[i];
+                        const unsigned long long m = __ballot(sv);
+                        if (m) {
+                            const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
+                                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                            if (sv) {
+                                const uint32_t slot = 4 * h + i;
+                                q[qn + pre] = make_uint2((uint32_t)(v[e] + toff[i]) | (slot << 28),
+                                                         p * 8 + 4 * tt + (t >> 2));
+                            }
+                            qn += (uint32_t)__popcll(m);
+                            qtotal += (uint32_t)__popcll(m);
                         }
-                        qn += (uint32_t)__popcll(m);
-                        qtotal += (uint32_t)__popcll(m);
+                    }
+                }
+                // Drain full batches; keep the tail (< 64) for later.
+                if (qn >= 64) {
+                    uint32_t b0 = 0;
+                    for (; qn - b0 >= 64; b0 += 64) process(b0, 64);
+                    const uint32_t rem = qn - b0;
+                    const uint2 tail = q[b0 + ((uint32_t)lane < rem ? lane : 0)];
+                    if ((uint32_t)lane < rem) q[lane] = tail;
+                    qn = rem;
+                }
+            }
+        };
+
+        if constexpr (G::PRELOAD_A) {
+            // Row tile by row tile: row tile 1's MFMAs run while row tile
+            // 0's epilogue uses the VALU, and only one tile's accumulators
+            // are live at a time.
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++) {
+                v16i ah = (v16i){}, al = (v16i){};
+#pragma unroll
+                for (int kk = 0; kk < NK; kk++) {
+                    if constexpr (G::SMALLK) {
+                        ah = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], cur.h8[kk], ah, 0, 0, 0);
+                        al = __builtin_amdgcn_mfma_i32_32x32x16_i8(a8[tt][kk], cur.l8[kk], al, 0, 0, 0);
+                    } else {
+                        ah = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], cur.h16[kk], ah, 0, 0, 0);
+                        al = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], cur.l16[kk], al, 0, 0, 0);
+                    }
+                }
+                epilogue(ah, al, tt);
+            }
+        } else {
+            v16i acc_lo[2], acc_hi[2];
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++) {
+                acc_lo[tt] = (v16i){};
+                acc_hi[tt] = (v16i){};
+            }
+            const bool has_next = tile + 4 < t_end;
+            const uint32_t nboff = nxt.pin.x;
+            // Runtime loop over blocks of PD K-steps (only the block is
+            // unrolled, so the loads in flight stay at PD per plane).
+#pragma unroll 1
+            for (int k0 = 0; k0 < NK; k0 += PD) {
+#pragma unroll
+                for (int d = 0; d < PD; d++) {
+                    const int kk = k0 + d;
+                    const v4i a0 = *(const v4i *)(s_rng + arow0 * ASTRIDE + kk * 32 + h * 16);
+                    const v4i a1 = *(const v4i *)(s_rng + (arow0 + 4) * ASTRIDE + kk * 32 + h * 16);
+                    acc_hi[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbh[d], acc_hi[0], 0, 0, 0);
+                    acc_hi[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbh[d], acc_hi[1], 0, 0, 0);
+                    acc_lo[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbl[d], acc_lo[0], 0, 0, 0);
+                    acc_lo[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbl[d], acc_lo[1], 0, 0, 0);
+                    // Refill the slot with the fragment PD steps ahead.
+                    long dummy8 = 0;
+                    if (kk + PD < NK) {
+                        load_b<S>(T.lo, boff, T.pitch, kk + PD, h, &dummy8, &pbl[d]);
+                        load_b<S>(T.hi, boff, T.pitch, kk + PD, h, &dummy8, &pbh[d]);
+                    } else if (has_next) {
+                        load_b<S>(T.lo, nboff, T.pitch, kk + PD - NK, h, &dummy8, &pbl[d]);
+                        load_b<S>(T.hi, nboff, T.pitch, kk + PD - NK, h, &dummy8, &pbh[d]);
                     }
                 }
             }
-            // Drain full batches; keep the tail (< 64) for later.
-            if (qn >= 64) {
-                uint32_t b0 = 0;
-                for (; qn - b0 >= 64; b0 += 64) process(b0, 64);
-                const uint32_t rem = qn - b0;
-                const uint2 tail = q[b0 + ((uint32_t)lane < rem ? lane : 0)];
-                if ((uint32_t)lane < rem) q[lane] = tail;
-                qn = rem;
-            }
+            epilogue(acc_hi[0], acc_lo[0], 0);
+            epilogue(acc_hi[1], acc_lo[1], 1);
         }
     }
     if (qn > 0) process(0, qn);
