@@ -47,6 +47,7 @@ def _gpu_leg(steps: int, warmup: int):
     for name, kern in (("mfma_i8", N.KERNEL_MFMA_I8), ("fp32", N.KERNEL_FP32)):
         ctx = HipContext(0, kern)
         ctx.prepare(img, params, [SHIFT])
+        tune = ctx.autotune(reps=3)  # chunking shape per size, timed on this workload
         if warmup:
             ctx.run(iters=warmup, use_graph=True)
         # Per-kernel breakdown: average of several eager steps (HIP events on
@@ -65,6 +66,8 @@ def _gpu_leg(steps: int, warmup: int):
                                             for i in range(6) if br[0].ms_by_size[i] > 0},
                          exact_fits_by_size={str(2 << i): int(br[0].exact_fits_by_size[i])
                                              for i in range(6) if br[0].ms_by_size[i] > 0},
+                         shapes={str(2 << i): [int(tune.target_blocks[i]), int(tune.min_tiles[i])]
+                                 for i in range(6) if tune.ms[i] > 0},
                          winners=[[(c.dom_row, c.dom_col, c.isometry, c.qalfa, c.qbeta)
                                    if c is not None else None for c in tt[4]] for tt in tabs])
         ctx.close()
@@ -231,6 +234,7 @@ def main() -> int:
         "fp32_kernel": {"value": fp_value, "unit": "evals/s", "ms_per_step": fp["ms_step"],
                         "ranges_differing_from_mfma_winner": diverge},
         "speedup_vs_fp32": value / fp_value,
+        "autotuned_shapes": {"mfma_i8": mf["shapes"], "fp32": fp["shapes"]},
         "breakdown_ms": {"pool_and_ranges": mf["ms_pool"], "search": mf["ms_search"],
                          "search_by_size": mf["ms_by_size"],
                          "fp32_search_by_size": fp["ms_by_size"]},

@@ -54,7 +54,7 @@ struct Shape {
     uint32_t target_blocks = 2048; // >> 256 CUs: fill the chip per launch
     uint32_t min_tiles = 8;        // >= 2 tiles per wave
 };
-constexpr uint32_t kShapeTargets[] = {1024, 2048, 4096, 8192, 16384};
+constexpr uint32_t kShapeTargets[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
 constexpr uint32_t kShapeMinTiles[] = {4, 8, 16};
 constexpr uint32_t kMaxTarget = 16384, kMinTiles = 4; // bounds of the autotuner's space
 
@@ -83,7 +83,11 @@ struct Table {
     int32_t *d_rpos = nullptr;
     uint32_t *d_gthr = nullptr;
     uint4 *d_posinfo = nullptr;
+    uint32_t *d_iperm = nullptr;
+    int32_t *d_posx = nullptr;
     double *d_posd = nullptr;
+    uint32_t P_pad = 0;
+    std::vector<uint32_t> iperm; // search slot of each position (empty: identity)
     Partial *d_partial = nullptr;
     mars_hip_candidate *d_out = nullptr;
     uint8_t *d_found = nullptr;
@@ -253,6 +257,26 @@ void chunking(uint32_t ntiles, uint32_t ngroups, uint32_t target, uint32_t min_t
 uint32_t tiles_of(const Table &t, int k) { return cdiv(t.P, k == 0 ? 32 : 64); }
 uint32_t groups_of(const Table &t, int k) { return t.nr_pad / (k == 0 ? 8 : 4); }
 
+// Search order of a table's positions.  For sizes >= 16 the MFMA search
+// loads each lane's window row segments (16 bytes) straight from the split
+// planes; a tile whose 32 lanes take positions of one domain row whose
+// window origins lie 16 bytes apart turns every such load into one
+// contiguous run per half-wave.  Positions are therefore ordered by domain
+// row, then by column residue modulo 32/shift, then by column (when shift
+// divides 32; otherwise the natural order is kept).
+void search_order(Table &t)
+{
+    t.iperm.clear();
+    if (t.size < 16 || t.P == 0 || 32 % t.shift != 0) return;
+    const uint32_t m = 32 / t.shift; // column step whose window origins are 16 bytes apart
+    if (m == 1) return;              // already contiguous
+    t.iperm.resize(t.P);
+    uint32_t q = 0;
+    for (uint32_t pi = 0; pi < t.npr; pi++)
+        for (uint32_t c = 0; c < m; c++)
+            for (uint32_t pj = c; pj < t.npc; pj += m) t.iperm[pi * t.npc + pj] = q++;
+}
+
 // Bump allocator over one device allocation.
 struct ArenaPlan {
     struct Item {
@@ -330,6 +354,8 @@ int layout(mars_hip_ctx *ctx)
         d.rpos = t.d_rpos;
         d.gthr = t.d_gthr;
         d.posinfo = t.d_posinfo;
+        d.iperm = t.iperm.empty() ? nullptr : t.d_iperm;
+        d.posx = t.d_posx;
         d.posd = t.d_posd;
         d.partial = t.d_partial;
         d.out = t.d_out;
@@ -337,6 +363,7 @@ int layout(mars_hip_ctx *ctx)
         d.size = t.size;
         d.shift = t.shift;
         d.P = t.P;
+        d.P_pad = t.P_pad;
         d.npc = t.npc;
         d.nr = t.nr;
         d.nr_pad = t.nr_pad;
@@ -368,7 +395,7 @@ int layout(mars_hip_ctx *ctx)
     end2(ctx->wl_sat_cols);
     begin2(ctx->wl_possums);
     for (uint32_t i = 0; i < ctx->tables.size(); i++)
-        for (uint32_t b = 0; b < cdiv(ctx->tables[i].P, 256); b++) w2.push_back(make_uint2(i, b));
+        for (uint32_t b = 0; b < cdiv(ctx->tables[i].P_pad, 256); b++) w2.push_back(make_uint2(i, b));
     end2(ctx->wl_possums);
     begin2(ctx->wl_ranges);
     for (uint32_t i = 0; i < ctx->tables.size(); i++) {
@@ -451,11 +478,26 @@ void enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st)
         mars::launch_seed(ctx->d_tabs, w2 + ctx->wl_seed.off, ctx->wl_seed.n, ctx->common, st);
 }
 
+// Sizes whose MFMA search first runs a threshold pre-pass over every
+// kPrepassStride-th position tile: their stage-1 bound lets many candidates
+// through until the per-range thresholds are close to final, so tightening
+// them on a sample first cuts the exact fits of the full pass.
+constexpr uint32_t kPrepassMinSize = 16, kPrepassStride = 8;
+
 int enqueue_search(mars_hip_ctx *ctx, const Group &g, int kernel, hipStream_t st)
 {
     const int k = kernel == MARS_HIP_KERNEL_MFMA_I8 ? 0 : 1;
+    Common c = ctx->common;
+    c.sample = 1;
+    if (k == 0 && g.size >= kPrepassMinSize) {
+        Common pre = c;
+        pre.sample = kPrepassStride;
+        if (mars::launch_search(kernel, g.size, ctx->d_tabs, ctx->d_work3 + g.search[k].off,
+                                g.search[k].n, pre, st) != 0)
+            return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
+    }
     if (mars::launch_search(kernel, g.size, ctx->d_tabs, ctx->d_work3 + g.search[k].off,
-                            g.search[k].n, ctx->common, st) != 0)
+                            g.search[k].n, c, st) != 0)
         return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
     return MARS_HIP_OK;
 }
@@ -626,6 +668,8 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
                 }
                 t.P = t.npr * t.npc;
                 if (t.P > (0xffffffffu / 8)) return fail(ctx, MARS_HIP_E_UNSUPPORTED, "too many positions");
+                t.P_pad = (t.P + 31) / 32 * 32;
+                search_order(t);
                 ctx->table_of[std::make_tuple(pi, t.size, shift)] = (uint32_t)ctx->tables.size();
                 ctx->tables.push_back(std::move(t));
             }
@@ -682,7 +726,9 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         plan.add(&t.d_rstat, sizeof(int32_t) * 2 * t.nr_pad);
         plan.add(&t.d_rpos, sizeof(int32_t) * t.nr_pad);
         plan.add(&t.d_gthr, sizeof(uint32_t) * t.nr_pad);
-        plan.add(&t.d_posinfo, sizeof(uint4) * (size_t)t.P);
+        plan.add(&t.d_posinfo, sizeof(uint4) * (size_t)t.P_pad);
+        plan.add(&t.d_iperm, sizeof(uint32_t) * t.iperm.size());
+        plan.add(&t.d_posx, 2 * sizeof(int32_t) * (size_t)t.P);
         plan.add(&t.d_posd, 2 * sizeof(double) * (size_t)t.P);
         plan.add(&t.d_partial, sizeof(Partial) * (size_t)t.nr_pad * std::max(t.nch_cap[0], t.nch_cap[1]));
         plan.add(&t.d_out, sizeof(mars_hip_candidate) * (size_t)t.nr);
@@ -700,6 +746,9 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
     for (Table &t : ctx->tables) {
         HIP_TRY(ctx, hipMemcpyAsync(t.d_rows, t.rows.data(), 4 * (size_t)t.nr, hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(t.d_cols, t.cols.data(), 4 * (size_t)t.nr, hipMemcpyHostToDevice, ctx->stream));
+        if (!t.iperm.empty())
+            HIP_TRY(ctx, hipMemcpyAsync(t.d_iperm, t.iperm.data(), 4 * t.iperm.size(),
+                                        hipMemcpyHostToDevice, ctx->stream));
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
@@ -728,6 +777,7 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         HIP_TRY(ctx, hipMemcpy(ctx->d_yscale, ys.data(), sizeof(double) * nq, hipMemcpyHostToDevice));
         c.yscale = ctx->d_yscale;
         c.counters = ctx->d_counters;
+        c.sample = 1;
     }
     RC_TRY(layout(ctx));
     RC_TRY(enqueue_step(ctx, ctx->kernel, nullptr));

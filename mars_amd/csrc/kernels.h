@@ -49,12 +49,18 @@ struct TableDesc {
     int32_t *rstat;      // [nr_pad][2] t0, t2
     int32_t *rpos;       // [nr_pad] 3 * sum max(r - 128, 0)
     uint32_t *gthr;      // [nr_pad] pruning threshold (float bits, SSE units)
-    uint4 *posinfo;      // [P] {window byte offset, S1, sqrt(n*S2 - S1^2) as f32 rounded down, S2}
+    // Search-order records, P_pad = 32 * ceil(P / 32) slots:
+    // {window byte offset, S1, sqrt(n*S2 - S1^2) as f32 rounded down, p}
+    // (p = ~0 in padding slots).  Slot of position p: iperm[p], or p when
+    // iperm is null.
+    uint4 *posinfo;
+    const uint32_t *iperm;
+    int32_t *posx;       // [P][2] S1, S2
     double *posd;        // [P][2] Di = n*S2 - S1^2, 1/Di (binary64, stage 2)
     Partial *partial;    // [nr_pad][nchunks]
     mars_hip_candidate *out;
     uint8_t *found;
-    uint32_t size, shift, P, npc, nr, nr_pad, kmem;
+    uint32_t size, shift, P, P_pad, npc, nr, nr_pad, kmem;
     // Position chunking of the search: [0] MFMA kernel (32-position
     // tiles), [1] FP32 baseline (64-position tiles).  partial holds
     // nr_pad x nchunks[kernel] entries.
@@ -72,6 +78,7 @@ struct Common {
     double qa_max, qb_max; // 2^bits_alfa - 1, 2^bits_beta - 1
     const double *yscale; // [2^bits_alfa] (2^bits_beta-1) / ((1 + alfa2) * 255)
     unsigned long long *counters; // [16]: [log2 s] stage-1 survivors, [8 + log2 s] stage-3 fits
+    uint32_t sample;              // MFMA search: 1, or the tile stride of a threshold pre-pass
 };
 
 // Work items: x = descriptor index, y/z = kernel-specific block coordinates.

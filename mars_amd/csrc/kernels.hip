@@ -124,11 +124,13 @@ struct TableG {
     GLB int32_t *rpos;
     GLB uint32_t *gthr;
     GLB u32x4 *posinfo;
+    GLB int32_t *posx;
+    const GLB uint32_t *iperm;
     GLB double *posd;
     GLB Partial *partial;
     GLB mars_hip_candidate *out;
     GLB uint8_t *found;
-    uint32_t size, shift, P, npc, nr, nr_pad, kmem;
+    uint32_t size, shift, P, P_pad, npc, nr, nr_pad, kmem;
     uint32_t nchunks[2], tiles_per_chunk[2];
 };
 
@@ -154,6 +156,8 @@ __device__ __forceinline__ TableG view(const TableDesc &t)
     g.rpos = glb(t.rpos);
     g.gthr = glb(t.gthr);
     g.posinfo = (GLB u32x4 *)t.posinfo;
+    g.posx = glb(t.posx);
+    g.iperm = glb(t.iperm);
     g.posd = glb(t.posd);
     g.partial = glb(t.partial);
     g.out = glb(t.out);
@@ -161,6 +165,7 @@ __device__ __forceinline__ TableG view(const TableDesc &t)
     g.size = t.size;
     g.shift = t.shift;
     g.P = t.P;
+    g.P_pad = t.P_pad;
     g.npc = t.npc;
     g.nr = t.nr;
     g.nr_pad = t.nr_pad;
@@ -416,15 +421,20 @@ __global__ void k_sat_cols(const PlaneDesc *__restrict__ planes, const Work2 *__
 }
 
 // Work item (table, block of 256 domain positions): window sums from the
-// SATs, packed as {window byte offset, S1, sqrt(Di) as f32, S2} plus
-// {Di, 1/Di} in binary64.  The search's hot loop reads the first three
-// words only.
+// SATs.  Per position p: {S1, S2} and {Di, 1/Di} (binary64); the search's
+// hot record {window byte offset, S1, sqrt(Di) as f32, p} goes to slot
+// iperm[p] of the search order (identity when iperm is null), and slots
+// P..P_pad-1 get a padding record (p = ~0).
 __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work)
 {
     const Work2 wk = work[blockIdx.x];
     const TableG T = view(tabs[wk.x]);
     const uint32_t p = wk.y * blockDim.x + threadIdx.x;
-    if (p >= T.P) return;
+    if (p >= T.P_pad) return;
+    if (p >= T.P) { // padding slot of the last search tile: no position
+        T.posinfo[p] = (u32x4){0u, 0u, 0u, 0xffffffffu};
+        return;
+    }
     const uint32_t W1 = T.wc + 1, size = T.size;
     const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
     const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
@@ -440,7 +450,10 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
     const uint32_t cmask = size == 2 ? 0 : size == 4 ? 3 : size == 8 ? 7 : 15;
     const uint32_t cp = dc & cmask;
     const uint32_t boff = cp * T.copy_stride + dr * T.pitch + (dc - cp);
-    T.posinfo[p] = (u32x4){boff, (uint32_t)s1, __float_as_uint(sq), (uint32_t)s2};
+    const uint32_t q = T.iperm ? T.iperm[p] : p; // slot in search order
+    T.posinfo[q] = (u32x4){boff, (uint32_t)s1, __float_as_uint(sq), p};
+    T.posx[2 * p] = (int32_t)s1;
+    T.posx[2 * p + 1] = (int32_t)s2;
     T.posd[2 * p] = (double)di;
     T.posd[2 * p + 1] = di > 0 ? 1.0 / (double)di : 0.0;
 }
@@ -464,7 +477,7 @@ __global__ void k_seed(const TableDesc *__restrict__ tabs, const Work2 *__restri
     __syncthreads();
     const uint32_t j = c >> 3, iso = c & 7;
     const uint32_t p = P >= npos ? (uint32_t)(((uint64_t)j * (P - 1)) / (npos - 1)) : j % P;
-    const uint4 pin = ld_u4(T.posinfo, p);
+    const int32_t S1 = T.posx[2 * p], S2 = T.posx[2 * p + 1];
     const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
     const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
     const GLB int8_t *rk = T.rng8 + ((size_t)r * 8 + iso) * T.kmem;
@@ -483,7 +496,7 @@ __global__ void k_seed(const TableDesc *__restrict__ tabs, const Work2 *__restri
         const int64_t n = (int64_t)K;
         uint32_t qa, qb;
         double rms, sum;
-        fit_f64(n, (int32_t)pin.y, (int32_t)pin.w, T.rstat[2 * r], s_t1[c], T.rstat[2 * r + 1],
+        fit_f64(n, S1, S2, T.rstat[2 * r], s_t1[c], T.rstat[2 * r + 1],
                 cm.max_alfa, cm.bits_alfa, cm.bits_beta, &qa, &qb, &rms, &sum);
         double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
         atomicMin(&s_min, __float_as_uint((float)(tau * (1.0 + 1e-7))));
@@ -802,12 +815,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
 #pragma unroll
             for (int kk = 0; kk < NK; kk++) load_a<S>(T.rng8, g, tt, kk, lane, &a8[tt][kk], &a16[tt][kk]);
     } else {
-        // The group's 8 ranges x 8 isometries are 64 contiguous rows of rng8.
+        // The group's 8 ranges x 8 isometries are 64 contiguous rows of rng8;
+        // every load of a thread is issued before the first LDS write, so
+        // the staging costs one memory latency, not one per chunk.
         const GLB int8_t *src = T.rng8 + (size_t)g * 64 * G::KMEM;
         constexpr int CPR = G::KMEM / 16; // 16-byte chunks per row
-        for (int cc = threadIdx.x; cc < 64 * CPR; cc += 256) {
-            const int row = cc / CPR, col = cc - row * CPR;
-            *(v4i *)(s_rng + row * ASTRIDE + col * 16) = *(const GLB v4i *)(src + (size_t)row * G::KMEM + col * 16);
+        constexpr int PER = 64 * CPR / 256; // chunks per thread
+        static_assert(64 * CPR % 256 == 0, "chunks divide over the block");
+        v4i stg[PER];
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int cc = threadIdx.x + 256 * j;
+            stg[j] = *(const GLB v4i *)(src + (size_t)cc * 16);
+        }
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const int cc = threadIdx.x + 256 * j, row = cc / CPR, col = cc - row * CPR;
+            *(v4i *)(s_rng + row * ASTRIDE + col * 16) = stg[j];
         }
     }
     __syncthreads();
@@ -815,9 +839,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     const int am = lane & 31;
     const int arow0 = (((am >> 2) & 1) * 4 + (am & 3)) * 8 + (am >> 3);
 
-    const uint32_t ntiles = (T.P + 31) / 32;
-    const uint32_t t_begin = chunk * tiles_per_chunk;
-    uint32_t t_end = t_begin + tiles_per_chunk;
+    // With c.sample > 1 (threshold pre-pass) the block covers every
+    // sample-th tile: tile index i below stands for tile i * sample.
+    const uint32_t sample = c.sample;
+    const uint32_t ntiles = (T.P_pad / 32 + sample - 1) / sample;
+    const uint32_t tpc = (tiles_per_chunk + sample - 1) / sample;
+    const uint32_t t_begin = chunk * tpc;
+    uint32_t t_end = t_begin + tpc;
     if (t_end > ntiles) t_end = ntiles;
     unsigned long long qtotal = 0, stage3 = 0; // stage-1 survivors (wave), stage-3 fits (lane)
     uint32_t tcount = 0;
@@ -834,8 +862,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         bool keep = false, win = false;
         double rms = 0.0, sum = 0.0;
         if (act) {
-            const uint4 pin = ld_u4(T.posinfo, p);
-            const int32_t S1 = (int32_t)pin.y, S2 = (int32_t)pin.w;
+            const int32_t S1 = T.posx[2 * p], S2 = T.posx[2 * p + 1];
             const double Di = T.posd[2 * p], invDi = T.posd[2 * p + 1];
             const int64_t t0 = s_T0[slot], t2 = s_T2[slot];
             keep = !c.stage2 || stage2_keep(c, nd, inv_n, t1, S1, t0, Di, invDi, s_A[slot],
@@ -869,15 +896,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // (for sizes whose A fragments live in registers) the B fragments one
     // tile ahead, so neither load's latency sits in front of the MFMAs.
     struct Frag {
-        uint3 pin; // {window byte offset, S1, sqrt(Di) bits}
+        uint4 pin; // {window byte offset, S1, sqrt(Di) bits, position index or ~0}
         long l8[NK], h8[NK];
         v4i l16[NK], h16[NK];
     };
-    auto load_pin = [&](uint32_t tile) {
-        const uint32_t p = tile * 32 + jj;
-        return ld_u3(T.posinfo, p < T.P ? p : T.P - 1);
-    };
-    auto fetch_b = [&](const uint3 pin, Frag &f) {
+    auto load_pin = [&](uint32_t tile) { return ld_u4(T.posinfo, tile * sample * 32 + jj); };
+    auto fetch_b = [&](const uint4 pin, Frag &f) {
         f.pin = pin;
         if constexpr (G::PRELOAD_A) {
 #pragma unroll
@@ -888,7 +912,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         }
     };
     Frag nxt = {};
-    uint3 pin_ahead = make_uint3(0, 0, 0);
+    uint4 pin_ahead = make_uint4(0, 0, 0, 0);
     const bool has_tiles = t_begin + wave < t_end;
     if (has_tiles) fetch_b(load_pin(t_begin + wave), nxt);
     if (t_begin + wave + 4 < t_end) pin_ahead = load_pin(t_begin + wave + 4);
@@ -923,11 +947,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         const Frag cur = nxt;
         if (tile + 4 < t_end) fetch_b(pin_ahead, nxt);
         if (tile + 8 < t_end) pin_ahead = load_pin(tile + 8);
-        const uint32_t p = tile * 32 + jj;
+        const uint32_t p = cur.pin.w; // position index, ~0 for padding
         const uint32_t boff = cur.pin.x;
         const int32_t S1p = (int32_t)cur.pin.y;
         const float sqDi = __uint_as_float(cur.pin.z);
-        // INT_MIN, or INT_MAX past the last position
+        // INT_MIN, or INT_MAX for a padding slot
         const int32_t pmin = p < T.P ? (int32_t)0x80000000 : 0x7fffffff;
 
         // Stage-1 limit per (range, position) on v = 4*hi + lo.
@@ -1081,12 +1105,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     if (threadIdx.x < 8) {
         const uint32_t sl = threadIdx.x, r = g * 8 + sl;
         if (r < T.nr) {
+            // The block's tightest threshold for later blocks and passes.
+            atomicMin((uint32_t *)&T.gthr[r], lds_load(&s_thr[sl]));
             BestRec b = s_wb[0][sl];
             for (int wv = 1; wv < 4; wv++) {
                 const BestRec &e = s_wb[wv][sl];
                 if (better(e.rms, e.key, b.rms, b.key)) b = e;
             }
-            st_partial(T.partial, (size_t)r * nchunks + chunk, b.rms, b.key, b.t1);
+            // The pre-pass only tightens thresholds; the full pass writes
+            // every partial.
+            if (sample == 1) st_partial(T.partial, (size_t)r * nchunks + chunk, b.rms, b.key, b.t1);
         }
     }
 }
@@ -1139,7 +1167,7 @@ __global__ __launch_bounds__(256) void k_search_fp32(const TableDesc *__restrict
         const uint32_t p = tile * 64 + lane;
         const bool pv = p < T.P;
         const uint32_t pc = pv ? p : T.P - 1;
-        const uint4 pin = ld_u4(T.posinfo, pc);
+        const int32_t S1i = T.posx[2 * pc], S2i = T.posx[2 * pc + 1];
         const uint32_t pi = pc / T.npc, pj = pc - pi * T.npc;
         const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
         const GLB float *D = T.df + (size_t)dr * T.wc + dc;
@@ -1156,7 +1184,7 @@ __global__ __launch_bounds__(256) void k_search_fp32(const TableDesc *__restrict
             }
         }
         if (!pv) continue;
-        const float S1 = (float)(int32_t)pin.y, S2 = (float)(int32_t)pin.w;
+        const float S1 = (float)S1i, S2 = (float)S2i;
 #pragma unroll
         for (int r = 0; r < 32; r++) {
             const int i = r & 3;
@@ -1203,14 +1231,13 @@ __device__ __forceinline__ void write_winner(const TableG &T, const Common &cm, 
 {
     mars_hip_candidate c = {};
     const uint32_t p = k >> 3, iso = k & 7, S = T.size;
-    const uint4 pin = ld_u4(T.posinfo, p);
     const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
     c.dom_row = pi * T.shift;
     c.dom_col = pj * T.shift;
     c.isometry = iso;
     c.s0 = (int64_t)S * S;
-    c.s1_x4 = (int32_t)pin.y;
-    c.s2_x16 = (int32_t)pin.w;
+    c.s1_x4 = T.posx[2 * p];
+    c.s2_x16 = T.posx[2 * p + 1];
     c.t0 = T.rstat[2 * r];
     c.t1_x4 = t1;
     c.t2 = T.rstat[2 * r + 1];
