@@ -120,7 +120,7 @@ def _cpu_leg(img, budget_s: float = 12.0):
     t0 = time.perf_counter()
     used = {}
     rounds = 0
-    while time.perf_counter() - t0 < budget_s and rounds < 50:
+    while time.perf_counter() - t0 < budget_s and rounds < 1000:
         for size in sorted(nodes):
             pick = rng.choice(len(nodes[size]), size=threads, replace=False)
             rows = [nodes[size][i][0] for i in pick]
@@ -137,8 +137,9 @@ def _cpu_leg(img, budget_s: float = 12.0):
 
 
 def _traffic_probe() -> int:
-    """One eager step of the MFMA search and one 256 MiB copy kernel: the
-    command rocprofv3 --pmc runs to read FETCH_SIZE / WRITE_SIZE."""
+    """Autotuned shapes, one eager step of the MFMA search and one 256 MiB copy
+    kernel: the command rocprofv3 --pmc runs to read FETCH_SIZE / WRITE_SIZE
+    (the step's launches are the last of each search kernel)."""
     from mars_amd import _native as N
     from mars_amd import synth
     from mars_amd.provider import EncodeParams, HipContext
@@ -146,6 +147,7 @@ def _traffic_probe() -> int:
     ctx = HipContext(0, N.KERNEL_MFMA_I8)
     ctx.prepare(synth.plane(W, H, SEED), EncodeParams(min_size=MIN_SIZE, max_size=MAX_SIZE,
                                                      shift=SHIFT), [SHIFT])
+    ctx.autotune(reps=3)  # the shapes the bench's timed steps use
     ctx.run(iters=1, use_graph=False)
     ms = ctx.copy_probe(COPY_PROBE_BYTES)
     print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES, "copy_probe_ms": ms}), flush=True)
