@@ -806,6 +806,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         off[i] = (int32_t)(512 * (t0 - 128 * n));
         rmin[i] = rv ? (int32_t)0x80000000 : 0x7fffffff;
     }
+    // Stage-1 cache per range: sqrt(A - n tau) (negative when A - n tau <= 0)
+    // and the threshold bits it was computed from.
+    float sa[4];
+    uint32_t thc[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        sa[i] = -1.0f;
+        thc[i] = 0xffffffffu; // no threshold has these bits (NaN)
+    }
 
     long a8[2][NK];
     v4i a16[2][NK];
@@ -954,23 +963,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // INT_MIN, or INT_MAX for a padding slot
         const int32_t pmin = p < T.P ? (int32_t)0x80000000 : 0x7fffffff;
 
+        // Per-range sqrt(A - n tau), recomputed only when one of the wave's
+        // thresholds moved since the last tile (a wave-uniform branch).
+        {
+            bool moved = false;
+#pragma unroll
+            for (int i = 0; i < 4; i++) moved |= thb[i] != thc[i];
+            if (__ballot(moved)) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    thc[i] = thb[i];
+                    const float thr = __uint_as_float(thb[i]);
+                    float amt;
+                    if constexpr (INT_LIMIT) {
+                        amt = Alo[i] - (float)n * thr;
+                    } else {
+                        const double nthr = nd * (double)thr;
+                        amt = nthr < A[i] ? (float)(A[i] - nthr) : 0.0f;
+                    }
+                    // sqrt within 1 ulp; (1 - 2e-5) keeps nt below
+                    // sqrt((A - n tau) Di) after the product's rounding
+                    // (sqDi is sqrt(Di) rounded down).
+                    // negative: A - n tau <= 0, no pruning for this range
+                    sa[i] = amt > 0.0f ? __builtin_amdgcn_sqrtf(amt) * (1.0f - 2e-5f) : -1.0f;
+                }
+            }
+        }
         // Stage-1 limit per (range, position) on v = 4*hi + lo.
         int32_t lim[4], toff[4];
         const int32_t o128 = S1p << 7;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             toff[i] = off[i] + o128;
-            const float thr = __uint_as_float(thb[i]);
-            float amt;
-            if constexpr (INT_LIMIT) {
-                amt = Alo[i] - (float)n * thr;
-            } else {
-                const double nthr = nd * (double)thr;
-                amt = nthr < A[i] ? (float)(A[i] - nthr) : 0.0f;
-            }
-            // sqrt within 1 ulp; (1 - 2e-5) keeps nt below sqrt((A - n tau) Di)
-            // after the product's rounding (sqDi is sqrt(Di) rounded down).
-            const float nt = __builtin_amdgcn_sqrtf(fmaxf(amt, 0.0f)) * sqDi * (1.0f - 2e-5f);
+            const float nt = fabsf(sa[i]) * sqDi; // <= sqrt((A - n tau) Di) when sa >= 0
             int32_t L;
             if constexpr (INT_LIMIT) {
                 // S1*t0 + nt < 2^31 for sizes <= 8, S1 and t0 < 2^24.
@@ -980,8 +1005,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                 const double X = ((double)S1p * (double)T0[i] + (double)nt) * inv_n;
                 L = (int32_t)floor(X) - 1 - toff[i];
             }
-            L = amt > 0.0f ? L : (int32_t)0x80000000;
-            lim[i] = max(L, max(rmin[i], pmin));
+            lim[i] = max(sa[i] < 0.0f ? (int32_t)0x80000000 : L, max(rmin[i], pmin));
         }
 
         // Epilogue of one row tile: one compare per value, issued 8 at a
