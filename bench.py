@@ -136,6 +136,84 @@ def _cpu_leg(img, budget_s: float = 12.0):
                        f"oracle/liboracle.so over {threads} threads")
 
 
+def _precision(sample: int) -> int:
+    """Dense-stride precision study: a 2048x2048 plane (seed 7), stride 1, the
+    fixed 8x8 and 16x16 grids (2.70e12 evals), searched by the split-INT8
+    MFMA kernel and by the plain FP32 kernel.  Reports, per size, the FP32
+    kernel's error against binary64 -- its own binary32 score of the winner
+    it picked, and that winner after the binary64 re-score, against the exact
+    winner -- and checks both against the oracle on a sample of ranges."""
+    from mars_amd import _native as N
+    from mars_amd import synth
+    from mars_amd.provider import EncodeParams, HipContext
+    from tests import _oracle as O
+
+    size_w = 2048
+    img = synth.plane(size_w, size_w, synth.WORKLOAD_SEEDS["dense2048"])
+    params = EncodeParams(min_size=8, max_size=16, shift=1)
+    runs = {}
+    for name, kern in (("mfma_i8", N.KERNEL_MFMA_I8), ("fp32", N.KERNEL_FP32)):
+        ctx = HipContext(0, kern)
+        t0 = time.perf_counter()
+        ctx.prepare(img, params, [1])
+        prep_s = time.perf_counter() - t0
+        t = ctx.run(iters=1, use_graph=True)
+        runs[name] = dict(ms_step=t.ms_step, evals=t.evals, prepare_s=prep_s,
+                          ms_by_size={str(2 << i): t.ms_by_size[i] for i in range(6) if t.ms_by_size[i] > 0},
+                          tables={(tt[0], tt[1]): tt for tt in ctx.tables()},
+                          scores=ctx.table_scores())
+        ctx.close()
+    codes = lambda c: None if c is None else (c.dom_row, c.dom_col, c.isometry, c.qalfa, c.qbeta)
+    per_size = {}
+    rng = np.random.default_rng(0)
+    threads = min(os.cpu_count() or 1, 16)
+    for key in sorted(runs["mfma_i8"]["tables"]):
+        size, shift, rows, cols, exact, P = runs["mfma_i8"]["tables"][key]
+        fp = runs["fp32"]["tables"][key][4]
+        fsc = runs["fp32"]["scores"][key]
+        msc = runs["mfma_i8"]["scores"][key]
+        code_diff = sum(1 for a, b in zip(exact, fp) if codes(a) != codes(b))
+        rms_diff = [abs(b.rms - a.rms) for a, b in zip(exact, fp)]
+        own = [abs(s_ - b.rms) for s_, b in zip(fsc, fp)]           # binary32 score vs binary64
+        own_rel = [o / b.rms for o, b in zip(own, fp) if b.rms > 0]
+        mf_own = max(abs(s_ - a.rms) for s_, a in zip(msc, exact))
+        pick = np.sort(rng.choice(len(rows), size=min(sample, len(rows)), replace=False))
+        t0 = time.perf_counter()
+        ref, _ = O.search_table(img, size, shift, [rows[i] for i in pick], [cols[i] for i in pick],
+                                threads=threads)
+        oracle_s = time.perf_counter() - t0
+        okey = lambda c: None if c is None else (c.dom_row, c.dom_col, c.isometry, c.qalfa, c.qbeta,
+                                                  c.rms, c.m.t1_x4)
+        gkey = lambda c: None if c is None else (c.dom_row, c.dom_col, c.isometry, c.qalfa, c.qbeta,
+                                                  c.rms, c.t1_x4)
+        per_size[str(size)] = {
+            "ranges": len(rows), "positions": P, "evals": len(rows) * P * 8,
+            "mfma_ms": runs["mfma_i8"]["ms_by_size"].get(str(size)),
+            "fp32_ms": runs["fp32"]["ms_by_size"].get(str(size)),
+            "fp32_ranges_with_other_codes": code_diff,
+            "fp32_ranges_with_other_rms": sum(1 for d in rms_diff if d != 0.0),
+            "fp32_rescored_rms_err_max": max(rms_diff), "fp32_rescored_rms_err_mean": float(np.mean(rms_diff)),
+            "fp32_own_score_err_max": max(own), "fp32_own_score_rel_err_max": max(own_rel),
+            "fp32_own_score_rel_err_mean": float(np.mean(own_rel)),
+            "mfma_own_score_err_max": mf_own,
+            "oracle_sample": len(pick), "oracle_s": oracle_s,
+            "mfma_vs_oracle_mismatches": sum(1 for j, i in enumerate(pick) if gkey(exact[i]) != okey(ref[j])),
+            "fp32_vs_oracle_code_mismatches": sum(1 for j, i in enumerate(pick)
+                                                  if codes(fp[i]) != (None if ref[j] is None else
+                                                                      (ref[j].dom_row, ref[j].dom_col, ref[j].isometry,
+                                                                       ref[j].qalfa, ref[j].qbeta))),
+        }
+    line = {"metric": "precision study: split-INT8 MFMA vs FP32 kernel, error vs binary64",
+            "config": {"workload": "dense2048: 2048x2048 u8 plane (seed 7), stride 1, fixed 8x8 and 16x16 grids"},
+            "mfma_ms_per_step": runs["mfma_i8"]["ms_step"], "fp32_ms_per_step": runs["fp32"]["ms_step"],
+            "evals_per_step": runs["mfma_i8"]["evals"],
+            "mfma_evals_per_s": runs["mfma_i8"]["evals"] / (runs["mfma_i8"]["ms_step"] * 1e-3),
+            "fp32_evals_per_s": runs["fp32"]["evals"] / (runs["fp32"]["ms_step"] * 1e-3),
+            "per_size": per_size}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def _traffic_probe() -> int:
     """Autotuned shapes, one eager step of the MFMA search and one 256 MiB copy
     kernel: the command rocprofv3 --pmc runs to read FETCH_SIZE / WRITE_SIZE
@@ -181,9 +259,15 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-image legs")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one eager MFMA step + a 256 MiB copy kernel (for rocprofv3 --pmc)")
+    ap.add_argument("--precision", action="store_true",
+                    help="the 2048x2048 dense-stride precision study instead of the benchmark")
+    ap.add_argument("--precision-sample", type=int, default=24,
+                    help="ranges per size checked against the oracle in --precision")
     args = ap.parse_args()
     if args.traffic_probe:
         return _traffic_probe()
+    if args.precision:
+        return _precision(args.precision_sample)
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share of the search; --gpus {args.gpus} is not "
               "supported (the search shards by range block with no exchange but a per-range "

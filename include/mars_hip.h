@@ -168,6 +168,14 @@ int mars_hip_table_read(const mars_hip_ctx *ctx, uint32_t i, uint32_t *rows, uin
                         mars_hip_candidate *out, uint8_t *found);
 
 /*
+ * The search kernel's own score of each winner of table i (n_ranges
+ * entries): the MFMA search's binary64 rms (equal to the table's rms), or
+ * the FP32 baseline's binary32 rms of the candidate it picked, before the
+ * binary64 re-score that fills the table.  For precision reports.
+ */
+int mars_hip_table_scores(mars_hip_ctx *ctx, uint32_t i, double *score);
+
+/*
  * Device self-test of the binary64 fit: evaluates the search's own fit
  * (encode.rs:247-289) on the GPU for n moment sets (6 int64 each, in
  * RawMoments order) and writes qalfa, qbeta, rms.

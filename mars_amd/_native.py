@@ -19,7 +19,8 @@ EXPORTS = (
     "mars_hip_open", "mars_hip_close", "mars_hip_prepare", "mars_hip_prepare_batch",
     "mars_hip_search", "mars_hip_search_plane", "mars_hip_last_error", "mars_hip_set_kernel",
     "mars_hip_run", "mars_hip_autotune", "mars_hip_table_count", "mars_hip_table_plane",
-    "mars_hip_table_info", "mars_hip_table_read", "mars_hip_fit_f64", "mars_hip_copy_probe",
+    "mars_hip_table_info", "mars_hip_table_read", "mars_hip_table_scores", "mars_hip_fit_f64",
+    "mars_hip_copy_probe",
 )
 
 
@@ -84,6 +85,7 @@ def lib() -> C.CDLL:
     L.mars_hip_table_info.argtypes = [C.c_void_p, u32, P(u32), P(u32), P(u32), P(u32)]
     L.mars_hip_table_read.argtypes = [C.c_void_p, u32, P(u32), P(u32), P(CCandidate),
                                       P(C.c_uint8)]
+    L.mars_hip_table_scores.argtypes = [C.c_void_p, u32, P(C.c_double)]
     L.mars_hip_fit_f64.argtypes = [C.c_void_p, P(C.c_int64), u32, C.c_double, u32, u32, P(u32),
                                    P(u32), P(C.c_double)]
     L.mars_hip_copy_probe.argtypes = [C.c_void_p, C.c_size_t, P(C.c_double)]

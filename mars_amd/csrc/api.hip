@@ -91,6 +91,7 @@ struct Table {
     Partial *d_partial = nullptr;
     mars_hip_candidate *d_out = nullptr;
     uint8_t *d_found = nullptr;
+    double *d_score = nullptr;
     std::vector<mars_hip_candidate> h_out;
     std::vector<uint8_t> h_found;
 };
@@ -360,6 +361,7 @@ int layout(mars_hip_ctx *ctx)
         d.partial = t.d_partial;
         d.out = t.d_out;
         d.found = t.d_found;
+        d.score = t.d_score;
         d.size = t.size;
         d.shift = t.shift;
         d.P = t.P;
@@ -733,6 +735,7 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         plan.add(&t.d_partial, sizeof(Partial) * (size_t)t.nr_pad * std::max(t.nch_cap[0], t.nch_cap[1]));
         plan.add(&t.d_out, sizeof(mars_hip_candidate) * (size_t)t.nr);
         plan.add(&t.d_found, (size_t)t.nr);
+        plan.add(&t.d_score, sizeof(double) * (size_t)t.nr);
     }
     HIP_TRY(ctx, hipMalloc(&ctx->arena, plan.total));
     ctx->arena_bytes = plan.total;
@@ -1049,6 +1052,17 @@ int mars_hip_table_read(const mars_hip_ctx *ctx, uint32_t i, uint32_t *rows, uin
     if (cols) memcpy(cols, t.cols.data(), 4 * t.nr);
     if (out) memcpy(out, t.h_out.data(), sizeof(mars_hip_candidate) * t.nr);
     if (found) memcpy(found, t.h_found.data(), t.nr);
+    return MARS_HIP_OK;
+}
+
+int mars_hip_table_scores(mars_hip_ctx *ctx, uint32_t i, double *score)
+{
+    if (!ctx || !score || i >= ctx->tables.size()) return fail(ctx, MARS_HIP_E_ARG, "bad argument");
+    if (!ctx->prepared) return fail(ctx, MARS_HIP_E_NOT_PREPARED, "prepare() first");
+    const Table &t = ctx->tables[i];
+    if (t.nr == 0) return MARS_HIP_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpy(score, t.d_score, sizeof(double) * t.nr, hipMemcpyDeviceToHost));
     return MARS_HIP_OK;
 }
 

@@ -60,6 +60,7 @@ struct TableDesc {
     Partial *partial;    // [nr_pad][nchunks]
     mars_hip_candidate *out;
     uint8_t *found;
+    double *score;       // [nr] the search kernel's own score of the winner (FP32: binary32 rms)
     uint32_t size, shift, P, P_pad, npc, nr, nr_pad, kmem;
     // Position chunking of the search: [0] MFMA kernel (32-position
     // tiles), [1] FP32 baseline (64-position tiles).  partial holds

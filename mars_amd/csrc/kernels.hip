@@ -130,6 +130,7 @@ struct TableG {
     GLB Partial *partial;
     GLB mars_hip_candidate *out;
     GLB uint8_t *found;
+    GLB double *score;
     uint32_t size, shift, P, P_pad, npc, nr, nr_pad, kmem;
     uint32_t nchunks[2], tiles_per_chunk[2];
 };
@@ -162,6 +163,7 @@ __device__ __forceinline__ TableG view(const TableDesc &t)
     g.partial = glb(t.partial);
     g.out = glb(t.out);
     g.found = glb(t.found);
+    g.score = glb(t.score);
     g.size = t.size;
     g.shift = t.shift;
     g.P = t.P;
@@ -1300,6 +1302,7 @@ __global__ void k_merge_exact(const TableDesc *__restrict__ tabs, const Work2 *_
                 t1 = e.t1;
             }
         }
+    T.score[r] = b;
     if (k == 0xffffffffu) {
         st_cand(T.out, r, mars_hip_candidate{});
         T.found[r] = 0;
@@ -1350,8 +1353,10 @@ __global__ void k_merge(const TableDesc *__restrict__ tabs, const Work2 *__restr
         s_key[0] = k;
         s_t1[0] = t1;
     }
+    if (t == 0) s_rms[0] = b;
     __syncthreads();
     k = s_key[0];
+    if (t == 0) T.score[r] = s_rms[0];
     if (k == 0xffffffffu) {
         if (t == 0) {
             st_cand(T.out, r, mars_hip_candidate{});

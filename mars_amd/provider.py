@@ -182,6 +182,24 @@ class HipContext:
                         [cands[j] if found[j] else None for j in range(nr.value)], P.value))
         return res
 
+    def table_scores(self, plane: int = 0):
+        """{(size, shift): the search kernel's own winner scores} for one plane."""
+        n = C.c_uint32()
+        self._check(self._L.mars_hip_table_count(self._ctx, C.byref(n)), "table_count")
+        res = {}
+        for i in range(n.value):
+            pl, s, sh, nr, P = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+            self._check(self._L.mars_hip_table_plane(self._ctx, i, C.byref(pl)), "table_plane")
+            if pl.value != plane:
+                continue
+            self._check(self._L.mars_hip_table_info(self._ctx, i, C.byref(s), C.byref(sh),
+                                                    C.byref(nr), C.byref(P)), "table_info")
+            out = np.zeros(max(nr.value, 1), np.float64)
+            self._check(self._L.mars_hip_table_scores(
+                self._ctx, i, out.ctypes.data_as(C.POINTER(C.c_double))), "table_scores")
+            res[(s.value, sh.value)] = out[:nr.value]
+        return res
+
     def fit_f64(self, moments: np.ndarray, max_alfa: float, bits_alfa: int, bits_beta: int):
         m = np.ascontiguousarray(moments, dtype=np.int64).reshape(-1, 6)
         n = m.shape[0]
