@@ -15,6 +15,18 @@
  *   mars_hip_encode_plane  encmars --method <hip> (encmars.rs:780-796): the
  *                          provider is the GPU search table of a context.
  *   mars_hip_encode_prepared  the same for one plane of a prepared batch.
+ *   mars_hip_encode_plane_method  encode_image_with_search with
+ *                          IndexedSearchProvider::build(.., method)
+ *                          (crates/mars-search/src/lib.rs:468-500; the
+ *                          reference's own tests do exactly this,
+ *                          crates/mars-search/tests/production_search.rs:79-90).
+ *   mars_ycbcr             mars_core::metrics::ycbcr (metrics.rs:377-417)
+ *   mars_downsample_box    color::downsample_box (color.rs:60-78)
+ *   mars_write_container   color.rs:778-790 write_container
+ *   mars_hip_encode_rgb    color::encode_color_image_with_residual_quantisation
+ *                          (color.rs:400-489, RGB arm, no colour regions)
+ *                          with each plane encoded through
+ *                          mars_hip_encode_plane_method.
  *
  * Status: 0 ok, < 0 error (MARS_HIP_E_* codes of mars_hip.h).
  */
@@ -90,6 +102,24 @@ int mars_wrap_gray(const uint8_t *plane, size_t len, uint8_t **out, size_t *out_
 
 void mars_free(void *p);
 
+/* Colour container modes (color.rs:369-375, MODE_*). */
+#define MARS_COLOR_GRAY 0
+#define MARS_COLOR_RGB_444 1
+#define MARS_COLOR_RGB_420 2
+
+/* metrics.rs:377-417: n interleaved RGB pixels -> BT.601 full-range Y, Cb, Cr
+ * (binary64, rounded half away from zero, clamped to 0..255). */
+int mars_ycbcr(const uint8_t *rgb, size_t n, uint8_t *y, uint8_t *cb, uint8_t *cr);
+
+/* color.rs:60-78: 2x2 box average with +2 rounding into ceil(w/2) x ceil(h/2);
+ * odd sides repeat the last row / column. */
+int mars_downsample_box(const uint8_t *px, uint32_t w, uint32_t h, uint8_t *out);
+
+/* color.rs:778-790: "MARC", version 0, mode, stream count, then each stream
+ * as a u32 little-endian length and its bytes.  *out is malloc'd. */
+int mars_write_container(uint8_t mode, const uint8_t *const *streams, const size_t *lens,
+                         uint32_t n_streams, uint8_t **out, size_t *out_len);
+
 /*
  * The encode of mars_encode_plane with the GPU search table of `ctx` as the
  * provider for both the warm-up and the final walk.  Prepares ctx for the
@@ -109,6 +139,34 @@ int mars_hip_encode_plane(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint
 int mars_hip_encode_prepared(mars_hip_ctx *ctx, uint32_t plane, const mars_encode_params *p,
                              const mars_encode_options *o, uint8_t **out, size_t *out_len,
                              mars_encode_stats *stats);
+
+/*
+ * mars_hip_encode_plane with the provider built for `method`
+ * (MARS_HIP_METHOD_*): the final walk is served by that method's table, the
+ * RD warm-up by the exhaustive one, as in the reference.
+ */
+int mars_hip_encode_plane_method(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t h,
+                                 const mars_encode_params *p, const mars_encode_options *o,
+                                 uint32_t method, uint8_t **out, size_t *out_len,
+                                 mars_encode_stats *stats);
+
+/* color.rs:338-345 ColorEncodeStats, with each plane's encode counters. */
+typedef struct {
+    uint64_t y_bytes, cb_bytes, cr_bytes;
+    mars_encode_stats y, cb, cr;
+} mars_color_stats;
+
+/*
+ * Encode a w x h interleaved RGB image: BT.601 YCbCr, chroma box-downsampled
+ * when subsampling is MARS_COLOR_RGB_420 (MARS_COLOR_RGB_444 keeps it), Y
+ * encoded with *y_params and Cb, Cr with *chroma_params, every plane with
+ * `method` as its provider, and the three streams wrapped in the MARC
+ * container of that mode.  *out is malloc'd.
+ */
+int mars_hip_encode_rgb(mars_hip_ctx *ctx, const uint8_t *rgb, uint32_t w, uint32_t h,
+                        uint32_t subsampling, const mars_encode_params *y_params,
+                        const mars_encode_params *chroma_params, const mars_encode_options *o,
+                        uint32_t method, uint8_t **out, size_t *out_len, mars_color_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,9 @@
  * library is what a thin FFI provider crate binds: `mars_hip_prepare` is the
  * factory body (it owns the plane's pool and runs the whole-plane search table
  * on the GPU), `mars_hip_search` is SearchProvider::search (a read-only,
- * thread-safe table lookup).  The binding is shown in INTEGRATION.md.
+ * thread-safe table lookup) of the prepared method -- the exhaustive search,
+ * or IndexedSearchProvider with the Fisher retriever
+ * (crates/mars-search/src/lib.rs:468-552).  The binding is shown in INTEGRATION.md.
  *
  * Conventions: status 0 = ok, < 0 = error (mars_hip_last_error() says why).
  * Nothing unwinds across the ABI.  No CPU search fallback exists: a caller
@@ -38,16 +40,24 @@ extern "C" {
 #define MARS_HIP_KERNEL_MFMA_I8 0 /* split-INT8 MFMA + exact certified epilogue (default) */
 #define MARS_HIP_KERNEL_FP32 1    /* plain FP32-FMA correlation + binary32 fit (baseline) */
 
+/* Candidate retrieval methods (crates/mars-search/src/lib.rs:357-410 MethodName). */
+#define MARS_HIP_METHOD_EXHAUSTIVE 0 /* every position x isometry (exhaustive.rs:15-33) */
+#define MARS_HIP_METHOD_FISHER 1     /* Fisher classified buckets (fisher.rs:29-78) */
+
 typedef struct mars_hip_ctx mars_hip_ctx;
 
-/* The EncodeParams fields the search reads (encode.rs:27-60). */
+/* The EncodeParams fields the search reads (encode.rs:27-60), plus the
+ * retrieval method IndexedSearchProvider::build takes (lib.rs:478-484).
+ * With MARS_HIP_METHOD_FISHER the context holds both the Fisher tables
+ * (the provider) and the exhaustive ones (the RD rate warm-up, which the
+ * reference keeps exhaustive for every method, encode.rs:1050, 1139-1169). */
 typedef struct {
     uint32_t min_size;
     uint32_t max_size;
     uint32_t shift;
     uint32_t bits_alfa;
     uint32_t bits_beta;
-    uint32_t _reserved;
+    uint32_t method; /* MARS_HIP_METHOD_* (0: exhaustive) */
     double max_alfa;
 } mars_hip_params;
 
@@ -73,6 +83,8 @@ typedef struct {
     uint64_t survivors; /* candidates past the integer bound (MFMA kernel) */
     uint64_t survivors_by_size[6];  /* the same per size 2..64 */
     uint64_t exact_fits_by_size[6]; /* candidates that took the verbatim binary64 fit */
+    double ms_fisher;      /* Fisher classification, buckets and bucket scans */
+    uint64_t fisher_evals; /* Fisher candidates fitted per step */
 } mars_hip_timing;
 
 /* One plane of a batch: w x h u8 pixels, row-major. */
@@ -134,6 +146,18 @@ int mars_hip_search_plane(const mars_hip_ctx *ctx, uint32_t plane, uint32_t row,
                           uint32_t size, uint32_t shift, uint32_t k, mars_hip_candidate *out,
                           uint32_t *n_out, uint64_t *evals);
 
+/*
+ * The winner of one retrieval method for (plane, row, col, size, shift):
+ * method MARS_HIP_METHOD_EXHAUSTIVE is ExhaustiveSearch::search
+ * (encode.rs:582-602), MARS_HIP_METHOD_FISHER is IndexedSearchProvider::search
+ * with the Fisher retriever (lib.rs:515-552: search_block_fitted over the
+ * range's bucket; *evals = the bucket's size, *n_out = 0 for an empty pool).
+ * Fisher needs a context prepared with that method.
+ */
+int mars_hip_search_method(const mars_hip_ctx *ctx, uint32_t plane, uint32_t method, uint32_t row,
+                           uint32_t col, uint32_t size, uint32_t shift, uint32_t k,
+                           mars_hip_candidate *out, uint32_t *n_out, uint64_t *evals);
+
 /* Last error message of this context ("" if none). */
 const char *mars_hip_last_error(const mars_hip_ctx *ctx);
 
@@ -166,6 +190,10 @@ int mars_hip_table_info(const mars_hip_ctx *ctx, uint32_t i, uint32_t *size, uin
 /* Copy table i's winners (n_ranges entries, found[j] = 0 for no candidate). */
 int mars_hip_table_read(const mars_hip_ctx *ctx, uint32_t i, uint32_t *rows, uint32_t *cols,
                         mars_hip_candidate *out, uint8_t *found);
+
+/* Table i's winners of one method and each range's evals (evals may be null). */
+int mars_hip_table_read_method(const mars_hip_ctx *ctx, uint32_t i, uint32_t method,
+                               mars_hip_candidate *out, uint8_t *found, uint64_t *evals);
 
 /*
  * The search kernel's own score of each winner of table i (n_ranges

@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libmars_hip.so"
 OK = 0
 E_ARG, E_DEVICE, E_NOT_PREPARED, E_NOT_IN_TABLE, E_UNSUPPORTED = -1, -2, -3, -4, -5
 KERNEL_MFMA_I8, KERNEL_FP32 = 0, 1
+METHOD_EXHAUSTIVE, METHOD_FISHER = 0, 1
 
 # Every symbol include/mars_hip.h declares.
 EXPORTS = (
@@ -20,13 +21,13 @@ EXPORTS = (
     "mars_hip_search", "mars_hip_search_plane", "mars_hip_last_error", "mars_hip_set_kernel",
     "mars_hip_run", "mars_hip_autotune", "mars_hip_table_count", "mars_hip_table_plane",
     "mars_hip_table_info", "mars_hip_table_read", "mars_hip_table_scores", "mars_hip_fit_f64",
-    "mars_hip_copy_probe",
+    "mars_hip_copy_probe", "mars_hip_search_method", "mars_hip_table_read_method",
 )
 
 
 class Params(C.Structure):
     _fields_ = [("min_size", C.c_uint32), ("max_size", C.c_uint32), ("shift", C.c_uint32),
-                ("bits_alfa", C.c_uint32), ("bits_beta", C.c_uint32), ("_reserved", C.c_uint32),
+                ("bits_alfa", C.c_uint32), ("bits_beta", C.c_uint32), ("method", C.c_uint32),
                 ("max_alfa", C.c_double)]
 
 
@@ -42,7 +43,8 @@ class Timing(C.Structure):
     _fields_ = [("ms_step", C.c_double), ("ms_pool", C.c_double), ("ms_search", C.c_double),
                 ("ms_by_size", C.c_double * 6), ("evals", C.c_uint64),
                 ("survivors", C.c_uint64), ("survivors_by_size", C.c_uint64 * 6),
-                ("exact_fits_by_size", C.c_uint64 * 6)]
+                ("exact_fits_by_size", C.c_uint64 * 6), ("ms_fisher", C.c_double),
+                ("fisher_evals", C.c_uint64)]
 
 
 class Plane(C.Structure):
@@ -89,6 +91,10 @@ def lib() -> C.CDLL:
     L.mars_hip_fit_f64.argtypes = [C.c_void_p, P(C.c_int64), u32, C.c_double, u32, u32, P(u32),
                                    P(u32), P(C.c_double)]
     L.mars_hip_copy_probe.argtypes = [C.c_void_p, C.c_size_t, P(C.c_double)]
+    L.mars_hip_search_method.argtypes = [C.c_void_p, u32, u32, u32, u32, u32, u32, u32,
+                                         P(CCandidate), P(u32), P(u64)]
+    L.mars_hip_table_read_method.argtypes = [C.c_void_p, u32, u32, P(CCandidate), P(C.c_uint8),
+                                             P(u64)]
     for name in EXPORTS:
         if name not in ("mars_hip_close", "mars_hip_last_error"):
             getattr(L, name).restype = C.c_int

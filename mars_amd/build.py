@@ -52,7 +52,7 @@ CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math"
 def build_hip() -> Path:
     out = PKG / "libmars_hip.so"
     objs = []
-    for src in ("kernels.hip", "api.hip"):
+    for src in ("kernels.hip", "fisher.hip", "api.hip"):
         obj = CSRC / (Path(src).stem + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)

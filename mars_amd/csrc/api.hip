@@ -94,6 +94,15 @@ struct Table {
     double *d_score = nullptr;
     std::vector<mars_hip_candidate> h_out;
     std::vector<uint8_t> h_found;
+    // Fisher classified search (method MARS_HIP_METHOD_FISHER only)
+    bool fisher = false;
+    uint16_t *d_fcls = nullptr, *d_frcls = nullptr;
+    uint32_t *d_fbuck = nullptr, *d_flist = nullptr, *d_fevals = nullptr;
+    mars_hip_candidate *d_fout = nullptr;
+    uint8_t *d_ffound = nullptr;
+    std::vector<mars_hip_candidate> h_fout;
+    std::vector<uint8_t> h_ffound;
+    std::vector<uint32_t> h_fevals;
 };
 
 // Slice of a work list in the device work buffer.
@@ -118,6 +127,7 @@ struct mars_hip_ctx {
     mutable std::mutex err_mu;
     std::string err;
     int kernel = MARS_HIP_KERNEL_MFMA_I8;
+    uint32_t method = MARS_HIP_METHOD_EXHAUSTIVE; // of the prepared batch
     bool prepared = false;
     mars_hip_params params{};
     std::vector<Plane> planes;
@@ -133,6 +143,7 @@ struct mars_hip_ctx {
     Work3 *d_work3 = nullptr;
     size_t work2_cap = 0, work3_cap = 0;
     Slice wl_contract, wl_sat_rows, wl_sat_cols, wl_possums, wl_ranges, wl_rstats, wl_seed;
+    Slice wl_fdom, wl_frng, wl_ftab; // Fisher: positions, ranges, tables
     unsigned long long *d_counters = nullptr;
     double *d_yscale = nullptr;
     Common common{};
@@ -374,6 +385,13 @@ int layout(mars_hip_ctx *ctx)
             d.nchunks[k] = t.nchunks[k];
             d.tiles_per_chunk[k] = t.tpc[k];
         }
+        d.fcls = t.d_fcls;
+        d.fbuck = t.d_fbuck;
+        d.flist = t.d_flist;
+        d.frcls = t.d_frcls;
+        d.fout = t.d_fout;
+        d.ffound = t.d_ffound;
+        d.fevals = t.d_fevals;
     }
     // work lists
     std::vector<Work2> w2;
@@ -421,6 +439,27 @@ int layout(mars_hip_ctx *ctx)
         for (uint32_t r = 0; r < t.nr; r++) w2.push_back(make_uint2(i, r));
     }
     end2(ctx->wl_seed);
+    // Fisher (tables with an empty pool keep their zeroed results)
+    begin2(ctx->wl_fdom);
+    for (uint32_t i = 0; i < ctx->tables.size(); i++) {
+        const Table &t = ctx->tables[i];
+        if (!t.fisher || t.P == 0) continue;
+        for (uint32_t b = 0; b < cdiv(t.P, 256); b++) w2.push_back(make_uint2(i, b));
+    }
+    end2(ctx->wl_fdom);
+    begin2(ctx->wl_frng);
+    for (uint32_t i = 0; i < ctx->tables.size(); i++) {
+        const Table &t = ctx->tables[i];
+        if (!t.fisher || t.P == 0) continue;
+        for (uint32_t b = 0; b < cdiv(t.nr, 4); b++) w2.push_back(make_uint2(i, b));
+    }
+    end2(ctx->wl_frng);
+    begin2(ctx->wl_ftab);
+    for (uint32_t i = 0; i < ctx->tables.size(); i++) {
+        const Table &t = ctx->tables[i];
+        if (t.fisher && t.P > 0) w2.push_back(make_uint2(i, 0));
+    }
+    end2(ctx->wl_ftab);
     for (Group &g : ctx->groups) {
         for (int k = 0; k < 2; k++) {
             g.search[k].off = (uint32_t)w3.size();
@@ -511,11 +550,20 @@ void enqueue_merge(mars_hip_ctx *ctx, const Group &g, int kernel, hipStream_t st
                        ctx->common, st);
 }
 
-// Enqueue one whole step.  With ev (3 + n_groups events: before the pool,
-// after the preparation, after each group's search, after the merges) the
-// step runs serially on the context stream so each phase can be timed;
-// without, each group's search and merge run on a side stream forked after
-// the preparation and joined at the end.
+// The Fisher search of every table (after the preparation).
+void enqueue_fisher(mars_hip_ctx *ctx, hipStream_t st)
+{
+    const Work2 *w2 = ctx->d_work2;
+    mars::launch_fisher(ctx->d_tabs, w2 + ctx->wl_fdom.off, ctx->wl_fdom.n, w2 + ctx->wl_frng.off,
+                        ctx->wl_frng.n, w2 + ctx->wl_ftab.off, ctx->wl_ftab.n, ctx->common, st);
+}
+
+// Enqueue one whole step.  With ev (4 + n_groups events: before the pool,
+// after the preparation, after each group's search, after the merges, after
+// the Fisher search) the step runs serially on the context stream so each
+// phase can be timed; without, each group's search and merge run on a side
+// stream forked after the preparation and joined at the end, the Fisher
+// search on the context stream meanwhile.
 int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
 {
     hipStream_t st = ctx->stream;
@@ -530,6 +578,7 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
             RC_TRY(enqueue_search(ctx, ctx->groups[i], kernel, ss));
             enqueue_merge(ctx, ctx->groups[i], kernel, ss);
         }
+        enqueue_fisher(ctx, st);
         for (int j = 0; j < nside; j++) {
             HIP_TRY(ctx, hipEventRecord(ctx->join_ev[j], ctx->side[j]));
             HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join_ev[j], 0));
@@ -544,6 +593,8 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
     }
     for (const Group &g : ctx->groups) enqueue_merge(ctx, g, kernel, st);
     HIP_TRY(ctx, hipEventRecord(ev[2 + ctx->groups.size()], st));
+    enqueue_fisher(ctx, st);
+    HIP_TRY(ctx, hipEventRecord(ev[3 + ctx->groups.size()], st));
     HIP_TRY(ctx, hipGetLastError());
     return MARS_HIP_OK;
 }
@@ -558,6 +609,16 @@ int download(mars_hip_ctx *ctx)
                                     hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(t.h_found.data(), t.d_found, t.nr, hipMemcpyDeviceToHost,
                                     ctx->stream));
+        if (!t.fisher) continue;
+        t.h_fout.resize(t.nr);
+        t.h_ffound.resize(t.nr);
+        t.h_fevals.resize(t.nr);
+        HIP_TRY(ctx, hipMemcpyAsync(t.h_fout.data(), t.d_fout, sizeof(mars_hip_candidate) * t.nr,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(t.h_ffound.data(), t.d_ffound, t.nr, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(t.h_fevals.data(), t.d_fevals, 4 * (size_t)t.nr,
+                                    hipMemcpyDeviceToHost, ctx->stream));
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return MARS_HIP_OK;
@@ -569,7 +630,7 @@ uint64_t table_evals(const Table &t) { return (uint64_t)t.P * 8; }
 int timed_step(mars_hip_ctx *ctx, mars_hip_timing *tm, std::vector<double> *group_ms)
 {
     const size_t ng = ctx->groups.size();
-    std::vector<hipEvent_t> ev(3 + ng);
+    std::vector<hipEvent_t> ev(4 + ng);
     for (auto &e : ev) HIP_TRY(ctx, hipEventCreate(&e));
     int rc = enqueue_step(ctx, ctx->kernel, ev.data());
     if (rc == 0) {
@@ -602,6 +663,10 @@ int timed_step(mars_hip_ctx *ctx, mars_hip_timing *tm, std::vector<double> *grou
             tm->survivors += cnt[i + 1];
         }
         for (const Table &t : ctx->tables) tm->evals += (uint64_t)t.nr * table_evals(t);
+        (void)hipEventElapsedTime(&ms, ev[2 + ng], ev[3 + ng]);
+        tm->ms_fisher = ms;
+        for (const Table &t : ctx->tables)
+            for (uint32_t e : t.h_fevals) tm->fisher_evals += e;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     return rc;
@@ -622,12 +687,15 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
             return fail(ctx, MARS_HIP_E_ARG, "bad plane size");
     }
     if (!valid_params(p)) return fail(ctx, MARS_HIP_E_ARG, "bad params");
+    if (p->method != MARS_HIP_METHOD_EXHAUSTIVE && p->method != MARS_HIP_METHOD_FISHER)
+        return fail(ctx, MARS_HIP_E_ARG, "unknown retrieval method");
     for (uint32_t i = 0; i < n_shifts; i++)
         if (shifts[i] == 0) return fail(ctx, MARS_HIP_E_ARG, "shift must be > 0");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     free_batch(ctx);
     ctx->params = *p;
+    ctx->method = p->method;
 
     std::vector<uint32_t> sh(shifts, shifts + n_shifts);
     std::sort(sh.begin(), sh.end());
@@ -671,6 +739,7 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
                 t.P = t.npr * t.npc;
                 if (t.P > (0xffffffffu / 8)) return fail(ctx, MARS_HIP_E_UNSUPPORTED, "too many positions");
                 t.P_pad = (t.P + 31) / 32 * 32;
+                t.fisher = p->method == MARS_HIP_METHOD_FISHER;
                 search_order(t);
                 ctx->table_of[std::make_tuple(pi, t.size, shift)] = (uint32_t)ctx->tables.size();
                 ctx->tables.push_back(std::move(t));
@@ -736,6 +805,15 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         plan.add(&t.d_out, sizeof(mars_hip_candidate) * (size_t)t.nr);
         plan.add(&t.d_found, (size_t)t.nr);
         plan.add(&t.d_score, sizeof(double) * (size_t)t.nr);
+        if (t.fisher) {
+            plan.add(&t.d_fcls, sizeof(uint16_t) * (size_t)t.P);
+            plan.add(&t.d_fbuck, sizeof(uint32_t) * 2 * 72);
+            plan.add(&t.d_flist, sizeof(uint32_t) * (size_t)t.P);
+            plan.add(&t.d_frcls, sizeof(uint16_t) * (size_t)t.nr);
+            plan.add(&t.d_fout, sizeof(mars_hip_candidate) * (size_t)t.nr);
+            plan.add(&t.d_ffound, (size_t)t.nr);
+            plan.add(&t.d_fevals, sizeof(uint32_t) * (size_t)t.nr);
+        }
     }
     HIP_TRY(ctx, hipMalloc(&ctx->arena, plan.total));
     ctx->arena_bytes = plan.total;
@@ -789,22 +867,27 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
     return MARS_HIP_OK;
 }
 
-int lookup(const mars_hip_ctx *ctx, uint32_t plane, uint32_t row, uint32_t col, uint32_t size,
-           uint32_t shift, uint32_t k, mars_hip_candidate *out, uint32_t *n_out, uint64_t *evals)
+int lookup(const mars_hip_ctx *ctx, uint32_t plane, uint32_t method, uint32_t row, uint32_t col,
+           uint32_t size, uint32_t shift, uint32_t k, mars_hip_candidate *out, uint32_t *n_out,
+           uint64_t *evals)
 {
     mars_hip_ctx *mctx = const_cast<mars_hip_ctx *>(ctx);
     if (!ctx || !out || !n_out || !evals) return fail(mctx, MARS_HIP_E_ARG, "null argument");
     if (k != 1) return fail(mctx, MARS_HIP_E_UNSUPPORTED, "only k == 1 is supported");
     if (!ctx->prepared) return fail(mctx, MARS_HIP_E_NOT_PREPARED, "prepare() first");
+    if (method != MARS_HIP_METHOD_EXHAUSTIVE &&
+        !(method == MARS_HIP_METHOD_FISHER && ctx->method == MARS_HIP_METHOD_FISHER))
+        return fail(mctx, MARS_HIP_E_ARG, "method not prepared");
     auto ti = ctx->table_of.find(std::make_tuple(plane, size, shift));
     if (ti != ctx->table_of.end()) {
         const Table &t = ctx->tables[ti->second];
         auto it = t.index.find(((uint64_t)row << 32) | col);
         if (it != t.index.end()) {
             const uint32_t i = it->second;
-            *evals = table_evals(t);
-            if (t.h_found[i]) {
-                *out = t.h_out[i];
+            const bool fisher = method == MARS_HIP_METHOD_FISHER;
+            *evals = fisher ? t.h_fevals[i] : table_evals(t);
+            if (fisher ? t.h_ffound[i] : t.h_found[i]) {
+                *out = fisher ? t.h_fout[i] : t.h_out[i];
                 *n_out = 1;
             } else {
                 *n_out = 0;
@@ -821,13 +904,14 @@ int lookup(const mars_hip_ctx *ctx, uint32_t plane, uint32_t row, uint32_t col, 
 struct PlaneRef {
     const mars_hip_ctx *ctx;
     uint32_t plane;
+    uint32_t method;
 };
 
 int plane_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
                     mars_hip_candidate *out, uint32_t *n_out, uint64_t *evals)
 {
     const PlaneRef *r = (const PlaneRef *)user;
-    return lookup(r->ctx, r->plane, row, col, size, shift, 1, out, n_out, evals);
+    return lookup(r->ctx, r->plane, r->method, row, col, size, shift, 1, out, n_out, evals);
 }
 
 } // namespace
@@ -1006,14 +1090,21 @@ int mars_hip_search(const mars_hip_ctx *ctx, uint32_t row, uint32_t col, uint32_
                     uint32_t shift, uint32_t k, mars_hip_candidate *out, uint32_t *n_out,
                     uint64_t *evals)
 {
-    return lookup(ctx, 0, row, col, size, shift, k, out, n_out, evals);
+    return lookup(ctx, 0, ctx ? ctx->method : 0, row, col, size, shift, k, out, n_out, evals);
 }
 
 int mars_hip_search_plane(const mars_hip_ctx *ctx, uint32_t plane, uint32_t row, uint32_t col,
                           uint32_t size, uint32_t shift, uint32_t k, mars_hip_candidate *out,
                           uint32_t *n_out, uint64_t *evals)
 {
-    return lookup(ctx, plane, row, col, size, shift, k, out, n_out, evals);
+    return lookup(ctx, plane, ctx ? ctx->method : 0, row, col, size, shift, k, out, n_out, evals);
+}
+
+int mars_hip_search_method(const mars_hip_ctx *ctx, uint32_t plane, uint32_t method, uint32_t row,
+                           uint32_t col, uint32_t size, uint32_t shift, uint32_t k,
+                           mars_hip_candidate *out, uint32_t *n_out, uint64_t *evals)
+{
+    return lookup(ctx, plane, method, row, col, size, shift, k, out, n_out, evals);
 }
 
 int mars_hip_table_count(const mars_hip_ctx *ctx, uint32_t *n_tables)
@@ -1052,6 +1143,27 @@ int mars_hip_table_read(const mars_hip_ctx *ctx, uint32_t i, uint32_t *rows, uin
     if (cols) memcpy(cols, t.cols.data(), 4 * t.nr);
     if (out) memcpy(out, t.h_out.data(), sizeof(mars_hip_candidate) * t.nr);
     if (found) memcpy(found, t.h_found.data(), t.nr);
+    return MARS_HIP_OK;
+}
+
+int mars_hip_table_read_method(const mars_hip_ctx *ctx, uint32_t i, uint32_t method,
+                               mars_hip_candidate *out, uint8_t *found, uint64_t *evals)
+{
+    if (!ctx || i >= ctx->tables.size()) return MARS_HIP_E_ARG;
+    if (!ctx->prepared) return MARS_HIP_E_NOT_PREPARED;
+    const Table &t = ctx->tables[i];
+    if (method == MARS_HIP_METHOD_EXHAUSTIVE) {
+        if (out) memcpy(out, t.h_out.data(), sizeof(mars_hip_candidate) * t.nr);
+        if (found) memcpy(found, t.h_found.data(), t.nr);
+        if (evals)
+            for (uint32_t r = 0; r < t.nr; r++) evals[r] = table_evals(t);
+        return MARS_HIP_OK;
+    }
+    if (method != MARS_HIP_METHOD_FISHER || !t.fisher) return MARS_HIP_E_ARG;
+    if (out) memcpy(out, t.h_fout.data(), sizeof(mars_hip_candidate) * t.nr);
+    if (found) memcpy(found, t.h_ffound.data(), t.nr);
+    if (evals)
+        for (uint32_t r = 0; r < t.nr; r++) evals[r] = t.h_fevals[r];
     return MARS_HIP_OK;
 }
 
@@ -1119,22 +1231,49 @@ int mars_hip_copy_probe(mars_hip_ctx *ctx, size_t bytes, double *ms)
     return MARS_HIP_OK;
 }
 
-int mars_hip_encode_plane(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t h,
-                          const mars_encode_params *p, const mars_encode_options *o,
-                          uint8_t **out, size_t *out_len, mars_encode_stats *stats)
+namespace {
+
+mars_hip_params search_params(const mars_encode_params *p, uint32_t method)
 {
-    if (!ctx || !p || !o) return fail(ctx, MARS_HIP_E_ARG, "null argument");
     mars_hip_params hp{};
     hp.min_size = p->min_size;
     hp.max_size = p->max_size;
     hp.shift = p->shift;
     hp.bits_alfa = p->bits_alfa;
     hp.bits_beta = p->bits_beta;
+    hp.method = method;
     hp.max_alfa = p->max_alfa;
-    // The walks request stride `shift`, and 2*shift for flat blocks when
-    // adaptive density is on (encode.rs:432-437, RD path only).
-    uint32_t shifts[2] = {p->shift, 2 * p->shift};
-    const uint32_t ns = (o->adaptive_density && p->has_lambda) ? 2 : 1;
+    return hp;
+}
+
+// The walks request stride `shift`, and 2*shift for flat blocks when
+// adaptive density is on (encode.rs:432-437, RD path only).
+uint32_t walk_shifts(const mars_encode_params *p, const mars_encode_options *o, uint32_t *shifts)
+{
+    shifts[0] = p->shift;
+    shifts[1] = 2 * p->shift;
+    return (o->adaptive_density && p->has_lambda) ? 2 : 1;
+}
+
+} // namespace
+
+int mars_hip_encode_plane(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t h,
+                          const mars_encode_params *p, const mars_encode_options *o,
+                          uint8_t **out, size_t *out_len, mars_encode_stats *stats)
+{
+    return mars_hip_encode_plane_method(ctx, px, w, h, p, o, MARS_HIP_METHOD_EXHAUSTIVE, out,
+                                        out_len, stats);
+}
+
+int mars_hip_encode_plane_method(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t h,
+                                 const mars_encode_params *p, const mars_encode_options *o,
+                                 uint32_t method, uint8_t **out, size_t *out_len,
+                                 mars_encode_stats *stats)
+{
+    if (!ctx || !p || !o) return fail(ctx, MARS_HIP_E_ARG, "null argument");
+    const mars_hip_params hp = search_params(p, method);
+    uint32_t shifts[2];
+    const uint32_t ns = walk_shifts(p, o, shifts);
     RC_TRY(mars_hip_prepare(ctx, px, w, h, &hp, shifts, ns));
     return mars_hip_encode_prepared(ctx, 0, p, o, out, out_len, stats);
 }
@@ -1151,14 +1290,74 @@ int mars_hip_encode_prepared(mars_hip_ctx *ctx, uint32_t plane, const mars_encod
         p->bits_beta != hp.bits_beta || p->max_alfa != hp.max_alfa)
         return fail(ctx, MARS_HIP_E_ARG, "encode params differ from the prepared search params");
     const Plane &pl = ctx->planes[plane];
-    PlaneRef ref{ctx, plane};
-    int rc = mars_encode_plane(pl.host.data(), pl.w, pl.h, p, o, plane_search_cb, &ref,
-                               plane_search_cb, &ref, out, out_len, stats);
+    // The final walk asks the prepared method's provider; the RD warm-up is
+    // exhaustive whatever the method (encode.rs:1050, 1139-1169).
+    PlaneRef search{ctx, plane, ctx->method};
+    PlaneRef warmup{ctx, plane, MARS_HIP_METHOD_EXHAUSTIVE};
+    int rc = mars_encode_plane(pl.host.data(), pl.w, pl.h, p, o, plane_search_cb, &search,
+                               plane_search_cb, &warmup, out, out_len, stats);
     if (rc) {
         std::lock_guard<std::mutex> lk(ctx->err_mu);
         if (ctx->err.empty()) ctx->err = "encode failed";
     }
     return rc;
+}
+
+int mars_hip_encode_rgb(mars_hip_ctx *ctx, const uint8_t *rgb, uint32_t w, uint32_t h,
+                        uint32_t subsampling, const mars_encode_params *y_params,
+                        const mars_encode_params *chroma_params, const mars_encode_options *o,
+                        uint32_t method, uint8_t **out, size_t *out_len, mars_color_stats *stats)
+{
+    if (!ctx || !rgb || !y_params || !chroma_params || !o || !out || !out_len)
+        return fail(ctx, MARS_HIP_E_ARG, "null argument");
+    if (w == 0 || h == 0) return fail(ctx, MARS_HIP_E_ARG, "empty image");
+    if (subsampling != MARS_COLOR_RGB_444 && subsampling != MARS_COLOR_RGB_420)
+        return fail(ctx, MARS_HIP_E_ARG, "subsampling must be 4:4:4 or 4:2:0");
+    *out = nullptr;
+    *out_len = 0;
+    // color.rs:429-463: YCbCr, then chroma subsampled for 4:2:0.
+    const size_t n = (size_t)w * h;
+    std::vector<uint8_t> y(n), cb(n), cr(n);
+    RC_TRY(mars_ycbcr(rgb, n, y.data(), cb.data(), cr.data()));
+    uint32_t cw = w, ch = h;
+    if (subsampling == MARS_COLOR_RGB_420) {
+        cw = (w + 1) / 2;
+        ch = (h + 1) / 2;
+        std::vector<uint8_t> a((size_t)cw * ch), b((size_t)cw * ch);
+        RC_TRY(mars_downsample_box(cb.data(), w, h, a.data()));
+        RC_TRY(mars_downsample_box(cr.data(), w, h, b.data()));
+        cb.swap(a);
+        cr.swap(b);
+    }
+    // color.rs:465-476: Y with the luma params, Cb and Cr with the chroma
+    // params (one batch: same params and strides).
+    mars_color_stats st{};
+    uint8_t *ys = nullptr, *cbs = nullptr, *crs = nullptr;
+    size_t yl = 0, cbl = 0, crl = 0;
+    int rc = mars_hip_encode_plane_method(ctx, y.data(), w, h, y_params, o, method, &ys, &yl, &st.y);
+    if (rc == 0) {
+        const mars_hip_params hp = search_params(chroma_params, method);
+        uint32_t shifts[2];
+        const uint32_t ns = walk_shifts(chroma_params, o, shifts);
+        const mars_hip_plane planes[2] = {{cb.data(), cw, ch}, {cr.data(), cw, ch}};
+        rc = mars_hip_prepare_batch(ctx, planes, 2, &hp, shifts, ns);
+    }
+    if (rc == 0) rc = mars_hip_encode_prepared(ctx, 0, chroma_params, o, &cbs, &cbl, &st.cb);
+    if (rc == 0) rc = mars_hip_encode_prepared(ctx, 1, chroma_params, o, &crs, &crl, &st.cr);
+    if (rc == 0) {
+        const uint8_t *streams[3] = {ys, cbs, crs};
+        const size_t lens[3] = {yl, cbl, crl};
+        rc = mars_write_container((uint8_t)subsampling, streams, lens, 3, out, out_len);
+    }
+    mars_free(ys);
+    mars_free(cbs);
+    mars_free(crs);
+    if (rc) return rc;
+    st.y_bytes = yl;
+    st.cb_bytes = cbl;
+    st.cr_bytes = crl;
+    if (stats) *stats = st;
+    return MARS_HIP_OK;
 }
 
 } // extern "C"

@@ -960,11 +960,18 @@ int mars_encode_plane(const uint8_t *px, uint32_t w, uint32_t h, const mars_enco
     return MARS_HIP_OK;
 }
 
-int mars_wrap_gray(const uint8_t *plane, size_t len, uint8_t **out, size_t *out_len)
+int mars_write_container(uint8_t mode, const uint8_t *const *streams, const size_t *lens,
+                         uint32_t n_streams, uint8_t **out, size_t *out_len)
 {
-    if ((!plane && len) || !out || !out_len || len > 0xffffffffu) return MARS_HIP_E_ARG;
-    // color.rs:778-790: "MARC", version 0, mode 0 (grey), 1 stream, u32 length.
-    const size_t total = 7 + 4 + len;
+    if (!out || !out_len || n_streams > 255 || (n_streams && (!streams || !lens)))
+        return MARS_HIP_E_ARG;
+    // color.rs:778-790: "MARC", version 0, mode, section count, then per
+    // stream a u32 little-endian length and the bytes.
+    size_t total = 7;
+    for (uint32_t i = 0; i < n_streams; i++) {
+        if (lens[i] > 0xffffffffu || (lens[i] && !streams[i])) return MARS_HIP_E_ARG;
+        total += 4 + lens[i];
+    }
     uint8_t *buf = (uint8_t *)malloc(total);
     if (!buf) return MARS_HIP_E_ARG;
     buf[0] = 'M';
@@ -972,12 +979,61 @@ int mars_wrap_gray(const uint8_t *plane, size_t len, uint8_t **out, size_t *out_
     buf[2] = 'R';
     buf[3] = 'C';
     buf[4] = 0;
-    buf[5] = 0;
-    buf[6] = 1;
-    for (int i = 0; i < 4; i++) buf[7 + i] = (uint8_t)(len >> (8 * i));
-    if (len) memcpy(buf + 11, plane, len);
+    buf[5] = mode;
+    buf[6] = (uint8_t)n_streams;
+    size_t off = 7;
+    for (uint32_t i = 0; i < n_streams; i++) {
+        for (int k = 0; k < 4; k++) buf[off + k] = (uint8_t)(lens[i] >> (8 * k));
+        off += 4;
+        if (lens[i]) memcpy(buf + off, streams[i], lens[i]);
+        off += lens[i];
+    }
     *out = buf;
     *out_len = total;
+    return MARS_HIP_OK;
+}
+
+int mars_wrap_gray(const uint8_t *plane, size_t len, uint8_t **out, size_t *out_len)
+{
+    if (!plane && len) return MARS_HIP_E_ARG;
+    // color.rs:382-384: mode 0 (grey), one stream.
+    return mars_write_container(0, &plane, &len, 1, out, out_len);
+}
+
+int mars_ycbcr(const uint8_t *rgb, size_t n, uint8_t *y, uint8_t *cb, uint8_t *cr)
+{
+    if (n && (!rgb || !y || !cb || !cr)) return MARS_HIP_E_ARG;
+    // metrics.rs:394-410: binary64, evaluated left to right without FMA
+    // contraction, round() half away from zero, clamp to 0..255.
+    auto u8 = [](double x) {
+        x = std::round(x);
+        x = x < 0.0 ? 0.0 : x > 255.0 ? 255.0 : x;
+        return (uint8_t)x;
+    };
+    for (size_t i = 0; i < n; i++) {
+        const double r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        y[i] = u8(0.299 * r + 0.587 * g + 0.114 * b);
+        cb[i] = u8(128.0 - 0.168736 * r - 0.331264 * g + 0.5 * b);
+        cr[i] = u8(128.0 + 0.5 * r - 0.418688 * g - 0.081312 * b);
+    }
+    return MARS_HIP_OK;
+}
+
+int mars_downsample_box(const uint8_t *px, uint32_t w, uint32_t h, uint8_t *out)
+{
+    if (!px || !out || w == 0 || h == 0) return MARS_HIP_E_ARG;
+    // color.rs:60-78: (sum of the 2x2 neighbourhood + 2) / 4, the last
+    // column / row standing in for the missing one on odd sides.
+    const uint32_t ow = (w + 1) / 2, oh = (h + 1) / 2;
+    for (uint32_t oy = 0; oy < oh; oy++) {
+        const size_t y0 = 2 * oy, y1 = std::min<size_t>(y0 + 1, h - 1);
+        for (uint32_t ox = 0; ox < ow; ox++) {
+            const size_t x0 = 2 * ox, x1 = std::min<size_t>(x0 + 1, w - 1);
+            const uint32_t sum = (uint32_t)px[y0 * w + x0] + px[y0 * w + x1] + px[y1 * w + x0] +
+                                 px[y1 * w + x1];
+            out[(size_t)oy * ow + ox] = (uint8_t)((sum + 2) / 4);
+        }
+    }
     return MARS_HIP_OK;
 }
 

@@ -66,6 +66,14 @@ struct TableDesc {
     // tiles), [1] FP32 baseline (64-position tiles).  partial holds
     // nr_pad x nchunks[kernel] entries.
     uint32_t nchunks[2], tiles_per_chunk[2];
+    // Fisher classified search (all null when the table has none).
+    uint16_t *fcls;          // [P] bucket * 8 + isometry of each domain position
+    uint32_t *fbuck;         // [2][72] start, count of each bucket's candidate list
+    uint32_t *flist;         // [P] p * 8 + isometry, grouped by bucket
+    uint16_t *frcls;         // [nr] bucket * 8 + isometry of each range
+    mars_hip_candidate *fout; // [nr] Fisher winners
+    uint8_t *ffound;          // [nr]
+    uint32_t *fevals;         // [nr] candidates fitted (the bucket's size)
 };
 
 // Step-wide constants.
@@ -100,6 +108,13 @@ int launch_search(int kernel, uint32_t size, const TableDesc *t, const Work3 *wo
                   uint32_t nwork, const Common &c, hipStream_t st);
 void launch_merge(const TableDesc *t, const Work2 *work, uint32_t nwork, bool t1_exact,
                   const Common &c, hipStream_t st);
+// Fisher classified search of the tables in `work` (x = table, y = 0):
+// classification of every domain position and range, the buckets, then
+// every range's bucket scan.  Needs the pool, k_possums, k_ranges and
+// k_rstats of those tables.
+void launch_fisher(const TableDesc *t, const Work2 *dom_work, uint32_t n_dom,
+                   const Work2 *rng_work, uint32_t n_rng, const Work2 *tab_work, uint32_t n_tab,
+                   const Common &c, hipStream_t st);
 void launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t st);
 void launch_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
                          uint32_t bits_beta, uint32_t *qa, uint32_t *qb, double *rms,

@@ -111,20 +111,21 @@ class HipContext:
     def set_kernel(self, kernel: int) -> None:
         self._check(self._L.mars_hip_set_kernel(self._ctx, kernel), "mars_hip_set_kernel")
 
-    def prepare(self, image: np.ndarray, params: EncodeParams, shifts: Sequence[int]) -> None:
+    def prepare(self, image: np.ndarray, params: EncodeParams, shifts: Sequence[int],
+                method: int = N.METHOD_EXHAUSTIVE) -> None:
         img = np.ascontiguousarray(image, dtype=np.uint8)
         if img.ndim != 2:
             raise ValueError("image must be a 2-D u8 plane")
         h, w = img.shape
         p = N.Params(params.min_size, params.max_size, params.shift, params.bits_alfa,
-                     params.bits_beta, 0, params.max_alfa)
+                     params.bits_beta, method, params.max_alfa)
         sh = (C.c_uint32 * len(shifts))(*shifts)
         self._check(self._L.mars_hip_prepare(
             self._ctx, img.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, C.byref(p), sh,
             len(shifts)), "mars_hip_prepare")
 
     def prepare_batch(self, images: Sequence[np.ndarray], params: EncodeParams,
-                      shifts: Sequence[int]) -> None:
+                      shifts: Sequence[int], method: int = N.METHOD_EXHAUSTIVE) -> None:
         """One pool and search table per plane, searched by one batched step."""
         imgs = [np.ascontiguousarray(im, dtype=np.uint8) for im in images]
         if not imgs or any(im.ndim != 2 for im in imgs):
@@ -134,7 +135,7 @@ class HipContext:
             planes[i].px = im.ctypes.data_as(C.POINTER(C.c_uint8))
             planes[i].h, planes[i].w = im.shape
         p = N.Params(params.min_size, params.max_size, params.shift, params.bits_alfa,
-                     params.bits_beta, 0, params.max_alfa)
+                     params.bits_beta, method, params.max_alfa)
         sh = (C.c_uint32 * len(shifts))(*shifts)
         self._check(self._L.mars_hip_prepare_batch(self._ctx, planes, len(imgs), C.byref(p), sh,
                                                    len(shifts)), "mars_hip_prepare_batch")
@@ -157,6 +158,48 @@ class HipContext:
         rc = self._L.mars_hip_search_plane(self._ctx, plane, row, col, size, shift, 1,
                                            C.byref(out), C.byref(n), C.byref(ev))
         return rc, (out if n.value else None), ev.value
+
+    def search_method(self, method: int, row: int, col: int, size: int, shift: int,
+                      plane: int = 0):
+        """(status, candidate or None, evals) of one retrieval method."""
+        out = N.CCandidate()
+        n = C.c_uint32()
+        ev = C.c_uint64()
+        rc = self._L.mars_hip_search_method(self._ctx, plane, method, row, col, size, shift, 1,
+                                            C.byref(out), C.byref(n), C.byref(ev))
+        return rc, (out if n.value else None), ev.value
+
+    def method_tables(self, method: int, plane: int = 0):
+        """{(size, shift): (rows, cols, candidates, evals per range)} of one method."""
+        res = {}
+        for s, sh, rows, cols, _, _, i in self._tables(plane):
+            nr = len(rows)
+            cands = (N.CCandidate * max(nr, 1))()
+            found = (C.c_uint8 * max(nr, 1))()
+            evals = (C.c_uint64 * max(nr, 1))()
+            self._check(self._L.mars_hip_table_read_method(self._ctx, i, method, cands, found,
+                                                           evals), "table_read_method")
+            res[(s, sh)] = (rows, cols, [cands[j] if found[j] else None for j in range(nr)],
+                            list(evals[:nr]))
+        return res
+
+    def _tables(self, plane: int):
+        n = C.c_uint32()
+        self._check(self._L.mars_hip_table_count(self._ctx, C.byref(n)), "table_count")
+        for i in range(n.value):
+            pl = C.c_uint32()
+            self._check(self._L.mars_hip_table_plane(self._ctx, i, C.byref(pl)), "table_plane")
+            if pl.value != plane:
+                continue
+            s, sh, nr, P = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+            self._check(self._L.mars_hip_table_info(self._ctx, i, C.byref(s), C.byref(sh),
+                                                    C.byref(nr), C.byref(P)), "table_info")
+            rows = (C.c_uint32 * max(nr.value, 1))()
+            cols = (C.c_uint32 * max(nr.value, 1))()
+            self._check(self._L.mars_hip_table_read(self._ctx, i, rows, cols, None, None),
+                        "table_read")
+            yield s.value, sh.value, list(rows[:nr.value]), list(cols[:nr.value]), nr.value, \
+                P.value, i
 
     def tables(self, plane: int = 0):
         """Tables of one plane: list of (size, shift, rows, cols, candidates[list or None per

@@ -111,6 +111,63 @@ void or_provider_free(or_provider *p);
 int or_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
                  void *out, uint32_t *n_out, uint64_t *evals);
 
+/* ---- Fisher classified search and colour plumbing (fisher.c) ---- */
+
+#define OR_FISHER_BUCKETS 72 /* 3 shape classes x 24 variance orders */
+#define OR_FISHER_SLOTS 16
+
+typedef struct {
+    uint32_t dom_row, dom_col, dom_iso;
+} or_fisher_entry;
+
+/* fisher.rs:23-27 buckets, stored flat: bucket b = clas*24 + var_class holds
+ * entries[start[b] .. start[b]+count[b]] in domain-position order; an empty
+ * bucket aliases the first non-empty one (own_count[b] is its own size). */
+typedef struct {
+    uint32_t size, shift, P;
+    uint32_t start[OR_FISHER_BUCKETS], count[OR_FISHER_BUCKETS], own_count[OR_FISHER_BUCKETS];
+    or_fisher_entry *entries;
+} or_fisher_index;
+
+typedef struct {
+    or_provider *base;
+    or_fisher_index *ix[OR_FISHER_SLOTS];
+} or_fisher_provider;
+
+void or_fisher_tables(int32_t *ordering /* [24][6] */, uint8_t *mapping /* [8][8] */);
+uint32_t or_fisher_match(const int32_t order[4]);
+void or_fisher_newclass(const double *b, uint32_t size, uint32_t *iso, uint32_t *clas);
+uint32_t or_fisher_variance_class(const double *b, uint32_t size);
+void or_fisher_flips(const double *b, uint32_t size, uint32_t iso, double *out);
+void or_fisher_classify(const double *b, uint32_t size, uint32_t *iso, uint32_t *clas,
+                        uint32_t *var_class);
+or_fisher_index *or_fisher_index_build(const int32_t *contracted, uint32_t w, uint32_t h,
+                                       uint32_t size, uint32_t shift);
+void or_fisher_index_free(or_fisher_index *ix);
+uint32_t or_fisher_range_class(const uint8_t *px, uint32_t w, uint32_t row, uint32_t col,
+                               uint32_t size, uint32_t *isom);
+void or_fisher_search(const uint8_t *px, uint32_t w, uint32_t h, const int32_t *contracted,
+                      const or_fisher_index *ix, uint32_t row, uint32_t col, double max_alfa,
+                      uint32_t bits_alfa, uint32_t bits_beta, or_candidate *out, uint32_t *n_out,
+                      uint64_t *evals);
+/* Fisher winners of a list of ranges of one size (threaded like or_search_table);
+ * evals_each (optional) receives each range's candidate count. */
+uint64_t or_fisher_table(const uint8_t *px, uint32_t w, uint32_t h, uint32_t size,
+                         uint32_t shift, double max_alfa, uint32_t bits_alfa, uint32_t bits_beta,
+                         const uint32_t *rows, const uint32_t *cols, uint32_t n, int threads,
+                         or_candidate *out, uint8_t *found, uint64_t *evals_each);
+or_fisher_provider *or_fisher_provider_new(const uint8_t *px, uint32_t w, uint32_t h,
+                                           double max_alfa, uint32_t bits_alfa,
+                                           uint32_t bits_beta);
+void or_fisher_provider_free(or_fisher_provider *p);
+int or_fisher_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift, void *out,
+                 uint32_t *n_out, uint64_t *evals);
+
+/* metrics.rs:377-417: interleaved RGB (n pixels) -> Y, Cb, Cr planes. */
+void or_ycbcr(const uint8_t *rgb, size_t n, uint8_t *y, uint8_t *cb, uint8_t *cr);
+/* color.rs:60-78: out is ceil(w/2) x ceil(h/2). */
+void or_downsample_box(const uint8_t *px, uint32_t w, uint32_t h, uint8_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,19 @@ class OrCandidate(C.Structure):
                 ("rms", C.c_double), ("m", Moments)]
 
 
+class FisherEntry(C.Structure):
+    _fields_ = [("dom_row", C.c_uint32), ("dom_col", C.c_uint32), ("dom_iso", C.c_uint32)]
+
+
+NB = 72  # Fisher buckets: 3 shape classes x 24 variance orders
+
+
+class FisherIndex(C.Structure):
+    _fields_ = [("size", C.c_uint32), ("shift", C.c_uint32), ("P", C.c_uint32),
+                ("start", C.c_uint32 * NB), ("count", C.c_uint32 * NB),
+                ("own_count", C.c_uint32 * NB), ("entries", C.POINTER(FisherEntry))]
+
+
 _lib = None
 
 
@@ -68,6 +81,32 @@ def lib() -> C.CDLL:
     L.or_provider_new.restype = C.c_void_p
     L.or_provider_free.argtypes = [C.c_void_p]
     L.or_provider_free.restype = None
+    D = P(dbl)
+    L.or_fisher_tables.argtypes = [P(C.c_int32), P(C.c_uint8)]
+    L.or_fisher_tables.restype = None
+    L.or_fisher_newclass.argtypes = [D, u32, P(u32), P(u32)]
+    L.or_fisher_newclass.restype = None
+    L.or_fisher_variance_class.argtypes = [D, u32]
+    L.or_fisher_variance_class.restype = u32
+    L.or_fisher_flips.argtypes = [D, u32, u32, D]
+    L.or_fisher_flips.restype = None
+    L.or_fisher_classify.argtypes = [D, u32, P(u32), P(u32), P(u32)]
+    L.or_fisher_classify.restype = None
+    L.or_fisher_index_build.argtypes = [P(C.c_int32), u32, u32, u32, u32]
+    L.or_fisher_index_build.restype = P(FisherIndex)
+    L.or_fisher_index_free.argtypes = [P(FisherIndex)]
+    L.or_fisher_index_free.restype = None
+    L.or_fisher_table.argtypes = [P(C.c_uint8), u32, u32, u32, u32, dbl, u32, u32, P(u32),
+                                  P(u32), u32, C.c_int, P(OrCandidate), P(C.c_uint8), P(u64)]
+    L.or_fisher_table.restype = u64
+    L.or_fisher_provider_new.argtypes = [P(C.c_uint8), u32, u32, dbl, u32, u32]
+    L.or_fisher_provider_new.restype = C.c_void_p
+    L.or_fisher_provider_free.argtypes = [C.c_void_p]
+    L.or_fisher_provider_free.restype = None
+    L.or_ycbcr.argtypes = [P(C.c_uint8), C.c_size_t, P(C.c_uint8), P(C.c_uint8), P(C.c_uint8)]
+    L.or_ycbcr.restype = None
+    L.or_downsample_box.argtypes = [P(C.c_uint8), u32, u32, P(C.c_uint8)]
+    L.or_downsample_box.restype = None
     _lib = L
     return L
 
@@ -204,3 +243,120 @@ class Provider:
             self.close()
         except Exception:
             pass
+
+
+# ---- Fisher classified search and colour plumbing ----------------------------
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _u8p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def fisher_tables():
+    o = (C.c_int32 * 144)()
+    m = (C.c_uint8 * 64)()
+    lib().or_fisher_tables(o, m)
+    return np.array(o, np.int32).reshape(24, 6), np.array(m, np.uint8).reshape(8, 8)
+
+
+def fisher_newclass(block: np.ndarray):
+    b = np.ascontiguousarray(block, np.float64)
+    iso, clas = C.c_uint32(), C.c_uint32()
+    lib().or_fisher_newclass(_dp(b), b.shape[0], C.byref(iso), C.byref(clas))
+    return iso.value, clas.value
+
+
+def fisher_variance_class(block: np.ndarray) -> int:
+    b = np.ascontiguousarray(block, np.float64)
+    return lib().or_fisher_variance_class(_dp(b), b.shape[0])
+
+
+def fisher_flips(block: np.ndarray, iso: int) -> np.ndarray:
+    b = np.ascontiguousarray(block, np.float64)
+    out = np.zeros_like(b)
+    lib().or_fisher_flips(_dp(b), b.shape[0], iso, _dp(out))
+    return out
+
+
+def fisher_classify(block: np.ndarray):
+    """(isometry, class, variance class) of one block (fisher.rs:35-38)."""
+    b = np.ascontiguousarray(block, np.float64)
+    iso, clas, vc = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    lib().or_fisher_classify(_dp(b), b.shape[0], C.byref(iso), C.byref(clas), C.byref(vc))
+    return iso.value, clas.value, vc.value
+
+
+def fisher_index(img: np.ndarray, size: int, shift: int):
+    """Buckets of one (size, stride): (list of 72 entry lists, own counts)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape
+    d = contract(img)
+    L = lib()
+    ix = L.or_fisher_index_build(d.ctypes.data_as(C.POINTER(C.c_int32)), w, h, size, shift)
+    try:
+        x = ix.contents
+        buckets = [[(x.entries[i].dom_row, x.entries[i].dom_col, x.entries[i].dom_iso)
+                    for i in range(x.start[b], x.start[b] + x.count[b])] for b in range(NB)]
+        own = list(x.own_count)
+    finally:
+        L.or_fisher_index_free(ix)
+    return buckets, own
+
+
+def fisher_table(img, size, shift, rows, cols, threads=8, max_alfa=1.0, bits_alfa=4,
+                 bits_beta=7):
+    """Fisher winners (None where no candidate) and per-range evals."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape
+    n = len(rows)
+    r = (C.c_uint32 * max(n, 1))(*rows)
+    c = (C.c_uint32 * max(n, 1))(*cols)
+    out = (OrCandidate * max(n, 1))()
+    found = (C.c_uint8 * max(n, 1))()
+    each = (C.c_uint64 * max(n, 1))()
+    lib().or_fisher_table(_u8p(img), w, h, size, shift, max_alfa, bits_alfa, bits_beta, r, c, n,
+                          threads, out, found, each)
+    return [out[i] if found[i] else None for i in range(n)], [each[i] for i in range(n)]
+
+
+class FisherProvider:
+    """The oracle's Fisher search as a mars_search_fn callback + user pointer."""
+
+    def __init__(self, img, max_alfa=1.0, bits_alfa=4, bits_beta=7):
+        L = lib()
+        img = np.ascontiguousarray(img, np.uint8)
+        h, w = img.shape
+        self._L = L
+        self.user = C.c_void_p(L.or_fisher_provider_new(_u8p(img), w, h, max_alfa, bits_alfa,
+                                                        bits_beta))
+        self.fn = C.cast(L.or_fisher_cb, C.c_void_p)
+
+    def close(self):
+        if self.user:
+            self._L.or_fisher_provider_free(self.user)
+            self.user = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ycbcr(rgb: np.ndarray):
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    h, w, _ = rgb.shape
+    y, cb, cr = (np.zeros((h, w), np.uint8) for _ in range(3))
+    lib().or_ycbcr(_u8p(rgb), h * w, _u8p(y), _u8p(cb), _u8p(cr))
+    return y, cb, cr
+
+
+def downsample_box(plane: np.ndarray) -> np.ndarray:
+    plane = np.ascontiguousarray(plane, np.uint8)
+    h, w = plane.shape
+    out = np.zeros(((h + 1) // 2, (w + 1) // 2), np.uint8)
+    lib().or_downsample_box(_u8p(plane), w, h, _u8p(out))
+    return out
