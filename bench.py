@@ -108,6 +108,67 @@ def _batch_leg(name: str, specs, steps: int):
                                  for i in range(6) if tune.ms[i] > 0}}
 
 
+def _colour_leg(steps: int):
+    """The colour workload: 768x512 RGB (seed 3) -> BT.601 YCbCr 4:2:0, Y quadtree
+    32->4, Cb/Cr 16->4, stride 4, Fisher classified search as the provider and
+    the exhaustive tables for the RD warm-up.  Device steps (both contexts,
+    graph replays) plus one full encode to MARC bytes (host walks included)."""
+    from mars_amd import _native as N
+    from mars_amd import codec, synth
+    from mars_amd.codec import EncodeOptions
+    from mars_amd.provider import EncodeParams, HipContext
+    from tests import _oracle as O
+
+    rgb = synth.rgb(768, 512, synth.WORKLOAD_SEEDS["rgb768x512"])
+    y, cb, cr = codec.ycbcr(rgb)
+    cb, cr = codec.downsample_box(cb), codec.downsample_box(cr)
+    yp = EncodeParams(min_size=4, max_size=32, shift=SHIFT, lambda_=200.0)
+    cp = EncodeParams(min_size=4, max_size=16, shift=SHIFT, lambda_=200.0)
+    out = {}
+    for name, planes, p in (("y", [y], yp), ("chroma", [cb, cr], cp)):
+        ctx = HipContext(0)
+        ctx.prepare_batch(planes, p, [SHIFT], method=N.METHOD_FISHER)
+        ctx.run(iters=2, use_graph=True)
+        br = [ctx.run(iters=1, use_graph=False) for _ in range(5)]
+        t = ctx.run(iters=steps, use_graph=True)
+        ctx.close()
+        out[name] = dict(ms_step=t.ms_step, exhaustive_evals=t.evals,
+                         fisher_evals=int(br[0].fisher_evals),
+                         ms_fisher=float(np.mean([b.ms_fisher for b in br])),
+                         ms_exhaustive_search=float(np.mean([b.ms_search for b in br])))
+    ctx = HipContext(0)
+    o = EncodeOptions(allowed_modes=(True, False, True, False))
+    t0 = time.perf_counter()
+    data, stats, nbytes = codec.hip_encode_rgb(ctx, rgb, yp, cp, o, N.METHOD_FISHER)
+    enc_ms = (time.perf_counter() - t0) * 1e3
+    ctx.close()
+    fe = out["y"]["fisher_evals"] + out["chroma"]["fisher_evals"]
+    fms = out["y"]["ms_fisher"] + out["chroma"]["ms_fisher"]
+    # the oracle's Fisher tables of the same nodes, all host threads
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    cpu_evals = 0
+    for plane, p in ((y, yp), (cb, cp), (cr, cp)):
+        h, w = plane.shape
+        for size, rc in O.nodes(w, h, p.min_size, p.max_size).items():
+            _, each = O.fisher_table(plane, size, SHIFT, [r for r, _ in rc], [c for _, c in rc],
+                                     threads=threads)
+            cpu_evals += sum(each)
+    cpu_s = time.perf_counter() - t0
+    return {"workload": "rgb768x512: 768x512 RGB (seed 3) -> YCbCr 4:2:0; Y 32->4, Cb/Cr 16->4, "
+                        "stride 4, lambda 200, Fisher provider + exhaustive RD warm-up",
+            "ms_device_step": out["y"]["ms_step"] + out["chroma"]["ms_step"],
+            "device_step": out,
+            "fisher_evals": fe, "ms_fisher": fms, "fisher_evals_per_s": fe / (fms * 1e-3),
+            "encode_ms_wall": enc_ms, "marc_bytes": len(data), "plane_bytes": list(nbytes),
+            "search_evals": {k: v.search_evals for k, v in stats.items()},
+            "warmup_evals": {k: v.warmup_evals for k, v in stats.items()},
+            "cpu_fisher": {"value": cpu_evals / cpu_s, "unit": "evals/s", "cores": threads,
+                           "kind": "port", "sample": f"the oracle's Fisher tables of every node "
+                                                     f"of the three planes, {cpu_evals} evals "
+                                                     f"in {cpu_s:.2f} s"}}
+
+
 def _cpu_leg(img, budget_s: float = 12.0):
     """The oracle (C restatement of the reference CPU search) on the host cores,
     on a bounded sample of the same workload: ranges drawn from every size."""
@@ -257,6 +318,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-image legs")
+    ap.add_argument("--no-colour", action="store_true", help="skip the RGB 4:2:0 Fisher leg")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one eager MFMA step + a 256 MiB copy kernel (for rocprofv3 --pmc)")
     ap.add_argument("--precision", action="store_true",
@@ -297,6 +359,7 @@ def main() -> int:
                    "batch64": _batch_leg("64 planes of uneven sizes (mars_amd.synth.batch_specs, "
                                          "seeds 100..163), quadtree 32->4, stride 4",
                                          synth.batch_specs(), 3)}
+    colour = None if args.no_colour else _colour_leg(args.steps)
     cpu = None if args.no_cpu else _cpu_leg(img)
     line = {
         "metric": "R×D×iso evals/s, split-INT8 MFMA vs FP32 kernel; batched img/s @512²",
@@ -317,6 +380,7 @@ def main() -> int:
         "images_per_s": 1e3 / mf["ms_step"],
         "batched_images_per_s_512": batches["batch512"]["images_per_s"] if batches else None,
         "batches": batches,
+        "colour": colour,
         "fp32_kernel": {"value": fp_value, "unit": "evals/s", "ms_per_step": fp["ms_step"],
                         "ranges_differing_from_mfma_winner": diverge},
         "speedup_vs_fp32": value / fp_value,

@@ -143,7 +143,7 @@ struct mars_hip_ctx {
     Work3 *d_work3 = nullptr;
     size_t work2_cap = 0, work3_cap = 0;
     Slice wl_contract, wl_sat_rows, wl_sat_cols, wl_possums, wl_ranges, wl_rstats, wl_seed;
-    Slice wl_fdom, wl_frng, wl_ftab; // Fisher: positions, ranges, tables
+    Slice wl_fdom, wl_frng, wl_ftab, wl_frange; // Fisher: positions, range blocks, tables, ranges
     unsigned long long *d_counters = nullptr;
     double *d_yscale = nullptr;
     Common common{};
@@ -460,6 +460,13 @@ int layout(mars_hip_ctx *ctx)
         if (t.fisher && t.P > 0) w2.push_back(make_uint2(i, 0));
     }
     end2(ctx->wl_ftab);
+    begin2(ctx->wl_frange);
+    for (uint32_t i = 0; i < ctx->tables.size(); i++) {
+        const Table &t = ctx->tables[i];
+        if (!t.fisher || t.P == 0) continue;
+        for (uint32_t r = 0; r < t.nr; r++) w2.push_back(make_uint2(i, r));
+    }
+    end2(ctx->wl_frange);
     for (Group &g : ctx->groups) {
         for (int k = 0; k < 2; k++) {
             g.search[k].off = (uint32_t)w3.size();
@@ -555,7 +562,8 @@ void enqueue_fisher(mars_hip_ctx *ctx, hipStream_t st)
 {
     const Work2 *w2 = ctx->d_work2;
     mars::launch_fisher(ctx->d_tabs, w2 + ctx->wl_fdom.off, ctx->wl_fdom.n, w2 + ctx->wl_frng.off,
-                        ctx->wl_frng.n, w2 + ctx->wl_ftab.off, ctx->wl_ftab.n, ctx->common, st);
+                        ctx->wl_frng.n, w2 + ctx->wl_ftab.off, ctx->wl_ftab.n,
+                        w2 + ctx->wl_frange.off, ctx->wl_frange.n, ctx->common, st);
 }
 
 // Enqueue one whole step.  With ev (4 + n_groups events: before the pool,

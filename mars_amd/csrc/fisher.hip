@@ -12,11 +12,12 @@
 //                     offsets, the positions grouped by bucket, and every
 //                     empty bucket aliased to the first non-empty one in
 //                     (class, variance class) order (fisher.rs:42-54);
-//   k_fisher_search   one wave per range: every candidate of the range's
-//                     bucket with isometry MAPPING[range iso][domain iso]
-//                     (fisher.rs:62-75), its exact integer t1 and the
-//                     verbatim binary64 fit, the first strict minimum of rms
-//                     in bucket order kept (lib.rs:254-286).
+//   k_fisher_search   one wave per range, one lane per candidate of the
+//                     range's bucket, isometry MAPPING[range iso][domain iso]
+//                     (fisher.rs:62-75): exact integer t1 by i8 dot products
+//                     on the split planes, the verbatim binary64 fit, the
+//                     first strict minimum of rms in bucket order kept
+//                     (lib.rs:254-286).
 //
 // Bucket order is domain-position order in the reference, and a position
 // appears at most once per bucket, so "first strict minimum in bucket
@@ -108,7 +109,8 @@ __global__ __launch_bounds__(256) void k_fisher_ranges(const TableDesc *__restri
 
 // Work item (table, 0), one 1024-thread block per table: bucket sizes,
 // offsets (empty buckets aliased to the first non-empty one) and the
-// positions grouped by bucket, each entry p * 8 + the position's isometry.
+// positions grouped by bucket, each entry (the position's slot in the
+// search-order records) * 8 + the position's isometry.
 __global__ __launch_bounds__(1024) void k_fisher_buckets(const TableDesc *__restrict__ tabs,
                                                          const Work2 *__restrict__ work)
 {
@@ -136,75 +138,126 @@ __global__ __launch_bounds__(1024) void k_fisher_buckets(const TableDesc *__rest
     for (uint32_t p = t; p < P; p += 1024) {
         const uint32_t c = T.fcls[p];
         const uint32_t slot = atomicAdd(&cur[c >> 3], 1u);
-        T.flist[slot] = p * 8 + (c & 7);
+        T.flist[slot] = (T.iperm ? T.iperm[p] : p) * 8 + (c & 7);
     }
 }
 
-// Work item (table, block of 4 ranges), one wave per range.  The wave
-// takes 64 / L candidates at a time, L = min(size, 16) lanes each; lane j
-// of a group correlates columns j, j + L, ... of every window row (a
-// coalesced row segment per load), and the group sums its t1 by shuffles.
-// Each candidate is fitted with the reference's binary64 fit and the
-// minimum by (rms, position) kept.
-__global__ __launch_bounds__(256) void k_fisher_search(const TableDesc *__restrict__ tabs,
-                                                       const Work2 *__restrict__ work, Common cm)
+// One row of S bytes of a window (or of a permuted range) as S/4 packed
+// dwords; the split planes' copies keep every row segment aligned to its
+// load width (4, 8 or 16 bytes).
+template <uint32_t S> struct RowW {
+    int32_t w[S / 4];
+};
+
+template <uint32_t S> __device__ __forceinline__ RowW<S> load_row(const GLB int8_t *p)
 {
-    const Work2 wk = work[blockIdx.x];
-    const TableG T = view(tabs[wk.x]);
-    const uint32_t r = wk.y * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (r >= T.nr) return;
-    const uint32_t size = T.size, K = size * size;
-    const uint32_t L = size < 16 ? size : 16, per = 64 / L, g = lane / L, li = lane % L;
+    typedef int32_t v2 __attribute__((ext_vector_type(2)));
+    typedef int32_t v4 __attribute__((ext_vector_type(4)));
+    RowW<S> r;
+    if constexpr (S == 4) {
+        r.w[0] = *(const GLB int32_t *)p;
+    } else if constexpr (S == 8) {
+        const v2 x = *(const GLB v2 *)p;
+        r.w[0] = x.x;
+        r.w[1] = x.y;
+    } else {
+#pragma unroll
+        for (uint32_t q = 0; q < S / 16; q++) {
+            const v4 x = *(const GLB v4 *)(p + 16 * q);
+            r.w[4 * q] = x.x;
+            r.w[4 * q + 1] = x.y;
+            r.w[4 * q + 2] = x.z;
+            r.w[4 * q + 3] = x.w;
+        }
+    }
+    return r;
+}
+
+// sum r' * hi and sum r' * lo over an S x S window of the split planes
+// (D = 4 * (hi + 128) + lo, r' = r - 128): four i8 products per dot
+// instruction.
+template <uint32_t S>
+__device__ __forceinline__ void window_dots(const GLB int8_t *hi, const GLB int8_t *lo,
+                                            uint32_t pitch, const GLB int8_t *rk, int32_t &ah,
+                                            int32_t &al)
+{
+    if constexpr (S == 2) {
+#pragma unroll
+        for (uint32_t u = 0; u < 2; u++)
+#pragma unroll
+            for (uint32_t v = 0; v < 2; v++) {
+                const int32_t rv = rk[u * 2 + v];
+                ah += rv * (int32_t)hi[u * pitch + v];
+                al += rv * (int32_t)lo[u * pitch + v];
+            }
+    } else {
+        constexpr uint32_t U = S <= 8 ? S : 4;
+#pragma unroll U
+        for (uint32_t u = 0; u < S; u++) {
+            const RowW<S> h = load_row<S>(hi + u * pitch), l = load_row<S>(lo + u * pitch);
+            const RowW<S> x = load_row<S>(rk + u * S);
+#pragma unroll
+            for (uint32_t q = 0; q < S / 4; q++) {
+                ah = __builtin_amdgcn_sdot4(x.w[q], h.w[q], ah, false);
+                al = __builtin_amdgcn_sdot4(x.w[q], l.w[q], al, false);
+            }
+        }
+    }
+}
+
+// One range (one wave): lane j takes candidates j, j + 64, ... of the
+// range's bucket.  Per candidate the search-order record gives the window
+// offset, S1 and the position index; t1 is exact from the split planes:
+//   t1 = 4 sum r'hi + sum r'lo + 512 sum r' + 128 S1;
+// then the reference's binary64 fit, and the minimum by (rms, position).
+template <uint32_t S>
+__device__ __forceinline__ void fisher_range(const TableG &T, uint32_t r, uint32_t lane,
+                                             const Common &cm)
+{
+    constexpr uint32_t K = S * S;
     const uint32_t code = T.frcls[r], b = code >> 3, isom = code & 7;
     const uint32_t start = T.fbuck[b], cnt = T.fbuck[kBuckets + b];
     const int64_t t0 = T.rstat[2 * r], t2 = T.rstat[2 * r + 1];
-    const GLB int32_t *d = T.d;
-    const uint32_t wc = T.wc;
+    const int32_t off_r = 512 * (int32_t)(t0 - 128 * (int64_t)K); // 512 sum r'
+    const GLB int8_t *rng = T.rng8 + (size_t)r * 8 * T.kmem;
     double best = __longlong_as_double(0x7ff0000000000000ll);
     uint32_t bp = 0xffffffffu, bk = 0;
-    int32_t bt1 = 0;
-    for (uint32_t base = 0; base < cnt; base += per) {
-        const uint32_t c = base + g;
-        const bool valid = c < cnt;
-        uint32_t p = 0, k = 0;
-        int32_t t1 = 0;
-        if (valid) {
-            const uint32_t e = T.flist[start + c];
-            p = e >> 3;
-            k = fisher::kMapping[isom][e & 7];
-            const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
-            const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
-            const GLB int8_t *rk = T.rng8 + ((size_t)r * 8 + k) * T.kmem;
-            for (uint32_t u = 0; u < size; u++) {
-                const GLB int32_t *drow = d + (size_t)(dr + u) * wc + dc;
-                for (uint32_t v = li; v < size; v += L)
-                    t1 += ((int32_t)rk[u * size + v] + 128) * drow[v];
-            }
-        }
-        for (uint32_t off = L / 2; off >= 1; off >>= 1) t1 += __shfl_xor(t1, (int)off, 64);
-        if (valid) {
-            uint32_t qa, qb;
-            double rms, sum;
-            fit_f64((int64_t)K, T.posx[2 * p], T.posx[2 * p + 1], t0, t1, t2, cm.max_alfa,
-                    cm.bits_alfa, cm.bits_beta, &qa, &qb, &rms, &sum);
-            if (rms < best || (rms == best && p < bp)) {
-                best = rms;
-                bp = p;
-                bk = k;
-                bt1 = t1;
-            }
+    int32_t bt1 = 0, bs1 = 0, bs2 = 0;
+    for (uint32_t c = lane; c < cnt; c += 64) {
+        const uint32_t e = T.flist[start + c];
+        const uint32_t k = fisher::kMapping[isom][e & 7];
+        const uint4 rec = ld_u4(T.posinfo, e >> 3); // {window offset, S1, -, p}
+        const uint32_t p = rec.w;
+        int32_t ah = 0, al = 0;
+        window_dots<S>(T.hi + rec.x, T.lo + rec.x, T.pitch, rng + (size_t)k * T.kmem, ah, al);
+        const int32_t s1 = (int32_t)rec.y, s2 = T.posx[2 * p + 1];
+        const int32_t t1 = 4 * ah + al + off_r + 128 * s1;
+        uint32_t qa, qb;
+        double rms, sum;
+        fit_f64((int64_t)K, s1, s2, t0, t1, t2, cm.max_alfa, cm.bits_alfa, cm.bits_beta, &qa,
+                &qb, &rms, &sum);
+        if (rms < best || (rms == best && p < bp)) {
+            best = rms;
+            bp = p;
+            bk = k;
+            bt1 = t1;
+            bs1 = s1;
+            bs2 = s2;
         }
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         const double ob = __shfl_xor(best, off, 64);
         const uint32_t op = __shfl_xor(bp, off, 64), ok = __shfl_xor(bk, off, 64);
-        const int32_t ot1 = __shfl_xor(bt1, off, 64);
+        const int32_t ot1 = __shfl_xor(bt1, off, 64), os1 = __shfl_xor(bs1, off, 64),
+                      os2 = __shfl_xor(bs2, off, 64);
         if (ob < best || (ob == best && op < bp)) {
             best = ob;
             bp = op;
             bk = ok;
             bt1 = ot1;
+            bs1 = os1;
+            bs2 = os2;
         }
     }
     if (lane != 0) return;
@@ -220,8 +273,8 @@ __global__ __launch_bounds__(256) void k_fisher_search(const TableDesc *__restri
     w.dom_col = pj * T.shift;
     w.isometry = bk;
     w.s0 = (int64_t)K;
-    w.s1_x4 = T.posx[2 * bp];
-    w.s2_x16 = T.posx[2 * bp + 1];
+    w.s1_x4 = bs1;
+    w.s2_x16 = bs2;
     w.t0 = t0;
     w.t1_x4 = bt1;
     w.t2 = t2;
@@ -236,14 +289,31 @@ __global__ __launch_bounds__(256) void k_fisher_search(const TableDesc *__restri
     T.ffound[r] = 1;
 }
 
+// Work item (table, range), one wave.
+__global__ __launch_bounds__(64) void k_fisher_search(const TableDesc *__restrict__ tabs,
+                                                      const Work2 *__restrict__ work, Common cm)
+{
+    const Work2 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t r = wk.y, lane = threadIdx.x;
+    switch (T.size) {
+    case 2: fisher_range<2>(T, r, lane, cm); break;
+    case 4: fisher_range<4>(T, r, lane, cm); break;
+    case 8: fisher_range<8>(T, r, lane, cm); break;
+    case 16: fisher_range<16>(T, r, lane, cm); break;
+    case 32: fisher_range<32>(T, r, lane, cm); break;
+    }
+}
+
 void launch_fisher(const TableDesc *t, const Work2 *dom_work, uint32_t n_dom,
                    const Work2 *rng_work, uint32_t n_rng, const Work2 *tab_work, uint32_t n_tab,
-                   const Common &c, hipStream_t st)
+                   const Work2 *range_work, uint32_t n_range, const Common &c, hipStream_t st)
 {
     if (n_dom) hipLaunchKernelGGL(k_fisher_domains, dim3(n_dom), dim3(256), 0, st, t, dom_work);
     if (n_rng) hipLaunchKernelGGL(k_fisher_ranges, dim3(n_rng), dim3(256), 0, st, t, rng_work);
     if (n_tab) hipLaunchKernelGGL(k_fisher_buckets, dim3(n_tab), dim3(1024), 0, st, t, tab_work);
-    if (n_rng) hipLaunchKernelGGL(k_fisher_search, dim3(n_rng), dim3(256), 0, st, t, rng_work, c);
+    if (n_range)
+        hipLaunchKernelGGL(k_fisher_search, dim3(n_range), dim3(64), 0, st, t, range_work, c);
 }
 
 } // namespace mars

@@ -108,13 +108,14 @@ int launch_search(int kernel, uint32_t size, const TableDesc *t, const Work3 *wo
                   uint32_t nwork, const Common &c, hipStream_t st);
 void launch_merge(const TableDesc *t, const Work2 *work, uint32_t nwork, bool t1_exact,
                   const Common &c, hipStream_t st);
-// Fisher classified search of the tables in `work` (x = table, y = 0):
-// classification of every domain position and range, the buckets, then
-// every range's bucket scan.  Needs the pool, k_possums, k_ranges and
-// k_rstats of those tables.
+// Fisher classified search: classification of every domain position
+// (dom_work: table, block of 256 positions) and range (rng_work: table,
+// block of 4 ranges), the buckets (tab_work: table, 0), then every range's
+// bucket scan (range_work: table, range).  Needs the pool, k_possums,
+// k_ranges and k_rstats of those tables.
 void launch_fisher(const TableDesc *t, const Work2 *dom_work, uint32_t n_dom,
                    const Work2 *rng_work, uint32_t n_rng, const Work2 *tab_work, uint32_t n_tab,
-                   const Common &c, hipStream_t st);
+                   const Work2 *range_work, uint32_t n_range, const Common &c, hipStream_t st);
 void launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t st);
 void launch_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
                          uint32_t bits_beta, uint32_t *qa, uint32_t *qb, double *rms,

@@ -166,7 +166,8 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
     const uint32_t p = wk.y * blockDim.x + threadIdx.x;
     if (p >= T.P_pad) return;
     if (p >= T.P) { // padding slot of the last search tile: no position
-        T.posinfo[p] = (u32x4){0u, 0u, 0u, 0xffffffffu};
+        // S1 = -2^21: the MFMA search's stage-1 limit excludes the slot.
+        T.posinfo[p] = (u32x4){0u, (uint32_t)-(1 << 21), 0u, 0xffffffffu};
         return;
     }
     const uint32_t W1 = T.wc + 1, size = T.size;
@@ -526,8 +527,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     int32_t T0[4];  // t0 (< 2^24)
     double A[4];    // A (sizes >= 16)
     float Alo[4];   // A rounded down to binary32 (sizes <= 8)
-    int32_t off[4]; // t1 = 4*hi + lo + off + 128*S1
     int32_t rmin[4]; // INT_MIN for a real range, INT_MAX for padding
+    // Sizes <= 8 fold the no-pruning and padding cases into the offset the
+    // stage-1 limit subtracts.  There |v| < 2^23, |off| < 2^23, S1*128 <
+    // 2^24, and X >> LOG2N < 2^25 for a real position (< 2^30 for any S1):
+    // +2^27 while a range cannot prune puts its limit below every v; -2^28
+    // for a padding range and, through S1 = -2^21 in the padding position
+    // record, -2^28 for a padding position put the limit above every v; no
+    // combination leaves int32.
+    int32_t offx[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const uint32_t r = g * 8 + 4 * h + i;
@@ -537,9 +545,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         T0[i] = (int32_t)t0;
         A[i] = (double)(n * t2 - t0 * t0);
         Alo[i] = (float)A[i] * (1.0f - 3e-7f); // <= A after both roundings
-        off[i] = (int32_t)(512 * (t0 - 128 * n));
         rmin[i] = rv ? (int32_t)0x80000000 : 0x7fffffff;
+        // sa < 0 until the first threshold
+        offx[i] = (int32_t)(512 * (t0 - 128 * n)) + (rv ? (1 << 27) : -(1 << 28));
     }
+    // t1 = 4*hi + lo + off + 128*S1 with off = 512 * (t0 - 128 n) (< 2^27 in
+    // magnitude), recomputed from t0 where needed rather than kept live.
+    auto off_of = [&](int i) { return (T0[i] - (int32_t)(128 * n)) * 512; };
     // Stage-1 cache per range: sqrt(A - n tau) (negative when A - n tau <= 0)
     // and the threshold bits it was computed from.
     float sa[4];
@@ -676,7 +688,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     }
     const bool exchange = nchunks > 1;
 
-    for (uint32_t tile = t_begin + wave; tile < t_end; tile += 4, tcount++) {
+    // One tile: `cur` holds its window fragments and position record, the
+    // next tile's are fetched into `nxt`.  The loop below runs two tiles per
+    // trip with the two fragment sets swapping roles, so no register copy
+    // (and no wait for the prefetch) sits between the tiles.
+    auto step = [&](const uint32_t tile, const Frag &cur, Frag &nxt) __attribute__((always_inline)) {
         // Exchange thresholds with the other blocks of this range group.
         if (exchange && (tcount & 15) == 15 && lane < 8) {
             uint32_t r = g * 8 + lane;
@@ -687,7 +703,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         uint32_t thb[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) thb[i] = lds_load(&s_thr[4 * h + i]);
-        const Frag cur = nxt;
+        // (cur: this tile; nxt: filled for tile + 4 below)
         if (tile + 4 < t_end) fetch_b(pin_ahead, nxt);
         if (tile + 8 < t_end) pin_ahead = load_pin(tile + 8);
         const uint32_t p = cur.pin.w; // position index, ~0 for padding
@@ -720,26 +736,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                     // (sqDi is sqrt(Di) rounded down).
                     // negative: A - n tau <= 0, no pruning for this range
                     sa[i] = amt > 0.0f ? __builtin_amdgcn_sqrtf(amt) * (1.0f - 2e-5f) : -1.0f;
+                    if constexpr (INT_LIMIT)
+                        offx[i] = off_of(i) + (rmin[i] == (int32_t)0x80000000
+                                                ? (sa[i] < 0.0f ? (1 << 27) : 0)
+                                                : -(1 << 28));
                 }
             }
         }
         // Stage-1 limit per (range, position) on v = 4*hi + lo.
-        int32_t lim[4], toff[4];
-        const int32_t o128 = S1p << 7;
+        int32_t lim[4];
+        const int32_t o128 = (int32_t)((uint32_t)S1p << 7);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            toff[i] = off[i] + o128;
             const float nt = fabsf(sa[i]) * sqDi; // <= sqrt((A - n tau) Di) when sa >= 0
-            int32_t L;
             if constexpr (INT_LIMIT) {
                 // S1*t0 + nt < 2^31 for sizes <= 8, S1 and t0 < 2^24.
                 const uint32_t X = __umul24((uint32_t)S1p, (uint32_t)T0[i]) + (uint32_t)nt;
-                L = (int32_t)(X >> LOG2N) - 1 - toff[i];
+                lim[i] = (int32_t)(X >> LOG2N) - 1 - (offx[i] + o128);
             } else {
                 const double X = ((double)S1p * (double)T0[i] + (double)nt) * inv_n;
-                L = (int32_t)floor(X) - 1 - toff[i];
+                const int32_t L = (int32_t)floor(X) - 1 - (off_of(i) + o128);
+                lim[i] = max(sa[i] < 0.0f ? (int32_t)0x80000000 : L, max(rmin[i], pmin));
             }
-            lim[i] = max(sa[i] < 0.0f ? (int32_t)0x80000000 : L, max(rmin[i], pmin));
         }
 
         // Epilogue of one row tile: one compare per value, issued 8 at a
@@ -757,25 +775,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
 #pragma unroll
                 for (int g0 = 0; g0 < DRAIN; g0 += GW) {
                     int32_t v[GW];
-                    unsigned long long any = 0;
+                    unsigned long long msk[GW], any = 0;
 #pragma unroll
                     for (int e = 0; e < GW; e++) {
                         const int t = grp * DRAIN + g0 + e;
                         v[e] = 4 * ah[t] + al[t];
-                        any |= __ballot(v[e] > lim[t & 3]);
+                        msk[e] = __ballot(v[e] > lim[t & 3]);
+                        any |= msk[e];
                     }
                     if (any == 0) continue;
+                    // The compare masks stay in scalar registers: a value
+                    // without survivors costs one scalar test here.
 #pragma unroll
                     for (int e = 0; e < GW; e++) {
                         const int t = grp * DRAIN + g0 + e, i = t & 3;
-                        const bool sv = v[e] > lim[i];
-                        const unsigned long long m = __ballot(sv);
+                        const unsigned long long m = msk[e];
                         if (m) {
+                            const bool sv = v[e] > lim[i]; // the compare behind m
                             const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
                                 (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
                             if (sv) {
                                 const uint32_t slot = 4 * h + i;
-                                q[qn + pre] = make_uint2((uint32_t)(v[e] + toff[i]) | (slot << 28),
+                                q[qn + pre] = make_uint2((uint32_t)(v[e] + off_of(i) + o128) | (slot << 28),
                                                          p * 8 + 4 * tt + (t >> 2));
                             }
                             qn += (uint32_t)__popcll(m);
@@ -850,6 +871,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
             epilogue(acc_hi[0], acc_lo[0], 0);
             epilogue(acc_hi[1], acc_lo[1], 1);
         }
+        tcount++;
+    };
+    Frag alt = {};
+    for (uint32_t tile = t_begin + wave; tile < t_end; tile += 8) {
+        step(tile, nxt, alt);
+        if (tile + 4 >= t_end) break;
+        step(tile + 4, alt, nxt);
     }
     if (qn > 0) process(0, qn);
     if (lane == 0) atomicAdd(&s_count[0], qtotal);
