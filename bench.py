@@ -12,6 +12,10 @@ A step runs pool build + range prep + search + merge on device-resident data
 one captured HIP graph.  `value` is evals/s of the split-INT8 MFMA kernel; the
 plain FP32-FMA kernel is timed beside it on the same workload.
 
+Also measured (extra keys of the same line): 64-plane batches, the colour
+workload (768x512 RGB -> YCbCr 4:2:0, Fisher classified search with the
+exhaustive RD warm-up) and the CPU oracle on a bounded sample.
+
 Prints ONE JSON line.
 """
 
@@ -319,6 +323,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-image legs")
     ap.add_argument("--no-colour", action="store_true", help="skip the RGB 4:2:0 Fisher leg")
+    ap.add_argument("--colour-only", action="store_true",
+                    help="only the RGB 4:2:0 Fisher leg (one JSON line of its own)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one eager MFMA step + a 256 MiB copy kernel (for rocprofv3 --pmc)")
     ap.add_argument("--precision", action="store_true",
@@ -330,6 +336,10 @@ def main() -> int:
         return _traffic_probe()
     if args.precision:
         return _precision(args.precision_sample)
+    if args.colour_only:
+        print(json.dumps({"metric": "RGB 4:2:0 Fisher encode, device step and Fisher evals/s",
+                          "colour": _colour_leg(args.steps)}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share of the search; --gpus {args.gpus} is not "
               "supported (the search shards by range block with no exchange but a per-range "

@@ -23,7 +23,7 @@
 // appears at most once per bucket, so "first strict minimum in bucket
 // order" is the minimum of (rms, position index): the grouped lists need no
 // particular order inside a bucket.  Evals = the bucket's size
-// (lib.rs:270, one per candidate).
+// (lib.rs:269, one per candidate).
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
