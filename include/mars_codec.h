@@ -10,7 +10,8 @@
  *   mars_encode_plane      encode::encode_image_with_search
  *                          (crates/mars-codec/src/encode.rs:1054-1126)
  *                          followed by mars_format::write (mars_format.rs:232-297)
- *   mars_search_fn         trait SearchProvider::search (encode.rs:557-579)
+ *   mars_search_fn         trait SearchProvider::search / search_candidates
+ *                          (encode.rs:557-579)
  *   mars_wrap_gray         color::wrap_gray_stream (color.rs:382-384, 778-790)
  *   mars_hip_encode_plane  encmars --method <hip> (encmars.rs:780-796): the
  *                          provider is the GPU search table of a context.
@@ -64,7 +65,7 @@ typedef struct {
     uint8_t lambda_adaptive_qstep; /* ResidualQuantisation::LambdaAdaptive */
     uint8_t _pad[2];
     double fixed_qstep;            /* ResidualQuantisation::Fixed (default 8.0) */
-    uint32_t rd_candidates;        /* must be 1 */
+    uint32_t rd_candidates;        /* P5c ranked candidates for modes 2/3: 1 (default) .. 32 */
     uint32_t _pad2;
 } mars_encode_options;
 
@@ -78,13 +79,20 @@ typedef struct {
     uint64_t n_leaves;
 } mars_encode_stats;
 
+/* Most ranked candidates a provider is asked for (rd_candidates). */
+#define MARS_MAX_RANKED 32
+
 /*
- * SearchProvider::search: the winner for range (row, col, size) at domain
- * stride `shift`; *n_out = 0 when there is none.  Returns 0 or an error.
+ * SearchProvider::search_candidates (encode.rs:557-579): up to `max` of the
+ * best candidates for range (row, col, size) at domain stride `shift`,
+ * ascending by rms (ties in the provider's encounter order), into out[0..max);
+ * *n_out = how many (0 when there is none).  max == 1 is SearchProvider::search.
+ * A provider that cannot rank returns its winner only (the trait default).
+ * Returns 0 or an error.
  */
 typedef int (*mars_search_fn)(void *user, uint32_t row, uint32_t col, uint32_t size,
-                              uint32_t shift, mars_hip_candidate *out, uint32_t *n_out,
-                              uint64_t *evals);
+                              uint32_t shift, uint32_t max, mars_hip_candidate *out,
+                              uint32_t *n_out, uint64_t *evals);
 
 /*
  * encode_image_with_search + mars_format::write.  `search` serves the final

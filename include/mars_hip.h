@@ -131,10 +131,13 @@ int mars_hip_prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *planes, uint
 
 /*
  * SearchProvider::search (encode.rs:557-602): the winner for the range block
- * (row, col, size) at domain stride `shift`.  k must be 1 (the default
- * search_candidates of encode.rs:572-578 returns the winner only).  *n_out is
- * 0 when no domain position is legal; *evals is the number of fitted
- * candidates the reference counts (encode.rs:768).  Thread-safe, read-only.
+ * (row, col, size) at domain stride `shift`; with k > 1,
+ * ExhaustiveSearch::search_candidates (encode.rs:598-601, search_top_k
+ * 695-788): the best k ascending by rms, ties in enumeration order, into
+ * out[0..k) -- k must not exceed what the context was prepared to rank
+ * (mars_hip_set_ranked).  *n_out = how many (0 when no domain position is
+ * legal); *evals is the number of fitted candidates the reference counts
+ * (encode.rs:768).  Thread-safe, read-only.
  */
 int mars_hip_search(const mars_hip_ctx *ctx, uint32_t row, uint32_t col, uint32_t size,
                     uint32_t shift, uint32_t k, mars_hip_candidate *out, uint32_t *n_out,
@@ -157,6 +160,13 @@ int mars_hip_search_plane(const mars_hip_ctx *ctx, uint32_t plane, uint32_t row,
 int mars_hip_search_method(const mars_hip_ctx *ctx, uint32_t plane, uint32_t method, uint32_t row,
                            uint32_t col, uint32_t size, uint32_t shift, uint32_t k,
                            mars_hip_candidate *out, uint32_t *n_out, uint64_t *evals);
+
+/*
+ * How many candidates per range the next prepare keeps, k = 1..32
+ * (search_top_k's k; default 1).  Ranked tables (k > 1) serve
+ * mars_hip_search with up to k and need the MFMA kernel.
+ */
+int mars_hip_set_ranked(mars_hip_ctx *ctx, uint32_t k);
 
 /* Last error message of this context ("" if none). */
 const char *mars_hip_last_error(const mars_hip_ctx *ctx);
@@ -190,6 +200,11 @@ int mars_hip_table_info(const mars_hip_ctx *ctx, uint32_t i, uint32_t *size, uin
 /* Copy table i's winners (n_ranges entries, found[j] = 0 for no candidate). */
 int mars_hip_table_read(const mars_hip_ctx *ctx, uint32_t i, uint32_t *rows, uint32_t *cols,
                         mars_hip_candidate *out, uint8_t *found);
+
+/* Table i's ranked lists: *k per range, out[r*k + j] (n_ranges * k entries),
+ * counts[r] filled entries of range r. */
+int mars_hip_table_read_ranked(const mars_hip_ctx *ctx, uint32_t i, uint32_t *k,
+                               mars_hip_candidate *out, uint8_t *counts);
 
 /* Table i's winners of one method and each range's evals (evals may be null). */
 int mars_hip_table_read_method(const mars_hip_ctx *ctx, uint32_t i, uint32_t method,

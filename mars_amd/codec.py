@@ -52,8 +52,10 @@ class CColorStats(C.Structure):
 
 COLOR_GRAY, COLOR_RGB_444, COLOR_RGB_420 = 0, 1, 2
 
+# (user, row, col, size, shift, max, out[max], n_out, evals) -> status
 SEARCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-                        C.POINTER(N.CCandidate), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64))
+                        C.c_uint32, C.POINTER(N.CCandidate), C.POINTER(C.c_uint32),
+                        C.POINTER(C.c_uint64))
 
 
 @dataclass(frozen=True)

@@ -89,11 +89,12 @@ struct Table {
     uint32_t P_pad = 0;
     std::vector<uint32_t> iperm; // search slot of each position (empty: identity)
     Partial *d_partial = nullptr;
-    mars_hip_candidate *d_out = nullptr;
-    uint8_t *d_found = nullptr;
+    mars_hip_candidate *d_out = nullptr; // [nr][topk]
+    uint8_t *d_found = nullptr;          // [nr] candidates kept (0..topk)
     double *d_score = nullptr;
-    std::vector<mars_hip_candidate> h_out;
+    std::vector<mars_hip_candidate> h_out; // [nr][topk]
     std::vector<uint8_t> h_found;
+    uint32_t topk = 1;
     // Fisher classified search (method MARS_HIP_METHOD_FISHER only)
     bool fisher = false;
     uint16_t *d_fcls = nullptr, *d_frcls = nullptr;
@@ -115,6 +116,7 @@ struct Group {
     uint32_t size = 0, shift = 0;
     std::vector<uint32_t> tables; // longest searches first
     Slice search[2], merge[2];    // per kernel
+    Slice merge_ranked;           // (table, range) items of the ranked merge
 };
 
 } // namespace
@@ -128,6 +130,8 @@ struct mars_hip_ctx {
     std::string err;
     int kernel = MARS_HIP_KERNEL_MFMA_I8;
     uint32_t method = MARS_HIP_METHOD_EXHAUSTIVE; // of the prepared batch
+    uint32_t ranked = 1;   // candidates per range the next prepare keeps (search_top_k's k)
+    uint32_t topk = 1;     // ... and the prepared batch keeps
     bool prepared = false;
     mars_hip_params params{};
     std::vector<Plane> planes;
@@ -485,6 +489,11 @@ int layout(mars_hip_ctx *ctx)
             }
             end2(g.merge[k]);
         }
+        begin2(g.merge_ranked);
+        if (ctx->topk > 1)
+            for (uint32_t ti : g.tables)
+                for (uint32_t r = 0; r < ctx->tables[ti].nr; r++) w2.push_back(make_uint2(ti, r));
+        end2(g.merge_ranked);
     }
     // every launch must stay below 2^32 threads
     if (w2.size() >= (1u << 24) || w3.size() >= (1u << 24))
@@ -553,6 +562,11 @@ int enqueue_search(mars_hip_ctx *ctx, const Group &g, int kernel, hipStream_t st
 void enqueue_merge(mars_hip_ctx *ctx, const Group &g, int kernel, hipStream_t st)
 {
     const int k = kernel == MARS_HIP_KERNEL_MFMA_I8 ? 0 : 1;
+    if (ctx->topk > 1) {
+        mars::launch_merge_ranked(ctx->d_tabs, ctx->d_work2 + g.merge_ranked.off, g.merge_ranked.n,
+                                  ctx->common, st);
+        return;
+    }
     mars::launch_merge(ctx->d_tabs, ctx->d_work2 + g.merge[k].off, g.merge[k].n, k == 0,
                        ctx->common, st);
 }
@@ -610,10 +624,11 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
 int download(mars_hip_ctx *ctx)
 {
     for (Table &t : ctx->tables) {
-        t.h_out.resize(t.nr);
+        t.h_out.resize((size_t)t.nr * t.topk);
         t.h_found.resize(t.nr);
         if (t.nr == 0) continue;
-        HIP_TRY(ctx, hipMemcpyAsync(t.h_out.data(), t.d_out, sizeof(mars_hip_candidate) * t.nr,
+        HIP_TRY(ctx, hipMemcpyAsync(t.h_out.data(), t.d_out,
+                                    sizeof(mars_hip_candidate) * t.nr * t.topk,
                                     hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(t.h_found.data(), t.d_found, t.nr, hipMemcpyDeviceToHost,
                                     ctx->stream));
@@ -701,9 +716,12 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         if (shifts[i] == 0) return fail(ctx, MARS_HIP_E_ARG, "shift must be > 0");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->ranked > 1 && ctx->kernel != MARS_HIP_KERNEL_MFMA_I8)
+        return fail(ctx, MARS_HIP_E_UNSUPPORTED, "ranked tables need the MFMA search kernel");
     free_batch(ctx);
     ctx->params = *p;
     ctx->method = p->method;
+    ctx->topk = ctx->ranked;
 
     std::vector<uint32_t> sh(shifts, shifts + n_shifts);
     std::sort(sh.begin(), sh.end());
@@ -748,6 +766,7 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
                 if (t.P > (0xffffffffu / 8)) return fail(ctx, MARS_HIP_E_UNSUPPORTED, "too many positions");
                 t.P_pad = (t.P + 31) / 32 * 32;
                 t.fisher = p->method == MARS_HIP_METHOD_FISHER;
+                t.topk = ctx->topk;
                 search_order(t);
                 ctx->table_of[std::make_tuple(pi, t.size, shift)] = (uint32_t)ctx->tables.size();
                 ctx->tables.push_back(std::move(t));
@@ -779,6 +798,8 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
                 Table &t = ctx->tables[ti];
                 uint32_t tpc;
                 chunking(tiles_of(t, k), ng, kMaxTarget, kMinTiles, &t.nch_cap[k], &tpc);
+                // the ranked merge gives each chunk list one lane of a wave
+                if (ctx->topk > 1) t.nch_cap[k] = std::min<uint32_t>(t.nch_cap[k], 64);
             }
         }
 
@@ -809,8 +830,9 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         plan.add(&t.d_iperm, sizeof(uint32_t) * t.iperm.size());
         plan.add(&t.d_posx, 2 * sizeof(int32_t) * (size_t)t.P);
         plan.add(&t.d_posd, 2 * sizeof(double) * (size_t)t.P);
-        plan.add(&t.d_partial, sizeof(Partial) * (size_t)t.nr_pad * std::max(t.nch_cap[0], t.nch_cap[1]));
-        plan.add(&t.d_out, sizeof(mars_hip_candidate) * (size_t)t.nr);
+        plan.add(&t.d_partial, sizeof(Partial) * (size_t)t.nr_pad *
+                                   std::max(t.nch_cap[0], t.nch_cap[1]) * t.topk);
+        plan.add(&t.d_out, sizeof(mars_hip_candidate) * (size_t)t.nr * t.topk);
         plan.add(&t.d_found, (size_t)t.nr);
         plan.add(&t.d_score, sizeof(double) * (size_t)t.nr);
         if (t.fisher) {
@@ -867,6 +889,7 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         c.yscale = ctx->d_yscale;
         c.counters = ctx->d_counters;
         c.sample = 1;
+        c.topk = ctx->topk;
     }
     RC_TRY(layout(ctx));
     RC_TRY(enqueue_step(ctx, ctx->kernel, nullptr));
@@ -881,8 +904,13 @@ int lookup(const mars_hip_ctx *ctx, uint32_t plane, uint32_t method, uint32_t ro
 {
     mars_hip_ctx *mctx = const_cast<mars_hip_ctx *>(ctx);
     if (!ctx || !out || !n_out || !evals) return fail(mctx, MARS_HIP_E_ARG, "null argument");
-    if (k != 1) return fail(mctx, MARS_HIP_E_UNSUPPORTED, "only k == 1 is supported");
+    if (k < 1) return fail(mctx, MARS_HIP_E_ARG, "k must be >= 1");
     if (!ctx->prepared) return fail(mctx, MARS_HIP_E_NOT_PREPARED, "prepare() first");
+    // A ranked request beyond what the table keeps cannot be served; Fisher
+    // ranks nothing (the trait's default search_candidates: the winner).
+    if (method == MARS_HIP_METHOD_EXHAUSTIVE && k > ctx->topk)
+        return fail(mctx, MARS_HIP_E_UNSUPPORTED,
+                    "more ranked candidates than prepared (mars_hip_set_ranked)");
     if (method != MARS_HIP_METHOD_EXHAUSTIVE &&
         !(method == MARS_HIP_METHOD_FISHER && ctx->method == MARS_HIP_METHOD_FISHER))
         return fail(mctx, MARS_HIP_E_ARG, "method not prepared");
@@ -893,12 +921,15 @@ int lookup(const mars_hip_ctx *ctx, uint32_t plane, uint32_t method, uint32_t ro
         if (it != t.index.end()) {
             const uint32_t i = it->second;
             const bool fisher = method == MARS_HIP_METHOD_FISHER;
-            *evals = fisher ? t.h_fevals[i] : table_evals(t);
-            if (fisher ? t.h_ffound[i] : t.h_found[i]) {
-                *out = fisher ? t.h_fout[i] : t.h_out[i];
-                *n_out = 1;
+            if (fisher) {
+                *evals = t.h_fevals[i];
+                *n_out = t.h_ffound[i] ? 1 : 0;
+                if (*n_out) out[0] = t.h_fout[i];
             } else {
-                *n_out = 0;
+                *evals = table_evals(t);
+                const uint32_t n = std::min<uint32_t>(k, t.h_found[i]);
+                for (uint32_t j = 0; j < n; j++) out[j] = t.h_out[(size_t)i * t.topk + j];
+                *n_out = n;
             }
             return MARS_HIP_OK;
         }
@@ -916,10 +947,12 @@ struct PlaneRef {
 };
 
 int plane_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
-                    mars_hip_candidate *out, uint32_t *n_out, uint64_t *evals)
+                    uint32_t max, mars_hip_candidate *out, uint32_t *n_out, uint64_t *evals)
 {
     const PlaneRef *r = (const PlaneRef *)user;
-    return lookup(r->ctx, r->plane, r->method, row, col, size, shift, 1, out, n_out, evals);
+    // Fisher keeps the trait's default search_candidates (the winner only).
+    const uint32_t k = r->method == MARS_HIP_METHOD_FISHER ? 1 : max;
+    return lookup(r->ctx, r->plane, r->method, row, col, size, shift, k, out, n_out, evals);
 }
 
 } // namespace
@@ -981,6 +1014,14 @@ int mars_hip_set_kernel(mars_hip_ctx *ctx, int kernel)
     if (kernel != MARS_HIP_KERNEL_MFMA_I8 && kernel != MARS_HIP_KERNEL_FP32)
         return fail(ctx, MARS_HIP_E_ARG, "unknown kernel");
     ctx->kernel = kernel;
+    return MARS_HIP_OK;
+}
+
+int mars_hip_set_ranked(mars_hip_ctx *ctx, uint32_t k)
+{
+    if (!ctx) return MARS_HIP_E_ARG;
+    if (k < 1 || k > mars::kMaxRanked) return fail(ctx, MARS_HIP_E_ARG, "k must be 1..32");
+    ctx->ranked = k;
     return MARS_HIP_OK;
 }
 
@@ -1149,8 +1190,22 @@ int mars_hip_table_read(const mars_hip_ctx *ctx, uint32_t i, uint32_t *rows, uin
     const Table &t = ctx->tables[i];
     if (rows) memcpy(rows, t.rows.data(), 4 * t.nr);
     if (cols) memcpy(cols, t.cols.data(), 4 * t.nr);
-    if (out) memcpy(out, t.h_out.data(), sizeof(mars_hip_candidate) * t.nr);
-    if (found) memcpy(found, t.h_found.data(), t.nr);
+    for (uint32_t r = 0; r < t.nr; r++) {
+        if (out) out[r] = t.h_out[(size_t)r * t.topk];
+        if (found) found[r] = t.h_found[r] ? 1 : 0;
+    }
+    return MARS_HIP_OK;
+}
+
+int mars_hip_table_read_ranked(const mars_hip_ctx *ctx, uint32_t i, uint32_t *k,
+                               mars_hip_candidate *out, uint8_t *counts)
+{
+    if (!ctx || i >= ctx->tables.size()) return MARS_HIP_E_ARG;
+    if (!ctx->prepared) return MARS_HIP_E_NOT_PREPARED;
+    const Table &t = ctx->tables[i];
+    if (k) *k = t.topk;
+    if (out) memcpy(out, t.h_out.data(), sizeof(mars_hip_candidate) * t.nr * t.topk);
+    if (counts) memcpy(counts, t.h_found.data(), t.nr);
     return MARS_HIP_OK;
 }
 
@@ -1161,8 +1216,10 @@ int mars_hip_table_read_method(const mars_hip_ctx *ctx, uint32_t i, uint32_t met
     if (!ctx->prepared) return MARS_HIP_E_NOT_PREPARED;
     const Table &t = ctx->tables[i];
     if (method == MARS_HIP_METHOD_EXHAUSTIVE) {
-        if (out) memcpy(out, t.h_out.data(), sizeof(mars_hip_candidate) * t.nr);
-        if (found) memcpy(found, t.h_found.data(), t.nr);
+        for (uint32_t r = 0; r < t.nr; r++) {
+            if (out) out[r] = t.h_out[(size_t)r * t.topk];
+            if (found) found[r] = t.h_found[r] ? 1 : 0;
+        }
         if (evals)
             for (uint32_t r = 0; r < t.nr; r++) evals[r] = table_evals(t);
         return MARS_HIP_OK;
@@ -1282,7 +1339,15 @@ int mars_hip_encode_plane_method(mars_hip_ctx *ctx, const uint8_t *px, uint32_t 
     const mars_hip_params hp = search_params(p, method);
     uint32_t shifts[2];
     const uint32_t ns = walk_shifts(p, o, shifts);
-    RC_TRY(mars_hip_prepare(ctx, px, w, h, &hp, shifts, ns));
+    // P5c: the exhaustive provider ranks rd_candidates per node.
+    const uint32_t want = method == MARS_HIP_METHOD_EXHAUSTIVE && p->has_lambda
+                              ? std::max<uint32_t>(1, std::min<uint32_t>(o->rd_candidates, mars::kMaxRanked))
+                              : 1;
+    const uint32_t keep = ctx->ranked;
+    RC_TRY(mars_hip_set_ranked(ctx, want));
+    const int prc = mars_hip_prepare(ctx, px, w, h, &hp, shifts, ns);
+    ctx->ranked = keep;
+    RC_TRY(prc);
     return mars_hip_encode_prepared(ctx, 0, p, o, out, out_len, stats);
 }
 
@@ -1348,7 +1413,12 @@ int mars_hip_encode_rgb(mars_hip_ctx *ctx, const uint8_t *rgb, uint32_t w, uint3
         uint32_t shifts[2];
         const uint32_t ns = walk_shifts(chroma_params, o, shifts);
         const mars_hip_plane planes[2] = {{cb.data(), cw, ch}, {cr.data(), cw, ch}};
+        const uint32_t keep = ctx->ranked;
+        ctx->ranked = method == MARS_HIP_METHOD_EXHAUSTIVE && chroma_params->has_lambda
+                          ? std::max<uint32_t>(1, std::min<uint32_t>(o->rd_candidates, mars::kMaxRanked))
+                          : 1;
         rc = mars_hip_prepare_batch(ctx, planes, 2, &hp, shifts, ns);
+        ctx->ranked = keep;
     }
     if (rc == 0) rc = mars_hip_encode_prepared(ctx, 0, chroma_params, o, &cbs, &cbl, &st.cb);
     if (rc == 0) rc = mars_hip_encode_prepared(ctx, 1, chroma_params, o, &crs, &crl, &st.cr);

@@ -492,22 +492,30 @@ struct Ctx {
     const RateModels *rate;
     bool allowed[4];
     bool adaptive_density;
+    uint32_t rd_candidates; // P5c: ranked candidates the RD walk asks for (<= 1: the winner)
     int err;
 };
 
+// The provider's answer: SearchOutcome (n <= 1) or RankedCandidates
+// (encode.rs:536-555), ascending by rms; c[0] is the winner.
 struct Found {
     bool ok;
-    mars_hip_candidate c;
+    uint32_t n;
+    mars_hip_candidate c[MARS_MAX_RANKED];
     uint64_t evals;
 };
 
-Found do_search(Ctx &ctx, uint32_t row, uint32_t col, uint32_t size, uint32_t shift)
+Found do_search(Ctx &ctx, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
+                uint32_t max = 1)
 {
     Found f{};
     uint32_t n = 0;
-    const int rc = ctx.provider.fn(ctx.provider.user, row, col, size, shift, &f.c, &n, &f.evals);
+    if (max < 1) max = 1;
+    const int rc =
+        ctx.provider.fn(ctx.provider.user, row, col, size, shift, max, f.c, &n, &f.evals);
     if (rc && !ctx.err) ctx.err = rc;
     f.ok = rc == 0 && n > 0;
+    f.n = f.ok ? std::min(n, max) : 0;
     return f;
 }
 
@@ -540,7 +548,7 @@ void walk(Ctx &ctx, uint32_t row, uint32_t col, uint32_t size, std::vector<Leaf>
         return;
     }
     const Found f = do_search(ctx, row, col, size, ctx.params->shift);
-    const double best_rms = f.ok ? f.c.rms : INFINITY;
+    const double best_rms = f.ok ? f.c[0].rms : INFINITY;
     if (best_rms > ctx.params->t_rms && size > h.min_size) {
         const uint32_t half = size / 2;
         walk(ctx, row, col, half, leaves, evals);
@@ -553,11 +561,11 @@ void walk(Ctx &ctx, uint32_t row, uint32_t col, uint32_t size, std::vector<Leaf>
     Leaf leaf = dc_leaf(row, col, size, 0);
     if (f.ok) {
         leaf.mode = 2;
-        leaf.qalfa = f.c.qalfa;
-        leaf.qbeta = f.c.qbeta;
-        leaf.isometry = (uint8_t)f.c.isometry;
-        leaf.dom_row = f.c.dom_row;
-        leaf.dom_col = f.c.dom_col;
+        leaf.qalfa = f.c[0].qalfa;
+        leaf.qbeta = f.c[0].qbeta;
+        leaf.isometry = (uint8_t)f.c[0].isometry;
+        leaf.dom_row = f.c[0].dom_row;
+        leaf.dom_col = f.c[0].dom_col;
     }
     if (leaf.qalfa <= ctx.params->zero_threshold) { // section 8.1 zero-alfa override
         leaf.qbeta = best_beta(block_sum(ctx, row, col, size), (int64_t)size * size, h.bits_beta);
@@ -687,7 +695,8 @@ struct Scored {
     double d, r;
 };
 
-// encode.rs:1536-1714 best_mode_leaf (ranked = the provider's winner, if any).
+// encode.rs:1536-1714 best_mode_leaf (ranked = the provider's ascending list:
+// the winner, or up to rd_candidates of them).
 Scored best_mode_leaf(const Ctx &ctx, uint32_t row, uint32_t col, uint32_t size, const Found &f,
                       double lambda, uint32_t size_class)
 {
@@ -727,8 +736,11 @@ Scored best_mode_leaf(const Ctx &ctx, uint32_t row, uint32_t col, uint32_t size,
         const double r = event_bits(rate, leaf_events(h, leaf, size_class));
         cands.push_back({std::move(leaf), sse, r});
     }
-    if (f.ok && f.c.qalfa >= 1) {
-        const mars_hip_candidate &c = f.c;
+    // encode.rs:1627-1674: modes 2 and 3 for every ranked candidate, in order,
+    // skipping those whose fit collapsed to qalfa 0.
+    for (uint32_t ci = 0; ci < f.n; ci++) {
+        const mars_hip_candidate &c = f.c[ci];
+        if (c.qalfa < 1) continue;
         if (ctx.allowed[2]) {
             Leaf leaf = dc_leaf(row, col, size, c.qbeta);
             leaf.mode = 2;
@@ -835,7 +847,8 @@ RdResult walk_rd(Ctx &ctx, uint32_t row, uint32_t col, uint32_t size, double lam
     const double node_lambda = lambda * 1.0;
     uint32_t shift = ctx.params->shift;
     if (ctx.adaptive_density && block_rms(ctx, row, col, size) <= 4.0) shift *= 2;
-    const Found f = do_search(ctx, row, col, size, shift);
+    // encode.rs:1946-1975: the winner, or the provider's ranked list (P5c).
+    const Found f = do_search(ctx, row, col, size, shift, ctx.rd_candidates <= 1 ? 1 : ctx.rd_candidates);
     Scored leaf = best_mode_leaf(ctx, row, col, size, f, node_lambda, size_class);
     if (size <= h.min_size) {
         RdResult out;
@@ -879,7 +892,7 @@ int mars_encode_plane(const uint8_t *px, uint32_t w, uint32_t h, const mars_enco
 {
     if (!px || !p || !o || !search || !out || !out_len || w == 0 || h == 0)
         return MARS_HIP_E_ARG;
-    if (o->rd_candidates > 1) return MARS_HIP_E_UNSUPPORTED;
+    if (o->rd_candidates > MARS_MAX_RANKED) return MARS_HIP_E_UNSUPPORTED;
     if (p->has_lambda && !(std::isfinite(p->lambda) && p->lambda >= 0.0)) return MARS_HIP_E_ARG;
     if (p->has_lambda && !warmup) return MARS_HIP_E_ARG;
     *out = nullptr;
@@ -915,6 +928,7 @@ int mars_encode_plane(const uint8_t *px, uint32_t w, uint32_t h, const mars_enco
         Ctx wctx = ctx;
         wctx.params = &wp;
         wctx.provider = {warmup, warmup_user};
+        wctx.rd_candidates = 1; // encode.rs:1160
         wctx.allowed[0] = wctx.allowed[1] = wctx.allowed[2] = wctx.allowed[3] = true;
         std::vector<Leaf> wl;
         uint64_t we = 0;
@@ -930,6 +944,7 @@ int mars_encode_plane(const uint8_t *px, uint32_t w, uint32_t h, const mars_enco
 
         ctx.provider = {search, search_user};
         ctx.rate = &rate;
+        ctx.rd_candidates = o->rd_candidates;
         for (int i = 0; i < 4; i++) ctx.allowed[i] = o->allowed_modes[i] != 0;
         ctx.adaptive_density = o->adaptive_density != 0;
         RdResult res = walk_rd(ctx, 0, 0, hdr.virtual_size(), P.lambda);

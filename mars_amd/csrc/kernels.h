@@ -16,6 +16,9 @@
 
 namespace mars {
 
+// Most candidates a ranked table keeps per range (search_top_k's k).
+constexpr uint32_t kMaxRanked = 32;
+
 // Per-(range, position chunk) partial winner.
 struct Partial {
     double rms;
@@ -57,9 +60,9 @@ struct TableDesc {
     const uint32_t *iperm;
     int32_t *posx;       // [P][2] S1, S2
     double *posd;        // [P][2] Di = n*S2 - S1^2, 1/Di (binary64, stage 2)
-    Partial *partial;    // [nr_pad][nchunks]
-    mars_hip_candidate *out;
-    uint8_t *found;
+    Partial *partial;    // [nr_pad][nchunks][topk]
+    mars_hip_candidate *out; // [nr][topk]
+    uint8_t *found;          // [nr] candidates found (0..topk)
     double *score;       // [nr] the search kernel's own score of the winner (FP32: binary32 rms)
     uint32_t size, shift, P, P_pad, npc, nr, nr_pad, kmem;
     // Position chunking of the search: [0] MFMA kernel (32-position
@@ -88,6 +91,7 @@ struct Common {
     const double *yscale; // [2^bits_alfa] (2^bits_beta-1) / ((1 + alfa2) * 255)
     unsigned long long *counters; // [16]: [log2 s] stage-1 survivors, [8 + log2 s] stage-3 fits
     uint32_t sample;              // MFMA search: 1, or the tile stride of a threshold pre-pass
+    uint32_t topk;                // candidates kept per range: 1, or a ranked table's k (<= 32)
 };
 
 // Work items: x = descriptor index, y/z = kernel-specific block coordinates.
@@ -108,6 +112,9 @@ int launch_search(int kernel, uint32_t size, const TableDesc *t, const Work3 *wo
                   uint32_t nwork, const Common &c, hipStream_t st);
 void launch_merge(const TableDesc *t, const Work2 *work, uint32_t nwork, bool t1_exact,
                   const Common &c, hipStream_t st);
+// Ranked tables (c.topk > 1, MFMA search only): work item (table, range).
+void launch_merge_ranked(const TableDesc *t, const Work2 *work, uint32_t nwork, const Common &c,
+                         hipStream_t st);
 // Fisher classified search: classification of every domain position
 // (dom_work: table, block of 256 positions) and range (rng_work: table,
 // block of 4 ranges), the buckets (tab_work: table, 0), then every range's

@@ -202,6 +202,7 @@ __global__ void k_seed(const TableDesc *__restrict__ tabs, const Work2 *__restri
 {
     __shared__ uint32_t s_min;
     __shared__ int32_t s_t1[256];
+    __shared__ uint32_t s_tau[64];
     const Work2 wk = work[blockIdx.x];
     const TableG T = view(tabs[wk.x]);
     const uint32_t size = T.size, K = size * size, P = T.P;
@@ -234,7 +235,42 @@ __global__ void k_seed(const TableDesc *__restrict__ tabs, const Work2 *__restri
         fit_f64(n, S1, S2, T.rstat[2 * r], s_t1[c], T.rstat[2 * r + 1],
                 cm.max_alfa, cm.bits_alfa, cm.bits_beta, &qa, &qb, &rms, &sum);
         double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
-        atomicMin(&s_min, __float_as_uint((float)(tau * (1.0 + 1e-7))));
+        const uint32_t tb = __float_as_uint((float)(tau * (1.0 + 1e-7)));
+        if (cm.topk <= 1) {
+            atomicMin(&s_min, tb);
+        } else {
+            // Ranked: only distinct samples count (positions repeat when
+            // P < npos).
+            s_tau[c] = (P >= npos || j < P) ? tb : 0x7f800000u;
+        }
+    }
+    __syncthreads();
+    if (cm.topk > 1 && t == 0) {
+        // The topk-th smallest sampled threshold (+inf with fewer distinct
+        // samples): topk real candidates reach it, so it bounds the range's
+        // topk-th best.
+        uint32_t th = 0x7f800000u, prev = 0;
+        uint32_t taken = 0;
+        for (uint32_t i = 0; i < cm.topk && i < ncand; i++) {
+            // next smallest value above `prev`, counting repeats
+            uint32_t m = 0x7f800000u, cnt = 0;
+            for (uint32_t e = 0; e < ncand; e++) {
+                const uint32_t x = s_tau[e];
+                if ((i == 0 || x > prev) && x < m) {
+                    m = x;
+                    cnt = 0;
+                }
+                if (x == m) cnt++;
+            }
+            if (m == 0x7f800000u) break;
+            taken += cnt;
+            prev = m;
+            if (taken >= cm.topk) {
+                th = m;
+                break;
+            }
+        }
+        s_min = th;
     }
     __syncthreads();
     if (t == 0) T.gthr[r] = s_min;
@@ -382,6 +418,16 @@ struct BestRec {
     int32_t t1; // exact t1_x4 of the record
 };
 
+// One entry of a ranked (top-k) list: the candidate, its exact t1 and its
+// inflated threshold (tau_bits of its SSE).
+struct TopRec {
+    double rms;
+    uint32_t key;
+    int32_t t1;
+    uint32_t tau;
+    uint32_t pad;
+};
+
 // Inflated threshold of a candidate whose binary64 SSE (`sum` of
 // encode.rs:283-285) is `sum`: anything whose exact SSE exceeds it has a
 // strictly larger binary64 rms (relative 1e-9 plus absolute 1e-6*n cover the
@@ -478,7 +524,7 @@ template <int S> struct Queue {
 // verbatim binary64 fit of encode.rs:247-289, and compete with the wave's
 // best by (rms, key).  Winners tighten tau through LDS; blocks exchange tau
 // through a global array every 16 tiles.  Tau starts from k_seed's sample.
-template <int S>
+template <int S, bool RANKED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 : S <= 8 ? 3 : 2))) void k_search_mfma(const TableDesc *__restrict__ tabs,
                                                      const Work3 *__restrict__ work, Common c)
 {
@@ -502,6 +548,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     __shared__ double s_A[8];
     __shared__ BestRec s_wb[4][8];
     __shared__ uint2 s_q[4][Queue<S>::CAP];
+    // RANKED: each wave's best c.topk candidates per range, ascending by
+    // (rms, key), and how many are filled.
+    __shared__ TopRec s_top[RANKED ? 4 : 1][RANKED ? 8 : 1][RANKED ? kMaxRanked : 1];
+    __shared__ uint32_t s_topn[4][8];
     // Range operand of the block (sizes >= 16): 64 rows of KMEM bytes,
     // padded rows so the lanes of a fragment read spread over the banks.
     constexpr int ASTRIDE = G::KMEM + 16;
@@ -521,6 +571,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         s_wb[wave][lane].rms = __longlong_as_double(0x7ff0000000000000ll);
         s_wb[wave][lane].key = 0xffffffffu;
         s_wb[wave][lane].t1 = 0;
+        s_topn[wave][lane] = 0;
     }
 
     // Range statistics of the lane's 4 ranges.
@@ -626,7 +677,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                 uint32_t qa, qb;
                 fit_f64(n, S1, S2, t0, t1, t2, c.max_alfa, c.bits_alfa, c.bits_beta, &qa, &qb,
                         &rms, &sum);
-                win = better(rms, key, wb[slot].rms, wb[slot].key);
+                if constexpr (RANKED) {
+                    const uint32_t kk = c.topk;
+                    const TopRec &last = s_top[wave][slot][kk - 1];
+                    win = s_topn[wave][slot] < kk || better(rms, key, last.rms, last.key);
+                } else {
+                    win = better(rms, key, wb[slot].rms, wb[slot].key);
+                }
             }
         }
         stage3 += keep ? 1 : 0;
@@ -638,7 +695,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
             const double r_l = __shfl(rms, l), s_l = __shfl(sum, l);
             const uint32_t k_l = __shfl(key, l), sl_l = __shfl(slot, l);
             const int32_t t_l = __shfl(t1, l);
-            if (lane == 0 && better(r_l, k_l, wb[sl_l].rms, wb[sl_l].key)) {
+            if constexpr (RANKED) {
+                // retain_top_k (encode.rs:676-685) on the wave's list; once
+                // it holds c.topk entries its last one bounds the range.
+                if (lane == 0) {
+                    TopRec *L = s_top[wave][sl_l];
+                    const uint32_t kk = c.topk;
+                    uint32_t nn = s_topn[wave][sl_l];
+                    if (nn < kk || better(r_l, k_l, L[kk - 1].rms, L[kk - 1].key)) {
+                        uint32_t pos = nn < kk ? nn : kk - 1;
+                        while (pos > 0 && better(r_l, k_l, L[pos - 1].rms, L[pos - 1].key)) {
+                            L[pos] = L[pos - 1];
+                            pos--;
+                        }
+                        L[pos] = TopRec{r_l, k_l, t_l, tau_bits(s_l, n)};
+                        if (nn < kk) s_topn[wave][sl_l] = ++nn;
+                        if (nn == kk) atomicMin(&s_thr[sl_l], L[kk - 1].tau);
+                    }
+                }
+            } else if (lane == 0 && better(r_l, k_l, wb[sl_l].rms, wb[sl_l].key)) {
                 wb[sl_l].rms = r_l;
                 wb[sl_l].key = k_l;
                 wb[sl_l].t1 = t_l;
@@ -893,14 +968,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         if (r < T.nr) {
             // The block's tightest threshold for later blocks and passes.
             atomicMin((uint32_t *)&T.gthr[r], lds_load(&s_thr[sl]));
-            BestRec b = s_wb[0][sl];
-            for (int wv = 1; wv < 4; wv++) {
-                const BestRec &e = s_wb[wv][sl];
-                if (better(e.rms, e.key, b.rms, b.key)) b = e;
+            if constexpr (RANKED) {
+                // The four waves' lists merged: the chunk's best c.topk,
+                // padded with empty records (key ~0, rms +inf).
+                if (sample == 1) {
+                    const uint32_t kk = c.topk;
+                    uint32_t hd[4] = {0, 0, 0, 0};
+                    const size_t base = ((size_t)r * nchunks + chunk) * kk;
+                    for (uint32_t i = 0; i < kk; i++) {
+                        int bw = -1;
+                        for (int wv = 0; wv < 4; wv++) {
+                            if (hd[wv] >= s_topn[wv][sl]) continue;
+                            const TopRec &e = s_top[wv][sl][hd[wv]];
+                            if (bw < 0 || better(e.rms, e.key, s_top[bw][sl][hd[bw]].rms,
+                                                 s_top[bw][sl][hd[bw]].key))
+                                bw = wv;
+                        }
+                        if (bw < 0) {
+                            st_partial(T.partial, base + i, __longlong_as_double(0x7ff0000000000000ll),
+                                       0xffffffffu, 0);
+                        } else {
+                            const TopRec &e = s_top[bw][sl][hd[bw]++];
+                            st_partial(T.partial, base + i, e.rms, e.key, e.t1);
+                        }
+                    }
+                }
+            } else {
+                BestRec b = s_wb[0][sl];
+                for (int wv = 1; wv < 4; wv++) {
+                    const BestRec &e = s_wb[wv][sl];
+                    if (better(e.rms, e.key, b.rms, b.key)) b = e;
+                }
+                // The pre-pass only tightens thresholds; the full pass writes
+                // every partial.
+                if (sample == 1)
+                    st_partial(T.partial, (size_t)r * nchunks + chunk, b.rms, b.key, b.t1);
             }
-            // The pre-pass only tightens thresholds; the full pass writes
-            // every partial.
-            if (sample == 1) st_partial(T.partial, (size_t)r * nchunks + chunk, b.rms, b.key, b.t1);
         }
     }
 }
@@ -1012,8 +1115,8 @@ __global__ __launch_bounds__(256) void k_search_fp32(const TableDesc *__restrict
 
 // ---------------------------------------------------------------- merge
 
-__device__ __forceinline__ void write_winner(const TableG &T, const Common &cm, uint32_t r,
-                                             uint32_t k, int64_t t1)
+__device__ __forceinline__ mars_hip_candidate winner_record(const TableG &T, const Common &cm,
+                                                            uint32_t r, uint32_t k, int64_t t1)
 {
     mars_hip_candidate c = {};
     const uint32_t p = k >> 3, iso = k & 7, S = T.size;
@@ -1034,8 +1137,64 @@ __device__ __forceinline__ void write_winner(const TableG &T, const Common &cm, 
     c.qalfa = qa;
     c.qbeta = qb;
     c.rms = rms;
-    st_cand(T.out, r, c);
+    return c;
+}
+
+__device__ __forceinline__ void write_winner(const TableG &T, const Common &cm, uint32_t r,
+                                             uint32_t k, int64_t t1)
+{
+    st_cand(T.out, r, winner_record(T, cm, r, k, t1));
     T.found[r] = 1;
+}
+
+// Ranked merge, work item (table, range), one wave: the range's chunk lists
+// (each ascending by (rms, key), c.topk long, padded with key ~0) merged into
+// its best c.topk -- lane j holds chunk j's head (at most 64 chunks) and the
+// wave takes the smallest head c.topk times -- each written with its exact
+// RawMoments and fit to out[r * topk + i]; found[r] = how many exist.
+__global__ __launch_bounds__(64) void k_merge_ranked(const TableDesc *__restrict__ tabs,
+                                                     const Work2 *__restrict__ work, Common cm)
+{
+    const Work2 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t r = wk.y, lane = threadIdx.x, kk = cm.topk;
+    const uint32_t nchunks = T.nchunks[0];
+    const bool mine = lane < nchunks && T.P > 0;
+    const size_t base = ((size_t)r * nchunks + lane) * kk;
+    uint32_t head = 0;
+    uint32_t nfound = 0;
+    for (uint32_t i = 0; i < kk; i++) {
+        double rms = __longlong_as_double(0x7ff0000000000000ll);
+        uint32_t key = 0xffffffffu;
+        int32_t t1 = 0;
+        if (mine && head < kk) {
+            const Partial e = ld_partial(T.partial, base + head);
+            rms = e.rms;
+            key = e.key;
+            t1 = e.t1;
+        }
+        uint32_t who = lane;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double orms = __shfl_xor(rms, off, 64);
+            const uint32_t okey = __shfl_xor(key, off, 64), owho = __shfl_xor(who, off, 64);
+            const int32_t ot1 = __shfl_xor(t1, off, 64);
+            if (better(orms, okey, rms, key)) {
+                rms = orms;
+                key = okey;
+                who = owho;
+                t1 = ot1;
+            }
+        }
+        if (key == 0xffffffffu) break; // every list is exhausted
+        if (lane == who) head++;
+        if (lane == 0) st_cand(T.out, (size_t)r * kk + i, winner_record(T, cm, r, key, t1));
+        nfound++;
+    }
+    if (lane == 0) {
+        T.found[r] = (uint8_t)nfound;
+        for (uint32_t i = nfound; i < kk; i++) st_cand(T.out, (size_t)r * kk + i, mars_hip_candidate{});
+    }
 }
 
 // Work item (table, block of 128 ranges), one thread per range, for partials
@@ -1171,11 +1330,11 @@ __global__ void k_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, ui
 
 // ---------------------------------------------------------------- launchers
 
-template <int S>
+template <int S, bool RANKED>
 static void launch_mfma(const TableDesc *t, const Work3 *work, uint32_t n, const Common &c,
                         hipStream_t st)
 {
-    hipLaunchKernelGGL(k_search_mfma<S>, dim3(n), dim3(256), 0, st, t, work, c);
+    hipLaunchKernelGGL((k_search_mfma<S, RANKED>), dim3(n), dim3(256), 0, st, t, work, c);
 }
 template <int S>
 static void launch_fp32(const TableDesc *t, const Work3 *work, uint32_t n, const Common &c,
@@ -1188,15 +1347,23 @@ int launch_search(int kernel, uint32_t size, const TableDesc *t, const Work3 *wo
                   uint32_t n, const Common &c, hipStream_t st)
 {
     if (n == 0) return 0;
-    if (kernel == 0) {
+    if (kernel == 0 && c.topk > 1) {
         switch (size) {
-        case 2: launch_mfma<2>(t, work, n, c, st); return 0;
-        case 4: launch_mfma<4>(t, work, n, c, st); return 0;
-        case 8: launch_mfma<8>(t, work, n, c, st); return 0;
-        case 16: launch_mfma<16>(t, work, n, c, st); return 0;
-        case 32: launch_mfma<32>(t, work, n, c, st); return 0;
+        case 2: launch_mfma<2, true>(t, work, n, c, st); return 0;
+        case 4: launch_mfma<4, true>(t, work, n, c, st); return 0;
+        case 8: launch_mfma<8, true>(t, work, n, c, st); return 0;
+        case 16: launch_mfma<16, true>(t, work, n, c, st); return 0;
+        case 32: launch_mfma<32, true>(t, work, n, c, st); return 0;
         }
-    } else {
+    } else if (kernel == 0) {
+        switch (size) {
+        case 2: launch_mfma<2, false>(t, work, n, c, st); return 0;
+        case 4: launch_mfma<4, false>(t, work, n, c, st); return 0;
+        case 8: launch_mfma<8, false>(t, work, n, c, st); return 0;
+        case 16: launch_mfma<16, false>(t, work, n, c, st); return 0;
+        case 32: launch_mfma<32, false>(t, work, n, c, st); return 0;
+        }
+    } else if (c.topk <= 1) {
         switch (size) {
         case 2: launch_fp32<2>(t, work, n, c, st); return 0;
         case 4: launch_fp32<4>(t, work, n, c, st); return 0;
@@ -1241,6 +1408,12 @@ void launch_ranges(const TableDesc *t, const Work2 *work, uint32_t n, bool fp32,
 void launch_rstats(const TableDesc *t, const Work2 *work, uint32_t n, hipStream_t st)
 {
     if (n) hipLaunchKernelGGL(k_rstats, dim3(n), dim3(256), 0, st, t, work);
+}
+
+void launch_merge_ranked(const TableDesc *t, const Work2 *work, uint32_t n, const Common &c,
+                         hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_merge_ranked, dim3(n), dim3(64), 0, st, t, work, c);
 }
 
 void launch_merge(const TableDesc *t, const Work2 *work, uint32_t n, bool t1_exact,

@@ -111,6 +111,10 @@ class HipContext:
     def set_kernel(self, kernel: int) -> None:
         self._check(self._L.mars_hip_set_kernel(self._ctx, kernel), "mars_hip_set_kernel")
 
+    def set_ranked(self, k: int) -> None:
+        """Candidates per range the next prepare keeps (search_top_k's k, 1..32)."""
+        self._check(self._L.mars_hip_set_ranked(self._ctx, k), "mars_hip_set_ranked")
+
     def prepare(self, image: np.ndarray, params: EncodeParams, shifts: Sequence[int],
                 method: int = N.METHOD_EXHAUSTIVE) -> None:
         img = np.ascontiguousarray(image, dtype=np.uint8)
@@ -158,6 +162,31 @@ class HipContext:
         rc = self._L.mars_hip_search_plane(self._ctx, plane, row, col, size, shift, 1,
                                            C.byref(out), C.byref(n), C.byref(ev))
         return rc, (out if n.value else None), ev.value
+
+    def search_ranked(self, row: int, col: int, size: int, shift: int, k: int, plane: int = 0):
+        """(status, up to k candidates ascending, evals): search_candidates."""
+        out = (N.CCandidate * k)()
+        n = C.c_uint32()
+        ev = C.c_uint64()
+        rc = self._L.mars_hip_search_plane(self._ctx, plane, row, col, size, shift, k, out,
+                                           C.byref(n), C.byref(ev))
+        return rc, [out[i] for i in range(n.value)], ev.value
+
+    def ranked_tables(self, plane: int = 0):
+        """{(size, shift): (rows, cols, [list of up to k candidates per range])}."""
+        res = {}
+        for s, sh, rows, cols, nr, _, i in self._tables(plane):
+            k = C.c_uint32()
+            self._check(self._L.mars_hip_table_read_ranked(self._ctx, i, C.byref(k), None, None),
+                        "table_read_ranked")
+            kk = k.value
+            out = (N.CCandidate * max(nr * kk, 1))()
+            cnt = (C.c_uint8 * max(nr, 1))()
+            self._check(self._L.mars_hip_table_read_ranked(self._ctx, i, C.byref(k), out, cnt),
+                        "table_read_ranked")
+            res[(s, sh)] = (rows, cols, [[out[r * kk + j] for j in range(cnt[r])]
+                                         for r in range(nr)])
+        return res
 
     def search_method(self, method: int, row: int, col: int, size: int, shift: int,
                       plane: int = 0):

@@ -382,9 +382,12 @@ void or_fisher_provider_free(or_fisher_provider *p)
     free(p);
 }
 
-int or_fisher_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift, void *out,
-                 uint32_t *n_out, uint64_t *evals)
+/* IndexedSearchProvider keeps the trait's default search_candidates
+ * (encode.rs:572-578): the winner alone, whatever `max` is. */
+int or_fisher_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
+                 uint32_t max, void *out, uint32_t *n_out, uint64_t *evals)
 {
+    (void)max;
     or_fisher_provider *p = (or_fisher_provider *)user;
     const or_provider *b = p->base;
     or_fisher_index *ix = NULL;

@@ -90,12 +90,19 @@ uint64_t or_search_table(const uint8_t *px, uint32_t w, uint32_t h, uint32_t siz
                          uint32_t shift, double max_alfa, uint32_t bits_alfa,
                          uint32_t bits_beta, const uint32_t *rows, const uint32_t *cols,
                          uint32_t n, int threads, or_candidate *out, uint8_t *found);
+/* The same with the best k per range (search_top_k): out holds n*k records,
+ * found[i] = how many of range i's k slots are filled. */
+uint64_t or_search_table_k(const uint8_t *px, uint32_t w, uint32_t h, uint32_t size,
+                           uint32_t shift, double max_alfa, uint32_t bits_alfa,
+                           uint32_t bits_beta, const uint32_t *rows, const uint32_t *cols,
+                           uint32_t n, int threads, uint32_t k, or_candidate *out,
+                           uint8_t *found);
 
 /*
  * The exhaustive search as a search-provider callback (the reference's
  * ExhaustiveSearch, encode.rs:582-602), for driving the encoder chain in
- * tests.  `out` receives an or_candidate (the same layout as the product's
- * candidate record).
+ * tests.  `out` receives up to `max` or_candidate records (the same layout
+ * as the product's candidate record), ascending.
  */
 typedef struct {
     const uint8_t *px;
@@ -109,7 +116,7 @@ or_provider *or_provider_new(const uint8_t *px, uint32_t w, uint32_t h, double m
                              uint32_t bits_alfa, uint32_t bits_beta);
 void or_provider_free(or_provider *p);
 int or_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
-                 void *out, uint32_t *n_out, uint64_t *evals);
+                 uint32_t max, void *out, uint32_t *n_out, uint64_t *evals);
 
 /* ---- Fisher classified search and colour plumbing (fisher.c) ---- */
 
@@ -160,8 +167,8 @@ or_fisher_provider *or_fisher_provider_new(const uint8_t *px, uint32_t w, uint32
                                            double max_alfa, uint32_t bits_alfa,
                                            uint32_t bits_beta);
 void or_fisher_provider_free(or_fisher_provider *p);
-int or_fisher_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift, void *out,
-                 uint32_t *n_out, uint64_t *evals);
+int or_fisher_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
+                 uint32_t max, void *out, uint32_t *n_out, uint64_t *evals);
 
 /* metrics.rs:377-417: interleaved RGB (n pixels) -> Y, Cb, Cr planes. */
 void or_ycbcr(const uint8_t *rgb, size_t n, uint8_t *y, uint8_t *cb, uint8_t *cr);

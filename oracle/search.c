@@ -258,12 +258,12 @@ void or_search_top_k(const uint8_t *px, uint32_t w, uint32_t h, const int32_t *c
 typedef struct {
     const uint8_t *px;
     const int32_t *contracted;
-    uint32_t w, h, size, shift, bits_alfa, bits_beta;
+    uint32_t w, h, size, shift, bits_alfa, bits_beta, k;
     double max_alfa;
     const uint32_t *rows, *cols;
     uint32_t lo, hi;
-    or_candidate *out;
-    uint8_t *found;
+    or_candidate *out; /* k per range */
+    uint8_t *found;    /* candidates kept per range */
     uint64_t evals;
 } table_job;
 
@@ -275,8 +275,9 @@ static void *table_worker(void *arg)
         uint32_t n;
         uint64_t e;
         or_search_top_k(j->px, j->w, j->h, j->contracted, j->rows[i], j->cols[i], j->size, j->shift,
-                        j->max_alfa, j->bits_alfa, j->bits_beta, 1, &j->out[i], &n, &e);
-        j->found[i] = (uint8_t)(n > 0);
+                        j->max_alfa, j->bits_alfa, j->bits_beta, j->k, &j->out[(size_t)i * j->k],
+                        &n, &e);
+        j->found[i] = (uint8_t)n;
         j->evals += e;
     }
     return NULL;
@@ -287,6 +288,17 @@ uint64_t or_search_table(const uint8_t *px, uint32_t w, uint32_t h, uint32_t siz
                          uint32_t bits_beta, const uint32_t *rows, const uint32_t *cols,
                          uint32_t n, int threads, or_candidate *out, uint8_t *found)
 {
+    return or_search_table_k(px, w, h, size, shift, max_alfa, bits_alfa, bits_beta, rows, cols,
+                             n, threads, 1, out, found);
+}
+
+uint64_t or_search_table_k(const uint8_t *px, uint32_t w, uint32_t h, uint32_t size,
+                           uint32_t shift, double max_alfa, uint32_t bits_alfa,
+                           uint32_t bits_beta, const uint32_t *rows, const uint32_t *cols,
+                           uint32_t n, int threads, uint32_t k, or_candidate *out,
+                           uint8_t *found)
+{
+    if (k == 0) k = 1;
     int32_t *contracted = (int32_t *)malloc(sizeof(int32_t) * ((size_t)(w / 2) * (h / 2) + 1));
     or_contract(px, w, h, contracted);
     if (threads < 1) threads = 1;
@@ -303,6 +315,7 @@ uint64_t or_search_table(const uint8_t *px, uint32_t w, uint32_t h, uint32_t siz
         j->shift = shift;
         j->bits_alfa = bits_alfa;
         j->bits_beta = bits_beta;
+        j->k = k;
         j->max_alfa = max_alfa;
         j->rows = rows;
         j->cols = cols;
@@ -349,11 +362,12 @@ void or_provider_free(or_provider *p)
     free(p);
 }
 
+/* ExhaustiveSearch::search_candidates (encode.rs:598-601): the best `max`. */
 int or_search_cb(void *user, uint32_t row, uint32_t col, uint32_t size, uint32_t shift,
-                 void *out, uint32_t *n_out, uint64_t *evals)
+                 uint32_t max, void *out, uint32_t *n_out, uint64_t *evals)
 {
     const or_provider *p = (const or_provider *)user;
     or_search_top_k(p->px, p->w, p->h, p->contracted, row, col, size, shift, p->max_alfa,
-                    p->bits_alfa, p->bits_beta, 1, (or_candidate *)out, n_out, evals);
+                    p->bits_alfa, p->bits_beta, max ? max : 1, (or_candidate *)out, n_out, evals);
     return 0;
 }

@@ -77,6 +77,9 @@ def lib() -> C.CDLL:
     L.or_search_table.argtypes = [P(C.c_uint8), u32, u32, u32, u32, dbl, u32, u32, P(u32),
                                   P(u32), u32, C.c_int, P(OrCandidate), P(C.c_uint8)]
     L.or_search_table.restype = u64
+    L.or_search_table_k.argtypes = [P(C.c_uint8), u32, u32, u32, u32, dbl, u32, u32, P(u32),
+                                    P(u32), u32, C.c_int, u32, P(OrCandidate), P(C.c_uint8)]
+    L.or_search_table_k.restype = u64
     L.or_provider_new.argtypes = [P(C.c_uint8), u32, u32, dbl, u32, u32]
     L.or_provider_new.restype = C.c_void_p
     L.or_provider_free.argtypes = [C.c_void_p]
@@ -189,6 +192,21 @@ def search_table(img, size, shift, rows, cols, threads=8, max_alfa=1.0, bits_alf
     ev = lib().or_search_table(img.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, size, shift,
                                max_alfa, bits_alfa, bits_beta, r, c, n, threads, out, found)
     return [out[i] if found[i] else None for i in range(n)], ev
+
+
+def search_table_k(img, size, shift, rows, cols, k, threads=8, max_alfa=1.0, bits_alfa=4,
+                   bits_beta=7):
+    """Each range's best k (search_top_k): a list of candidate lists, and evals."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape
+    n = len(rows)
+    r = (C.c_uint32 * max(n, 1))(*rows)
+    c = (C.c_uint32 * max(n, 1))(*cols)
+    out = (OrCandidate * max(n * k, 1))()
+    found = (C.c_uint8 * max(n, 1))()
+    ev = lib().or_search_table_k(img.ctypes.data_as(C.POINTER(C.c_uint8)), w, h, size, shift,
+                                 max_alfa, bits_alfa, bits_beta, r, c, n, threads, k, out, found)
+    return [[out[i * k + j] for j in range(found[i])] for i in range(n)], ev
 
 
 def nodes(w, h, min_size, max_size):

@@ -9,9 +9,13 @@ from tests import _oracle as O
 
 
 class TableProvider:
-    """Oracle winners for every searchable node, served as a mars_search_fn."""
+    """Oracle results for every searchable node, served as a mars_search_fn.
 
-    def __init__(self, img, params: EncodeParams, shifts):
+    With k > 1 the table holds each node's best k (search_top_k) and the
+    callback returns the first `max` of them.
+    """
+
+    def __init__(self, img, params: EncodeParams, shifts, k: int = 1):
         h, w = img.shape
         self.table = {}
         nodes = O.nodes(w, h, params.min_size, params.max_size)
@@ -19,25 +23,25 @@ class TableProvider:
             for size, rc in nodes.items():
                 rows = [r for r, _ in rc]
                 cols = [c for _, c in rc]
-                res, _ = O.search_table(img, size, shift, rows, cols, threads=16,
-                                        max_alfa=params.max_alfa, bits_alfa=params.bits_alfa,
-                                        bits_beta=params.bits_beta)
+                res, _ = O.search_table_k(img, size, shift, rows, cols, k, threads=16,
+                                          max_alfa=params.max_alfa,
+                                          bits_alfa=params.bits_alfa,
+                                          bits_beta=params.bits_beta)
                 P = 0
                 if h >= 2 * size and w >= 2 * size:
                     P = ((h - 2 * size) // shift + 1) * ((w - 2 * size) // shift + 1)
-                for r, c, cand in zip(rows, cols, res):
-                    self.table[(r, c, size, shift)] = (cand, P * 8)
+                for r, c, cands in zip(rows, cols, res):
+                    self.table[(r, c, size, shift)] = (cands, P * 8)
         self.calls = 0
 
-        def cb(user, row, col, size, shift, out, n_out, evals):
+        def cb(user, row, col, size, shift, mx, out, n_out, evals):
             self.calls += 1
-            cand, ev = self.table[(row, col, size, shift)]
+            cands, ev = self.table[(row, col, size, shift)]
             evals[0] = ev
-            if cand is None:
-                n_out[0] = 0
-            else:
-                C.memmove(out, C.byref(cand), C.sizeof(N.CCandidate))
-                n_out[0] = 1
+            take = cands[:max(1, mx)]
+            for i, cand in enumerate(take):
+                C.memmove(C.byref(out[i]), C.byref(cand), C.sizeof(N.CCandidate))
+            n_out[0] = len(take)
             return 0
 
         self._cb = SEARCH_FN(cb)
