@@ -173,6 +173,38 @@ def _colour_leg(steps: int):
                                                      f"in {cpu_s:.2f} s"}}
 
 
+def _ranked_leg(k: int, steps: int, warmup: int) -> int:
+    """The quadtree512 step with ranked tables (each range's best k,
+    search_top_k), against the same step at k = 1; one JSON line."""
+    from mars_amd import synth
+    from mars_amd.provider import EncodeParams, HipContext
+
+    img = synth.plane(W, H, SEED)
+    params = EncodeParams(min_size=MIN_SIZE, max_size=MAX_SIZE, shift=SHIFT)
+    res = {}
+    for kk in sorted({1, k}):
+        ctx = HipContext(0)
+        ctx.set_ranked(kk)
+        ctx.prepare(img, params, [SHIFT])
+        ctx.autotune(reps=1)
+        ctx.run(iters=max(warmup, 1), use_graph=True)
+        br = [ctx.run(iters=1, use_graph=False) for _ in range(3)]
+        t = ctx.run(iters=steps, use_graph=True)
+        ctx.close()
+        res[kk] = {"ms_per_step": t.ms_step, "evals_per_s": t.evals / (t.ms_step * 1e-3),
+                   "search_by_size_ms": {str(2 << i): float(np.mean([b.ms_by_size[i] for b in br]))
+                                         for i in range(6) if br[0].ms_by_size[i] > 0},
+                   "survivors_by_size": {str(2 << i): int(br[0].survivors_by_size[i])
+                                         for i in range(6) if br[0].ms_by_size[i] > 0},
+                   "exact_fits_by_size": {str(2 << i): int(br[0].exact_fits_by_size[i])
+                                          for i in range(6) if br[0].ms_by_size[i] > 0}}
+    print(json.dumps({"metric": f"quadtree512 step with ranked tables, k={k} vs k=1",
+                      "unit": "evals/s", "ranked": {str(kk): v for kk, v in res.items()},
+                      "slowdown_vs_k1": res[k]["ms_per_step"] / res[1]["ms_per_step"]}),
+          flush=True)
+    return 0
+
+
 def _cpu_leg(img, budget_s: float = 12.0):
     """The oracle (C restatement of the reference CPU search) on the host cores,
     on a bounded sample of the same workload: ranges drawn from every size."""
@@ -323,6 +355,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-image legs")
     ap.add_argument("--no-colour", action="store_true", help="skip the RGB 4:2:0 Fisher leg")
+    ap.add_argument("--ranked", type=int, default=0,
+                    help="time the quadtree512 step with ranked tables of this k (1..32) only")
     ap.add_argument("--colour-only", action="store_true",
                     help="only the RGB 4:2:0 Fisher leg (one JSON line of its own)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -336,6 +370,8 @@ def main() -> int:
         return _traffic_probe()
     if args.precision:
         return _precision(args.precision_sample)
+    if args.ranked:
+        return _ranked_leg(args.ranked, args.steps, args.warmup)
     if args.colour_only:
         print(json.dumps({"metric": "RGB 4:2:0 Fisher encode, device step and Fisher evals/s",
                           "colour": _colour_leg(args.steps)}), flush=True)

@@ -696,21 +696,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
             const uint32_t k_l = __shfl(key, l), sl_l = __shfl(slot, l);
             const int32_t t_l = __shfl(t1, l);
             if constexpr (RANKED) {
-                // retain_top_k (encode.rs:676-685) on the wave's list; once
-                // it holds c.topk entries its last one bounds the range.
-                if (lane == 0) {
-                    TopRec *L = s_top[wave][sl_l];
-                    const uint32_t kk = c.topk;
-                    uint32_t nn = s_topn[wave][sl_l];
-                    if (nn < kk || better(r_l, k_l, L[kk - 1].rms, L[kk - 1].key)) {
-                        uint32_t pos = nn < kk ? nn : kk - 1;
-                        while (pos > 0 && better(r_l, k_l, L[pos - 1].rms, L[pos - 1].key)) {
-                            L[pos] = L[pos - 1];
-                            pos--;
-                        }
+                // retain_top_k (encode.rs:676-685) on the wave's list, one
+                // entry per lane: the entries ahead of the new one are
+                // counted by a ballot and those behind it move down one slot
+                // in parallel.  Once the list holds c.topk entries its last
+                // one bounds the range.
+                TopRec *L = s_top[wave][sl_l];
+                const uint32_t kk = c.topk;
+                const uint32_t nn = lds_load(&s_topn[wave][sl_l]);
+                TopRec e = {};
+                bool ahead = false;
+                if ((uint32_t)lane < nn) {
+                    e = L[lane];
+                    ahead = !better(r_l, k_l, e.rms, e.key);
+                }
+                const uint32_t pos = (uint32_t)__popcll(__ballot(ahead));
+                if (pos < kk) {
+                    if ((uint32_t)lane >= pos && (uint32_t)lane < nn && (uint32_t)lane + 1 < kk)
+                        L[lane + 1] = e;
+                    if (lane == 0) {
                         L[pos] = TopRec{r_l, k_l, t_l, tau_bits(s_l, n)};
-                        if (nn < kk) s_topn[wave][sl_l] = ++nn;
-                        if (nn == kk) atomicMin(&s_thr[sl_l], L[kk - 1].tau);
+                        const uint32_t nn2 = nn < kk ? nn + 1 : kk;
+                        s_topn[wave][sl_l] = nn2;
+                        if (nn2 == kk) atomicMin(&s_thr[sl_l], L[kk - 1].tau);
                     }
                 }
             } else if (lane == 0 && better(r_l, k_l, wb[sl_l].rms, wb[sl_l].key)) {
