@@ -524,10 +524,13 @@ template <int S> struct Queue {
 // verbatim binary64 fit of encode.rs:247-289, and compete with the wave's
 // best by (rms, key).  Winners tighten tau through LDS; blocks exchange tau
 // through a global array every 16 tiles.  Tau starts from k_seed's sample.
-template <int S, bool RANKED>
+// KCAP: 0 for the winner-only search, else the capacity of the ranked lists
+// (>= c.topk; a smaller list keeps more blocks resident).
+template <int S, int KCAP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 : S <= 8 ? 3 : 2))) void k_search_mfma(const TableDesc *__restrict__ tabs,
                                                      const Work3 *__restrict__ work, Common c)
 {
+    constexpr bool RANKED = KCAP > 0;
     using G = Geo<S>;
     constexpr int NK = G::NK;
     constexpr int64_t n = (int64_t)S * S;
@@ -550,7 +553,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     __shared__ uint2 s_q[4][Queue<S>::CAP];
     // RANKED: each wave's best c.topk candidates per range, ascending by
     // (rms, key), and how many are filled.
-    __shared__ TopRec s_top[RANKED ? 4 : 1][RANKED ? 8 : 1][RANKED ? kMaxRanked : 1];
+    __shared__ TopRec s_top[RANKED ? 4 : 1][RANKED ? 8 : 1][RANKED ? KCAP : 1];
     __shared__ uint32_t s_topn[4][8];
     // Range operand of the block (sizes >= 16): 64 rows of KMEM bytes,
     // padded rows so the lanes of a fragment read spread over the banks.
@@ -1338,11 +1341,11 @@ __global__ void k_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, ui
 
 // ---------------------------------------------------------------- launchers
 
-template <int S, bool RANKED>
+template <int S, int KCAP>
 static void launch_mfma(const TableDesc *t, const Work3 *work, uint32_t n, const Common &c,
                         hipStream_t st)
 {
-    hipLaunchKernelGGL((k_search_mfma<S, RANKED>), dim3(n), dim3(256), 0, st, t, work, c);
+    hipLaunchKernelGGL((k_search_mfma<S, KCAP>), dim3(n), dim3(256), 0, st, t, work, c);
 }
 template <int S>
 static void launch_fp32(const TableDesc *t, const Work3 *work, uint32_t n, const Common &c,
@@ -1355,21 +1358,29 @@ int launch_search(int kernel, uint32_t size, const TableDesc *t, const Work3 *wo
                   uint32_t n, const Common &c, hipStream_t st)
 {
     if (n == 0) return 0;
-    if (kernel == 0 && c.topk > 1) {
+    if (kernel == 0 && c.topk > 8) {
         switch (size) {
-        case 2: launch_mfma<2, true>(t, work, n, c, st); return 0;
-        case 4: launch_mfma<4, true>(t, work, n, c, st); return 0;
-        case 8: launch_mfma<8, true>(t, work, n, c, st); return 0;
-        case 16: launch_mfma<16, true>(t, work, n, c, st); return 0;
-        case 32: launch_mfma<32, true>(t, work, n, c, st); return 0;
+        case 2: launch_mfma<2, 32>(t, work, n, c, st); return 0;
+        case 4: launch_mfma<4, 32>(t, work, n, c, st); return 0;
+        case 8: launch_mfma<8, 32>(t, work, n, c, st); return 0;
+        case 16: launch_mfma<16, 32>(t, work, n, c, st); return 0;
+        case 32: launch_mfma<32, 32>(t, work, n, c, st); return 0;
+        }
+    } else if (kernel == 0 && c.topk > 1) {
+        switch (size) {
+        case 2: launch_mfma<2, 8>(t, work, n, c, st); return 0;
+        case 4: launch_mfma<4, 8>(t, work, n, c, st); return 0;
+        case 8: launch_mfma<8, 8>(t, work, n, c, st); return 0;
+        case 16: launch_mfma<16, 8>(t, work, n, c, st); return 0;
+        case 32: launch_mfma<32, 8>(t, work, n, c, st); return 0;
         }
     } else if (kernel == 0) {
         switch (size) {
-        case 2: launch_mfma<2, false>(t, work, n, c, st); return 0;
-        case 4: launch_mfma<4, false>(t, work, n, c, st); return 0;
-        case 8: launch_mfma<8, false>(t, work, n, c, st); return 0;
-        case 16: launch_mfma<16, false>(t, work, n, c, st); return 0;
-        case 32: launch_mfma<32, false>(t, work, n, c, st); return 0;
+        case 2: launch_mfma<2, 0>(t, work, n, c, st); return 0;
+        case 4: launch_mfma<4, 0>(t, work, n, c, st); return 0;
+        case 8: launch_mfma<8, 0>(t, work, n, c, st); return 0;
+        case 16: launch_mfma<16, 0>(t, work, n, c, st); return 0;
+        case 32: launch_mfma<32, 0>(t, work, n, c, st); return 0;
         }
     } else if (c.topk <= 1) {
         switch (size) {
