@@ -3,6 +3,7 @@
 import ctypes as C
 
 from mars_amd import _native as N
+from mars_amd import oracle_cache as OC
 from mars_amd.codec import SEARCH_FN
 from mars_amd.provider import EncodeParams
 from tests import _oracle as O
@@ -46,3 +47,22 @@ class TableProvider:
 
         self._cb = SEARCH_FN(cb)
         self.fn = C.cast(self._cb, C.c_void_p)
+
+
+def oracle_cache(img, config):
+    """The top-32 oracle cache assembled from the oracle's search_top_k (k = 32)."""
+    h, w = img.shape
+    memo = {}
+
+    def lookup(row, col, size):
+        if size not in memo:
+            grid = [(r, c) for r in range(0, h - size + 1, size)
+                    for c in range(0, w - size + 1, size)]
+            lists, _ = O.search_table_k(img, size, config.shift, [r for r, _ in grid],
+                                        [c for _, c in grid], OC.SLOTS, threads=16,
+                                        max_alfa=config.max_alfa, bits_alfa=config.bits_alfa,
+                                        bits_beta=config.bits_beta)
+            memo[size] = dict(zip(grid, lists))
+        return memo[size][(row, col)]
+
+    return OC.assemble(img, "synthetic", config, lookup)
