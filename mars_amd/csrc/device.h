@@ -38,6 +38,18 @@ __device__ __forceinline__ uint32_t lds_load(const uint32_t *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// lane % 32, computed afresh at each call: the asm is volatile, so the
+// compiler can neither hoist it out of a loop nor keep an older copy live.
+__device__ __forceinline__ uint32_t lane_col()
+{
+    uint32_t l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\t"
+                 "v_mbcnt_hi_u32_b32 %0, -1, %0\n\t"
+                 "v_and_b32 %0, 31, %0"
+                 : "=v"(l));
+    return l;
+}
+
 // The first three words of a record: a 12-byte load, so no destination
 // register of an in-flight load is dead (the compiler would reuse it and
 // wait for the load).

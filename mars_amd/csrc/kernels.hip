@@ -741,7 +741,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         long l8[NK], h8[NK];
         v4i l16[NK], h16[NK];
     };
-    auto load_pin = [&](uint32_t tile) { return ld_u4(T.posinfo, tile * sample * 32 + jj); };
+    // The column (lane % 32) is recomputed at every use: kept live across the
+    // tile loop it is the value the register allocator spills, and the
+    // reload's wait would also wait for the prefetches issued before it.
+    auto load_pin = [&](uint32_t tile) { return ld_u4(T.posinfo, tile * sample * 32 + lane_col()); };
     auto fetch_b = [&](const uint4 pin, Frag &f) {
         f.pin = pin;
         if constexpr (G::PRELOAD_A) {
@@ -789,9 +792,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         uint32_t thb[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) thb[i] = lds_load(&s_thr[4 * h + i]);
-        // (cur: this tile; nxt: filled for tile + 4 below)
-        if (tile + 4 < t_end) fetch_b(pin_ahead, nxt);
-        if (tile + 8 < t_end) pin_ahead = load_pin(tile + 8);
+        // (cur: this tile; nxt: filled for tile + 4).  Sizes with register
+        // A fragments issue these loads after the first MFMAs: every load in
+        // flight at an MFMA is then one the MFMA needs, so the compiler's
+        // in-order wait count cannot make it wait for the prefetch too.
+        auto prefetch = [&]() __attribute__((always_inline)) {
+            if (tile + 4 < t_end) fetch_b(pin_ahead, nxt);
+            if (tile + 8 < t_end) pin_ahead = load_pin(tile + 8);
+        };
+        if constexpr (!G::PRELOAD_A) prefetch();
         const uint32_t p = cur.pin.w; // position index, ~0 for padding
         const uint32_t boff = cur.pin.x;
         const int32_t S1p = (int32_t)cur.pin.y;
@@ -919,6 +928,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                         al = __builtin_amdgcn_mfma_i32_32x32x32_i8(a16[tt][kk], cur.l16[kk], al, 0, 0, 0);
                     }
                 }
+                if (tt == 0) prefetch();
                 epilogue(ah, al, tt);
             }
         } else {
