@@ -193,8 +193,8 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
     T.posd[2 * p + 1] = di > 0 ? 1.0 / (double)di : 0.0;
 }
 
-// Threshold seed, work item (table, range): a 256-thread block fits 64
-// spread candidates (8 positions x 8 isometries; 32 = 4 x 8 at S >= 32), each
+// Threshold seed of sizes >= 16, work item (table, range): a 256-thread
+// block fits 64 spread candidates (8 positions x 8 isometries; 32 = 4 x 8 at S >= 32), each
 // candidate's correlation split over 256/candidates threads.  The smallest
 // inflated SSE becomes the range's starting threshold.
 __global__ void k_seed(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work,
@@ -436,6 +436,103 @@ __device__ __forceinline__ uint32_t tau_bits(double sum, int64_t n)
 {
     double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
     return __float_as_uint((float)(tau * (1.0 + 1e-7)));
+}
+
+// Threshold seed of sizes <= 8, work item (table, first of four ranges):
+// wave w seeds range 4 y + w with the same 64 candidates as k_seed (8 spread
+// positions x 8 isometries), one per lane: the lane's whole correlation
+// with every load in flight at once, then its binary64 fit.  The smallest
+// inflated SSE (ranked tables: the topk-th smallest over distinct
+// candidates) becomes the starting threshold -- the value k_seed computes,
+// without a block per range or a barrier.
+template <int S>
+__device__ __forceinline__ void seed_wave(const TableG &T, const uint32_t r, const Common &cm,
+                                          uint32_t *s_tau)
+{
+    constexpr int64_t n = (int64_t)S * S;
+    const uint32_t lane = threadIdx.x & 63, P = T.P;
+    const uint32_t j = lane >> 3, iso = lane & 7;
+    const uint32_t p = P >= 8 ? (uint32_t)(((uint64_t)j * (P - 1)) / 7) : j % P;
+    const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
+    const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
+    const GLB int8_t *rk = T.rng8 + ((size_t)r * 8 + iso) * T.kmem; // 16-byte aligned rows
+    const GLB int32_t *dw = T.d + (size_t)dr * T.wc + dc;
+    int32_t rv[S * S], dv[S * S];
+    if constexpr (S == 2) {
+        const uint32_t w = *(const GLB uint32_t *)rk;
+#pragma unroll
+        for (int k = 0; k < 4; k++) rv[k] = (int32_t)(int8_t)(w >> (8 * k));
+    } else {
+#pragma unroll
+        for (int k = 0; k < S * S; k += 16) {
+            const v4i w = *(const GLB v4i *)(rk + k);
+#pragma unroll
+            for (int e = 0; e < 16; e++) rv[k + e] = (int32_t)(int8_t)((uint32_t)w[e >> 2] >> (8 * (e & 3)));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < S; u++)
+#pragma unroll
+        for (int v = 0; v < S; v++) dv[u * S + v] = dw[(size_t)u * T.wc + v];
+    int32_t t1 = 0;
+#pragma unroll
+    for (int k = 0; k < S * S; k++) t1 += (rv[k] + 128) * dv[k];
+    uint32_t qa, qb;
+    double rms, sum;
+    fit_f64(n, T.posx[2 * p], T.posx[2 * p + 1], T.rstat[2 * r], t1, T.rstat[2 * r + 1],
+            cm.max_alfa, cm.bits_alfa, cm.bits_beta, &qa, &qb, &rms, &sum);
+    const uint32_t tb = tau_bits(sum, n);
+    if (cm.topk <= 1) {
+        uint32_t m = tb;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, off));
+        if (lane == 0) T.gthr[r] = m;
+        return;
+    }
+    // Ranked: only distinct samples count (positions repeat when P < 8).
+    s_tau[lane] = (P >= 8 || j < P) ? tb : 0x7f800000u;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    if (lane == 0) {
+        // The topk-th smallest sampled threshold (+inf with fewer distinct
+        // samples): topk real candidates reach it.
+        uint32_t th = 0x7f800000u, prev = 0, taken = 0;
+        for (uint32_t i = 0; i < cm.topk && i < 64; i++) {
+            uint32_t m = 0x7f800000u, cnt = 0;
+            for (uint32_t e = 0; e < 64; e++) {
+                const uint32_t x = s_tau[e];
+                if ((i == 0 || x > prev) && x < m) {
+                    m = x;
+                    cnt = 0;
+                }
+                if (x == m) cnt++;
+            }
+            if (m == 0x7f800000u) break;
+            taken += cnt;
+            prev = m;
+            if (taken >= cm.topk) {
+                th = m;
+                break;
+            }
+        }
+        T.gthr[r] = th;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_seed4(const TableDesc *__restrict__ tabs,
+                                               const Work2 *__restrict__ work, Common cm)
+{
+    __shared__ uint32_t s_tau[4][64];
+    const Work2 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t wave = threadIdx.x >> 6, r = wk.y * 4 + wave;
+    if (r >= T.nr) return; // the whole wave: no barrier below is block-wide
+    if (T.size == 8)
+        seed_wave<8>(T, r, cm, s_tau[wave]);
+    else if (T.size == 4)
+        seed_wave<4>(T, r, cm, s_tau[wave]);
+    else
+        seed_wave<2>(T, r, cm, s_tau[wave]);
 }
 
 // Stage 2 (see k_search_mfma): true when the candidate may beat or tie the
@@ -1421,6 +1518,12 @@ void launch_sat(const PlaneDesc *planes, const Work2 *rows_work, uint32_t n_rows
 void launch_possums(const TableDesc *t, const Work2 *work, uint32_t n, hipStream_t st)
 {
     if (n) hipLaunchKernelGGL(k_possums, dim3(n), dim3(256), 0, st, t, work);
+}
+
+void launch_seed4(const TableDesc *t, const Work2 *work, uint32_t n, const Common &c,
+                  hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_seed4, dim3(n), dim3(256), 0, st, t, work, c);
 }
 
 void launch_seed(const TableDesc *t, const Work2 *work, uint32_t n, const Common &c,
