@@ -146,7 +146,7 @@ struct mars_hip_ctx {
     Work2 *d_work2 = nullptr;
     Work3 *d_work3 = nullptr;
     size_t work2_cap = 0, work3_cap = 0;
-    Slice wl_contract, wl_sat_rows, wl_sat_cols, wl_possums, wl_ranges, wl_rstats, wl_seed, wl_seed4;
+    Slice wl_contract, wl_sat_rows, wl_sat_cols, wl_possums, wl_ranges, wl_rstats, wl_seed;
     Slice wl_fdom, wl_frng, wl_ftab, wl_frange; // Fisher: positions, range blocks, tables, ranges
     unsigned long long *d_counters = nullptr;
     double *d_yscale = nullptr;
@@ -439,17 +439,10 @@ int layout(mars_hip_ctx *ctx)
     begin2(ctx->wl_seed);
     for (uint32_t i = 0; i < ctx->tables.size(); i++) {
         const Table &t = ctx->tables[i];
-        if (t.P == 0 || t.size <= 8) continue;
-        for (uint32_t r = 0; r < t.nr; r++) w2.push_back(make_uint2(i, r));
-    }
-    end2(ctx->wl_seed);
-    begin2(ctx->wl_seed4);
-    for (uint32_t i = 0; i < ctx->tables.size(); i++) {
-        const Table &t = ctx->tables[i];
-        if (t.P == 0 || t.size > 8) continue;
+        if (t.P == 0) continue;
         for (uint32_t b = 0; b < cdiv(t.nr, 4); b++) w2.push_back(make_uint2(i, b));
     }
-    end2(ctx->wl_seed4);
+    end2(ctx->wl_seed);
     // Fisher (tables with an empty pool keep their zeroed results)
     begin2(ctx->wl_fdom);
     for (uint32_t i = 0; i < ctx->tables.size(); i++) {
@@ -538,10 +531,8 @@ void enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st)
     mars::launch_ranges(ctx->d_tabs, w2 + ctx->wl_ranges.off, ctx->wl_ranges.n,
                         kernel == MARS_HIP_KERNEL_FP32, st);
     mars::launch_rstats(ctx->d_tabs, w2 + ctx->wl_rstats.off, ctx->wl_rstats.n, st);
-    if (kernel == MARS_HIP_KERNEL_MFMA_I8) {
+    if (kernel == MARS_HIP_KERNEL_MFMA_I8)
         mars::launch_seed(ctx->d_tabs, w2 + ctx->wl_seed.off, ctx->wl_seed.n, ctx->common, st);
-        mars::launch_seed4(ctx->d_tabs, w2 + ctx->wl_seed4.off, ctx->wl_seed4.n, ctx->common, st);
-    }
 }
 
 // Sizes whose MFMA search first runs a threshold pre-pass over every

@@ -106,11 +106,9 @@ void launch_possums(const TableDesc *t, const Work2 *work, uint32_t nwork, hipSt
 void launch_ranges(const TableDesc *t, const Work2 *work, uint32_t nwork, bool fp32,
                    hipStream_t st);
 void launch_rstats(const TableDesc *t, const Work2 *work, uint32_t nwork, hipStream_t st);
+// Work item (table, first of four ranges).
 void launch_seed(const TableDesc *t, const Work2 *work, uint32_t nwork, const Common &c,
                  hipStream_t st);
-// Sizes <= 8: work item (table, first of four ranges).
-void launch_seed4(const TableDesc *t, const Work2 *work, uint32_t nwork, const Common &c,
-                  hipStream_t st);
 int launch_search(int kernel, uint32_t size, const TableDesc *t, const Work3 *work,
                   uint32_t nwork, const Common &c, hipStream_t st);
 void launch_merge(const TableDesc *t, const Work2 *work, uint32_t nwork, bool t1_exact,

@@ -193,89 +193,6 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
     T.posd[2 * p + 1] = di > 0 ? 1.0 / (double)di : 0.0;
 }
 
-// Threshold seed of sizes >= 16, work item (table, range): a 256-thread
-// block fits 64 spread candidates (8 positions x 8 isometries; 32 = 4 x 8 at S >= 32), each
-// candidate's correlation split over 256/candidates threads.  The smallest
-// inflated SSE becomes the range's starting threshold.
-__global__ void k_seed(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work,
-                       Common cm)
-{
-    __shared__ uint32_t s_min;
-    __shared__ int32_t s_t1[256];
-    __shared__ uint32_t s_tau[64];
-    const Work2 wk = work[blockIdx.x];
-    const TableG T = view(tabs[wk.x]);
-    const uint32_t size = T.size, K = size * size, P = T.P;
-    const uint32_t ncand = size >= 32 ? 32 : 64, q = 256 / ncand, npos = ncand / 8;
-    const uint32_t r = wk.y, t = threadIdx.x, c = t % ncand, sl = t / ncand;
-    if (t == 0) s_min = 0x7f800000u;
-    s_t1[t] = 0;
-    __syncthreads();
-    const uint32_t j = c >> 3, iso = c & 7;
-    const uint32_t p = P >= npos ? (uint32_t)(((uint64_t)j * (P - 1)) / (npos - 1)) : j % P;
-    const int32_t S1 = T.posx[2 * p], S2 = T.posx[2 * p + 1];
-    const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
-    const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
-    const GLB int8_t *rk = T.rng8 + ((size_t)r * 8 + iso) * T.kmem;
-    const GLB int32_t *d = T.d;
-    const uint32_t wc = T.wc;
-    const uint32_t k0 = sl * K / q, k1 = (sl + 1) * K / q;
-    int32_t t1 = 0;
-#pragma unroll 8
-    for (uint32_t k = k0; k < k1; k++) {
-        uint32_t u = k / size, v = k - u * size;
-        t1 += ((int32_t)rk[k] + 128) * d[(dr + u) * wc + dc + v];
-    }
-    atomicAdd(&s_t1[c], t1);
-    __syncthreads();
-    if (t < ncand) {
-        const int64_t n = (int64_t)K;
-        uint32_t qa, qb;
-        double rms, sum;
-        fit_f64(n, S1, S2, T.rstat[2 * r], s_t1[c], T.rstat[2 * r + 1],
-                cm.max_alfa, cm.bits_alfa, cm.bits_beta, &qa, &qb, &rms, &sum);
-        double tau = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)n;
-        const uint32_t tb = __float_as_uint((float)(tau * (1.0 + 1e-7)));
-        if (cm.topk <= 1) {
-            atomicMin(&s_min, tb);
-        } else {
-            // Ranked: only distinct samples count (positions repeat when
-            // P < npos).
-            s_tau[c] = (P >= npos || j < P) ? tb : 0x7f800000u;
-        }
-    }
-    __syncthreads();
-    if (cm.topk > 1 && t == 0) {
-        // The topk-th smallest sampled threshold (+inf with fewer distinct
-        // samples): topk real candidates reach it, so it bounds the range's
-        // topk-th best.
-        uint32_t th = 0x7f800000u, prev = 0;
-        uint32_t taken = 0;
-        for (uint32_t i = 0; i < cm.topk && i < ncand; i++) {
-            // next smallest value above `prev`, counting repeats
-            uint32_t m = 0x7f800000u, cnt = 0;
-            for (uint32_t e = 0; e < ncand; e++) {
-                const uint32_t x = s_tau[e];
-                if ((i == 0 || x > prev) && x < m) {
-                    m = x;
-                    cnt = 0;
-                }
-                if (x == m) cnt++;
-            }
-            if (m == 0x7f800000u) break;
-            taken += cnt;
-            prev = m;
-            if (taken >= cm.topk) {
-                th = m;
-                break;
-            }
-        }
-        s_min = th;
-    }
-    __syncthreads();
-    if (t == 0) T.gthr[r] = s_min;
-}
-
 // Range operands, work item (table, block): one thread per 4 bytes of
 // rng8, layout [range][iso][kmem] (i8 r-128, zero padded to kmem); with
 // `fp32` also rngf, layout [range/4][k][iso*4 + range%4] (f32 r).
@@ -438,13 +355,48 @@ __device__ __forceinline__ uint32_t tau_bits(double sum, int64_t n)
     return __float_as_uint((float)(tau * (1.0 + 1e-7)));
 }
 
-// Threshold seed of sizes <= 8, work item (table, first of four ranges):
-// wave w seeds range 4 y + w with the same 64 candidates as k_seed (8 spread
-// positions x 8 isometries), one per lane: the lane's whole correlation
-// with every load in flight at once, then its binary64 fit.  The smallest
-// inflated SSE (ranked tables: the topk-th smallest over distinct
-// candidates) becomes the starting threshold -- the value k_seed computes,
-// without a block per range or a barrier.
+// Correlation t1_x4 of one candidate from its range row rk (i8 r - 128,
+// 16-byte aligned) and its window dw of the contracted plane (row stride wc):
+// sizes <= 8 issue every load at once, larger sizes one window row at a time.
+template <int S>
+__device__ __forceinline__ int32_t seed_t1(const GLB int8_t *rk, const GLB int32_t *dw, uint32_t wc)
+{
+    constexpr int RS = S <= 8 ? S * S : S; // range bytes per load batch
+    constexpr int UN = S <= 8 ? 1 : 2;
+    int32_t t1 = 0;
+#pragma unroll UN
+    for (int u0 = 0; u0 < (S <= 8 ? 1 : S); u0++) {
+        int32_t rv[RS], dv[RS];
+        const GLB int8_t *rr = rk + u0 * RS;
+        if constexpr (S == 2) {
+            const uint32_t w = *(const GLB uint32_t *)rr;
+#pragma unroll
+            for (int k = 0; k < 4; k++) rv[k] = (int32_t)(int8_t)(w >> (8 * k));
+        } else {
+#pragma unroll
+            for (int k = 0; k < RS; k += 16) {
+                const v4i w = *(const GLB v4i *)(rr + k);
+#pragma unroll
+                for (int e = 0; e < 16; e++) rv[k + e] = (int32_t)(int8_t)((uint32_t)w[e >> 2] >> (8 * (e & 3)));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RS; k++) {
+            const int u = S <= 8 ? k / S : u0, v = S <= 8 ? k % S : k;
+            dv[k] = dw[(size_t)u * wc + v];
+        }
+#pragma unroll
+        for (int k = 0; k < RS; k++) t1 += (rv[k] + 128) * dv[k];
+    }
+    return t1;
+}
+
+// Threshold seed, work item (table, first of four ranges): wave w seeds
+// range 4 y + w from 64 spread candidates (8 positions x 8 isometries), one
+// per lane -- the lane's correlation with its loads in flight together,
+// then its binary64 fit.  The smallest inflated SSE (ranked tables: the
+// topk-th smallest over distinct candidates) becomes the range's starting
+// threshold.  No block-wide barrier: a wave per range.
 template <int S>
 __device__ __forceinline__ void seed_wave(const TableG &T, const uint32_t r, const Common &cm,
                                           uint32_t *s_tau)
@@ -456,27 +408,7 @@ __device__ __forceinline__ void seed_wave(const TableG &T, const uint32_t r, con
     const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
     const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
     const GLB int8_t *rk = T.rng8 + ((size_t)r * 8 + iso) * T.kmem; // 16-byte aligned rows
-    const GLB int32_t *dw = T.d + (size_t)dr * T.wc + dc;
-    int32_t rv[S * S], dv[S * S];
-    if constexpr (S == 2) {
-        const uint32_t w = *(const GLB uint32_t *)rk;
-#pragma unroll
-        for (int k = 0; k < 4; k++) rv[k] = (int32_t)(int8_t)(w >> (8 * k));
-    } else {
-#pragma unroll
-        for (int k = 0; k < S * S; k += 16) {
-            const v4i w = *(const GLB v4i *)(rk + k);
-#pragma unroll
-            for (int e = 0; e < 16; e++) rv[k + e] = (int32_t)(int8_t)((uint32_t)w[e >> 2] >> (8 * (e & 3)));
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < S; u++)
-#pragma unroll
-        for (int v = 0; v < S; v++) dv[u * S + v] = dw[(size_t)u * T.wc + v];
-    int32_t t1 = 0;
-#pragma unroll
-    for (int k = 0; k < S * S; k++) t1 += (rv[k] + 128) * dv[k];
+    const int32_t t1 = seed_t1<S>(rk, T.d + (size_t)dr * T.wc + dc, T.wc);
     uint32_t qa, qb;
     double rms, sum;
     fit_f64(n, T.posx[2 * p], T.posx[2 * p + 1], T.rstat[2 * r], t1, T.rstat[2 * r + 1],
@@ -519,20 +451,21 @@ __device__ __forceinline__ void seed_wave(const TableG &T, const uint32_t r, con
     }
 }
 
-__global__ __launch_bounds__(256) void k_seed4(const TableDesc *__restrict__ tabs,
-                                               const Work2 *__restrict__ work, Common cm)
+__global__ __launch_bounds__(256) void k_seed(const TableDesc *__restrict__ tabs,
+                                              const Work2 *__restrict__ work, Common cm)
 {
     __shared__ uint32_t s_tau[4][64];
     const Work2 wk = work[blockIdx.x];
     const TableG T = view(tabs[wk.x]);
     const uint32_t wave = threadIdx.x >> 6, r = wk.y * 4 + wave;
     if (r >= T.nr) return; // the whole wave: no barrier below is block-wide
-    if (T.size == 8)
-        seed_wave<8>(T, r, cm, s_tau[wave]);
-    else if (T.size == 4)
-        seed_wave<4>(T, r, cm, s_tau[wave]);
-    else
-        seed_wave<2>(T, r, cm, s_tau[wave]);
+    switch (T.size) {
+    case 2: seed_wave<2>(T, r, cm, s_tau[wave]); break;
+    case 4: seed_wave<4>(T, r, cm, s_tau[wave]); break;
+    case 8: seed_wave<8>(T, r, cm, s_tau[wave]); break;
+    case 16: seed_wave<16>(T, r, cm, s_tau[wave]); break;
+    default: seed_wave<32>(T, r, cm, s_tau[wave]); break;
+    }
 }
 
 // Stage 2 (see k_search_mfma): true when the candidate may beat or tie the
@@ -635,7 +568,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     constexpr int LOG2N = 2 * LOG2S;
     constexpr bool INT_LIMIT = S <= 8; // every stage-1 quantity fits 32 bits
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int h = lane >> 5, jj = lane & 31;
+    const int h = lane >> 5;
     const Work3 wk = work[blockIdx.x];
     const TableG T = view(tabs[wk.x]);
     const uint32_t chunk = wk.y, g = wk.z;
@@ -1518,12 +1451,6 @@ void launch_sat(const PlaneDesc *planes, const Work2 *rows_work, uint32_t n_rows
 void launch_possums(const TableDesc *t, const Work2 *work, uint32_t n, hipStream_t st)
 {
     if (n) hipLaunchKernelGGL(k_possums, dim3(n), dim3(256), 0, st, t, work);
-}
-
-void launch_seed4(const TableDesc *t, const Work2 *work, uint32_t n, const Common &c,
-                  hipStream_t st)
-{
-    if (n) hipLaunchKernelGGL(k_seed4, dim3(n), dim3(256), 0, st, t, work, c);
 }
 
 void launch_seed(const TableDesc *t, const Work2 *work, uint32_t n, const Common &c,
