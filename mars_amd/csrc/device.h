@@ -292,20 +292,19 @@ __device__ __forceinline__ double sel_d(const double (&v)[4], int i)
     return i == 0 ? v[0] : i == 1 ? v[1] : i == 2 ? v[2] : v[3];
 }
 
-// isometry.rs:7-19
+// isometry.rs:7-19: the source pixel (i, j) of cell (u, v) under isometry
+// k = 0..7, i.e. (u, v), (s-1-v, u), (v, s-1-u), (s-1-u, s-1-v),
+// (u, s-1-v), (s-1-u, v), (v, u), (s-1-v, s-1-u).  Written without
+// branches -- a transpose (k in {1, 2, 6, 7}) then a flip of the row
+// (k odd) and of the column (k in {2, 3, 4, 7}) -- so lanes of one wave
+// with different k do not run the eight cases one after another.
 __device__ __forceinline__ void iso_map(uint32_t k, uint32_t u, uint32_t v, uint32_t size,
                                         uint32_t *i, uint32_t *j)
 {
-    switch (k) {
-    case 0: *i = u; *j = v; break;
-    case 1: *i = size - 1 - v; *j = u; break;
-    case 2: *i = v; *j = size - 1 - u; break;
-    case 3: *i = size - 1 - u; *j = size - 1 - v; break;
-    case 4: *i = u; *j = size - 1 - v; break;
-    case 5: *i = size - 1 - u; *j = v; break;
-    case 6: *i = v; *j = u; break;
-    default: *i = size - 1 - v; *j = size - 1 - u; break;
-    }
+    const bool tr = (0xC6u >> k) & 1u;
+    const uint32_t a = tr ? v : u, b = tr ? u : v;
+    *i = (k & 1u) ? size - 1 - a : a;
+    *j = ((0x9Cu >> k) & 1u) ? size - 1 - b : b;
 }
 
 } // namespace mars

@@ -118,7 +118,9 @@ __global__ void k_sat_rows(const PlaneDesc *__restrict__ planes, const Work2 *__
 
 // Column pass: a block covers 16 columns x 16 row segments; each thread
 // scans its segment, the segment totals are combined in LDS, and a second
-// sweep adds the preceding segments' total.
+// sweep adds the preceding segments' total.  Both sweeps go CH rows at a
+// time, all loads of a batch before its stores, so a thread has CH loads in
+// flight instead of one.
 __global__ void k_sat_cols(const PlaneDesc *__restrict__ planes, const Work2 *__restrict__ work)
 {
     __shared__ int64_t tot1[16][16], tot2[16][16];
@@ -131,13 +133,27 @@ __global__ void k_sat_cols(const PlaneDesc *__restrict__ planes, const Work2 *__
     const uint32_t L = (hc + 15) / 16, r0 = 1 + seg * L;
     uint32_t r1 = r0 + L;
     if (r1 > hc + 1) r1 = hc + 1;
+    constexpr uint32_t CH = 8;
     int64_t a1 = 0, a2 = 0;
     if (j < W1)
-        for (uint32_t i = r0; i < r1; i++) {
-            a1 += sat1[(size_t)i * W1 + j];
-            a2 += sat2[(size_t)i * W1 + j];
-            sat1[(size_t)i * W1 + j] = a1;
-            sat2[(size_t)i * W1 + j] = a2;
+        for (uint32_t i0 = r0; i0 < r1; i0 += CH) {
+            int64_t x1[CH], x2[CH];
+#pragma unroll
+            for (uint32_t c = 0; c < CH; c++) {
+                const size_t o = (size_t)(i0 + c) * W1 + j;
+                x1[c] = i0 + c < r1 ? sat1[o] : 0;
+                x2[c] = i0 + c < r1 ? sat2[o] : 0;
+            }
+#pragma unroll
+            for (uint32_t c = 0; c < CH; c++) {
+                a1 += x1[c];
+                a2 += x2[c];
+                if (i0 + c < r1) {
+                    const size_t o = (size_t)(i0 + c) * W1 + j;
+                    sat1[o] = a1;
+                    sat2[o] = a2;
+                }
+            }
         }
     tot1[seg][cl] = a1;
     tot2[seg][cl] = a2;
@@ -148,9 +164,21 @@ __global__ void k_sat_cols(const PlaneDesc *__restrict__ planes, const Work2 *__
         o2 += tot2[s2][cl];
     }
     if (j < W1 && seg > 0)
-        for (uint32_t i = r0; i < r1; i++) {
-            sat1[(size_t)i * W1 + j] += o1;
-            sat2[(size_t)i * W1 + j] += o2;
+        for (uint32_t i0 = r0; i0 < r1; i0 += CH) {
+            int64_t x1[CH], x2[CH];
+#pragma unroll
+            for (uint32_t c = 0; c < CH; c++) {
+                const size_t o = (size_t)(i0 + c) * W1 + j;
+                x1[c] = i0 + c < r1 ? sat1[o] : 0;
+                x2[c] = i0 + c < r1 ? sat2[o] : 0;
+            }
+#pragma unroll
+            for (uint32_t c = 0; c < CH; c++)
+                if (i0 + c < r1) {
+                    const size_t o = (size_t)(i0 + c) * W1 + j;
+                    sat1[o] = x1[c] + o1;
+                    sat2[o] = x2[c] + o2;
+                }
         }
 }
 
@@ -196,18 +224,33 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
 // Range operands, work item (table, block): one thread per 4 bytes of
 // rng8, layout [range][iso][kmem] (i8 r-128, zero padded to kmem); with
 // `fp32` also rngf, layout [range/4][k][iso*4 + range%4] (f32 r).
-__global__ void k_ranges(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work,
-                         int fp32)
+__global__ __launch_bounds__(256) void k_ranges(const TableDesc *__restrict__ tabs,
+                                                const Work2 *__restrict__ work, int fp32)
 {
+    // The block's 1024 output bytes cover whole ranges (sizes <= 8) or part
+    // of one range; its ranges' pixels are staged in LDS first (row-wise,
+    // coalesced), so each output byte is an LDS read rather than a gather.
+    __shared__ uint8_t s_px[1024];
     const Work2 wk = work[blockIdx.x];
     const TableG T = view(tabs[wk.x]);
-    const uint32_t e = (wk.y * blockDim.x + threadIdx.x) * 4;
     const uint32_t lg_size = __builtin_ctz(T.size), lg_kmem = __builtin_ctz(T.kmem);
     const uint32_t size = T.size, K = size * size, kmem = T.kmem;
-    if (e >= T.nr_pad * 8 * kmem) return;
+    const uint32_t e0 = wk.y * 1024, total = T.nr_pad * 8 * kmem;
+    const uint32_t rb = e0 >> (lg_kmem + 3);
+    const uint32_t per = (1024u >> (lg_kmem + 3)) ? (1024u >> (lg_kmem + 3)) : 1u;
+    const uint32_t nrb = min(per, T.nr_pad - rb);
+    for (uint32_t idx = threadIdx.x; idx < nrb * K; idx += 256) {
+        const uint32_t rr = rb + (idx >> (2 * lg_size)), k = idx & (K - 1);
+        s_px[idx] = rr < T.nr
+                        ? T.px[(T.rows[rr] + (k >> lg_size)) * T.w + T.cols[rr] + (k & (size - 1))]
+                        : (uint8_t)128;
+    }
+    __syncthreads();
+    const uint32_t e = e0 + threadIdx.x * 4;
+    if (e >= total) return;
     const uint32_t r = e >> (lg_kmem + 3), iso = (e >> lg_kmem) & 7, k0 = e & (kmem - 1);
     const bool valid = r < T.nr;
-    const uint32_t row = valid ? T.rows[r] : 0, col = valid ? T.cols[r] : 0;
+    const uint8_t *src = s_px + (r - rb) * K;
     GLB float *rngf = fp32 ? T.rngf : nullptr;
     uint32_t packed = 0;
 #pragma unroll
@@ -217,7 +260,7 @@ __global__ void k_ranges(const TableDesc *__restrict__ tabs, const Work2 *__rest
             const uint32_t u = k >> lg_size, v = k & (size - 1);
             uint32_t i, j;
             iso_map(iso, u, v, size, &i, &j);
-            const uint32_t pix = T.px[(row + i) * T.w + col + j];
+            const uint32_t pix = src[i * size + j];
             packed |= ((pix - 128u) & 0xffu) << (8 * b);
             if (rngf) rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = (float)pix;
         } else if (rngf && k < K) {
