@@ -30,7 +30,7 @@ def _key(c):
             m.s2_x16, m.t0, m.t1_x4, m.t2)
 
 
-def _check_tables(img, ctx, label, sample=None, seed=0):
+def _check_tables(img, ctx, label, sample=None, seed=0, **quant):
     h, w = img.shape
     mism = 0
     total = 0
@@ -41,7 +41,7 @@ def _check_tables(img, ctx, label, sample=None, seed=0):
             idx = np.sort(rng.choice(idx, size=sample, replace=False))
         rr = [rows[i] for i in idx]
         cc = [cols[i] for i in idx]
-        ref, _ = O.search_table(img, size, shift, rr, cc, threads=16)
+        ref, _ = O.search_table(img, size, shift, rr, cc, threads=16, **quant)
         for j, i in enumerate(idx):
             total += 1
             if _key(cands[i]) != _key(ref[j]):
@@ -107,6 +107,25 @@ def test_parity_other_strides(shift):
     ctx = HipContext(0)
     ctx.prepare(img, EncodeParams(min_size=4, max_size=8, shift=shift), [shift])
     _check_tables(img, ctx, f"shift{shift}")
+
+
+@pytest.mark.parametrize("max_alfa,bits_alfa,bits_beta", [
+    (0.5, 4, 7),    # power-of-two scale: the exact integer stage 2 runs
+    (2.0, 3, 6),
+    (1.0, 8, 12),   # the widest codes stage 2 takes
+    (0.75, 4, 7),   # not a power of two: every stage-1 survivor gets the binary64 fit
+    (1.0, 9, 7),    # alfa codes beyond stage 2's range: likewise
+    (1.0, 4, 13),   # beta codes beyond stage 2's range: likewise
+])
+def test_parity_quantiser_settings(max_alfa, bits_alfa, bits_beta):
+    """The search parameters the fit and its quantisers depend on (encode.rs:247-289)."""
+    img = synth.plane(96, 64, 40 + bits_alfa)
+    print(f"quantiser max_alfa {max_alfa} bits {bits_alfa}/{bits_beta}", flush=True)
+    ctx = HipContext(0)
+    ctx.prepare(img, EncodeParams(min_size=4, max_size=16, shift=4, max_alfa=max_alfa,
+                                  bits_alfa=bits_alfa, bits_beta=bits_beta), [4])
+    _check_tables(img, ctx, f"quant{max_alfa}/{bits_alfa}/{bits_beta}", max_alfa=max_alfa,
+                  bits_alfa=bits_alfa, bits_beta=bits_beta)
 
 
 def test_parity_flat_plane_all_ties():
