@@ -60,3 +60,20 @@ def test_rd_variants_160x96(lam, modes, density):
                       lambda_adaptive_qstep=density)
     ref, _, got, _ = _both(img, p, o, [4, 8] if density else [4])
     assert got == ref
+
+
+@pytest.mark.parametrize("max_alfa,bits_alfa,bits_beta,zero,lam", [
+    (0.5, 5, 6, 1, None),     # threshold walk, zero-alfa override on small codes
+    (0.75, 4, 7, 0, 120.0),   # RD walk, a scale that is not a power of two
+])
+def test_quantiser_settings_odd_geometry(max_alfa, bits_alfa, bits_beta, zero, lam):
+    """75x53 plane (border nodes down to size 2) with non-default quantisers."""
+    img = synth.plane(75, 53, 91)
+    p = EncodeParams(min_size=4, max_size=16, shift=4, max_alfa=max_alfa, bits_alfa=bits_alfa,
+                     bits_beta=bits_beta, zero_threshold=zero, lambda_=lam)
+    o = EncodeOptions(allowed_modes=(True, True, True, True)) if lam else EncodeOptions()
+    print(f"encode 75x53 max_alfa {max_alfa} bits {bits_alfa}/{bits_beta} zero {zero} "
+          f"lambda {lam}", flush=True)
+    ref, ref_st, got, st = _both(img, p, o, [4])
+    assert (st.search_evals, st.warmup_evals) == (ref_st.search_evals, ref_st.warmup_evals)
+    assert got == ref
