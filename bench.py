@@ -38,6 +38,7 @@ PEAK_HBM = 8.0e12        # bytes/s
 W = H = 512
 SEED = 2
 MIN_SIZE, MAX_SIZE, SHIFT = 4, 32, 4
+WARMUP_T_RMS = 8.0  # the RD warm-up's threshold walk (encode.rs:1139-1169)
 
 
 def _gpu_leg(steps: int, warmup: int):
@@ -130,16 +131,24 @@ def _colour_leg(steps: int):
     cp = EncodeParams(min_size=4, max_size=16, shift=SHIFT, lambda_=200.0)
     out = {}
     for name, planes, p in (("y", [y], yp), ("chroma", [cb, cr], cp)):
-        ctx = HipContext(0)
-        ctx.prepare_batch(planes, p, [SHIFT], method=N.METHOD_FISHER)
-        ctx.run(iters=2, use_graph=True)
-        br = [ctx.run(iters=1, use_graph=False) for _ in range(5)]
-        t = ctx.run(iters=steps, use_graph=True)
-        ctx.close()
-        out[name] = dict(ms_step=t.ms_step, exhaustive_evals=t.evals,
-                         fisher_evals=int(br[0].fisher_evals),
-                         ms_fisher=float(np.mean([b.ms_fisher for b in br])),
-                         ms_exhaustive_search=float(np.mean([b.ms_search for b in br])))
+        # As the encoder prepares them: the exhaustive tables only for the
+        # nodes the RD warm-up's threshold walk visits; beside, full tables.
+        for walk in (True, False):
+            ctx = HipContext(0)
+            if walk:
+                ctx.set_walk_tables(WARMUP_T_RMS)
+            ctx.prepare_batch(planes, p, [SHIFT], method=N.METHOD_FISHER)
+            ctx.run(iters=2, use_graph=True)
+            br = [ctx.run(iters=1, use_graph=False) for _ in range(5)]
+            t = ctx.run(iters=steps, use_graph=True)
+            ctx.close()
+            if walk:
+                out[name] = dict(ms_step=t.ms_step, exhaustive_evals=t.evals,
+                                 fisher_evals=int(br[0].fisher_evals),
+                                 ms_fisher=float(np.mean([b.ms_fisher for b in br])),
+                                 ms_exhaustive_search=float(np.mean([b.ms_search for b in br])))
+            else:
+                out[name]["full_tables"] = dict(ms_step=t.ms_step, exhaustive_evals=t.evals)
     ctx = HipContext(0)
     o = EncodeOptions(allowed_modes=(True, False, True, False))
     t0 = time.perf_counter()

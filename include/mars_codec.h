@@ -81,6 +81,8 @@ typedef struct {
 
 /* Most ranked candidates a provider is asked for (rd_candidates). */
 #define MARS_MAX_RANKED 32
+/* The RD warm-up's threshold walk runs at this t_rms (encode.rs:1139-1169). */
+#define MARS_WARMUP_T_RMS 8.0
 
 /*
  * SearchProvider::search_candidates (encode.rs:557-579): up to `max` of the

@@ -79,7 +79,8 @@ typedef struct {
     double ms_pool;     /* contraction + per-size window sums */
     double ms_search;   /* search kernels, all sizes */
     double ms_by_size[6]; /* search kernel per size 2,4,8,16,32,64 (index log2(size)-1) */
-    uint64_t evals;     /* range x position x isometry candidates per step */
+    uint64_t evals;     /* range x position x isometry candidates per step (walk-only
+                           tables: of the nodes the walk visits) */
     uint64_t survivors; /* candidates past the integer bound (MFMA kernel) */
     uint64_t survivors_by_size[6];  /* the same per size 2..64 */
     uint64_t exact_fits_by_size[6]; /* candidates that took the verbatim binary64 fit */
@@ -167,6 +168,24 @@ int mars_hip_search_method(const mars_hip_ctx *ctx, uint32_t plane, uint32_t met
  * mars_hip_search with up to k and need the MFMA kernel.
  */
 int mars_hip_set_ranked(mars_hip_ctx *ctx, uint32_t k);
+
+/*
+ * Exhaustive tables for a threshold walk only.  With t_rms > 0, the next
+ * MARS_HIP_METHOD_FISHER prepare (k = 1) computes the exhaustive winners of
+ * just the nodes a threshold walk at t_rms visits at the base stride -- the
+ * RD warm-up's walk (encode.rs:1139-1169, t_rms 8) asks nothing else when the
+ * final walk uses the Fisher retriever: every node of the largest size, and
+ * the children of each searched node whose best rms exceeds t_rms and whose
+ * size exceeds min_size (nodes split without a search pass theirs on).  The
+ * step then runs the sizes one after another, largest first, each activated
+ * on the device from the previous size's winners.  An exhaustive lookup of a
+ * node the walk does not visit returns MARS_HIP_E_NOT_IN_TABLE; Fisher
+ * lookups are unaffected.  t_rms <= 0 (the default) keeps full tables.  No
+ * counterpart in the reference (whose warm-up searches on the CPU node by
+ * node); mars_hip_encode_plane_method and mars_hip_encode_rgb set it for
+ * Fisher encodes with a lambda.
+ */
+int mars_hip_set_walk_tables(mars_hip_ctx *ctx, double t_rms);
 
 /* Last error message of this context ("" if none). */
 const char *mars_hip_last_error(const mars_hip_ctx *ctx);

@@ -22,7 +22,7 @@ EXPORTS = (
     "mars_hip_run", "mars_hip_autotune", "mars_hip_table_count", "mars_hip_table_plane",
     "mars_hip_table_info", "mars_hip_table_read", "mars_hip_table_scores", "mars_hip_fit_f64",
     "mars_hip_copy_probe", "mars_hip_search_method", "mars_hip_table_read_method",
-    "mars_hip_set_ranked", "mars_hip_table_read_ranked",
+    "mars_hip_set_ranked", "mars_hip_table_read_ranked", "mars_hip_set_walk_tables",
 )
 
 
@@ -97,6 +97,7 @@ def lib() -> C.CDLL:
     L.mars_hip_table_read_method.argtypes = [C.c_void_p, u32, u32, P(CCandidate), P(C.c_uint8),
                                              P(u64)]
     L.mars_hip_set_ranked.argtypes = [C.c_void_p, u32]
+    L.mars_hip_set_walk_tables.argtypes = [C.c_void_p, C.c_double]
     L.mars_hip_table_read_ranked.argtypes = [C.c_void_p, u32, P(u32), P(CCandidate),
                                              P(C.c_uint8)]
     for name in EXPORTS:

@@ -95,6 +95,12 @@ struct Table {
     std::vector<mars_hip_candidate> h_out; // [nr][topk]
     std::vector<uint8_t> h_found;
     uint32_t topk = 1;
+    // Walk-only batches: the threshold walk's visits (see TableDesc)
+    uint8_t *d_active = nullptr;
+    int32_t *d_parent = nullptr;
+    uint32_t parent_tab = 0;
+    std::vector<int32_t> h_parent;
+    std::vector<uint8_t> h_active;
     // Fisher classified search (method MARS_HIP_METHOD_FISHER only)
     bool fisher = false;
     uint16_t *d_fcls = nullptr, *d_frcls = nullptr;
@@ -117,6 +123,7 @@ struct Group {
     std::vector<uint32_t> tables; // longest searches first
     Slice search[2], merge[2];    // per kernel
     Slice merge_ranked;           // (table, range) items of the ranked merge
+    Slice seed, activate;         // walk-only batches: (table, 4 ranges), (table, 256 ranges)
 };
 
 } // namespace
@@ -132,6 +139,9 @@ struct mars_hip_ctx {
     uint32_t method = MARS_HIP_METHOD_EXHAUSTIVE; // of the prepared batch
     uint32_t ranked = 1;   // candidates per range the next prepare keeps (search_top_k's k)
     uint32_t topk = 1;     // ... and the prepared batch keeps
+    // Threshold of the walk the next Fisher prepare serves its exhaustive
+    // tables to (0: full tables), and that of the prepared batch.
+    double walk_t = 0.0, walk_prepared = 0.0;
     bool prepared = false;
     mars_hip_params params{};
     std::vector<Plane> planes;
@@ -248,6 +258,7 @@ void free_batch(mars_hip_ctx *ctx)
     ctx->work2_cap = ctx->work3_cap = 0;
     ctx->d_yscale = nullptr;
     ctx->prepared = false;
+    ctx->walk_prepared = 0.0;
 }
 
 // Position chunks for a table of `ntiles` tiles in a launch that covers
@@ -324,7 +335,10 @@ int layout(mars_hip_ctx *ctx)
     drop_graph(ctx);
     // chunking per group: the launch covers all tables of the group
     for (Group &g : ctx->groups) {
-        const Shape &sh = ctx->shape[ceil_log2(g.size) - 1];
+        // Walk-only batches search few of their groups, each whole: a finer
+        // split of the positions than the autotuned one keeps more CUs busy.
+        const Shape walk_shape{8192, 8};
+        const Shape &sh = ctx->walk_prepared > 0.0 ? walk_shape : ctx->shape[ceil_log2(g.size) - 1];
         for (int k = 0; k < 2; k++) {
             uint32_t ng = 0;
             for (uint32_t ti : g.tables) ng += groups_of(ctx->tables[ti], k);
@@ -377,6 +391,9 @@ int layout(mars_hip_ctx *ctx)
         d.out = t.d_out;
         d.found = t.d_found;
         d.score = t.d_score;
+        d.active = t.d_active;
+        d.parent = t.d_parent;
+        d.parent_tab = t.parent_tab;
         d.size = t.size;
         d.shift = t.shift;
         d.P = t.P;
@@ -494,6 +511,20 @@ int layout(mars_hip_ctx *ctx)
             for (uint32_t ti : g.tables)
                 for (uint32_t r = 0; r < ctx->tables[ti].nr; r++) w2.push_back(make_uint2(ti, r));
         end2(g.merge_ranked);
+        // walk-only batches seed and activate each group after its parent's merge
+        const bool walk = ctx->walk_prepared > 0.0;
+        begin2(g.seed);
+        for (uint32_t ti : g.tables) {
+            const Table &t = ctx->tables[ti];
+            if (!walk || t.P == 0) continue;
+            for (uint32_t b = 0; b < cdiv(t.nr, 4); b++) w2.push_back(make_uint2(ti, b));
+        }
+        end2(g.seed);
+        begin2(g.activate);
+        for (uint32_t ti : g.tables)
+            for (uint32_t b = 0; walk && b < cdiv(ctx->tables[ti].nr_pad, 256); b++)
+                w2.push_back(make_uint2(ti, b));
+        end2(g.activate);
     }
     // every launch must stay below 2^32 threads
     if (w2.size() >= (1u << 24) || w3.size() >= (1u << 24))
@@ -531,14 +562,16 @@ void enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st)
     mars::launch_ranges(ctx->d_tabs, w2 + ctx->wl_ranges.off, ctx->wl_ranges.n,
                         kernel == MARS_HIP_KERNEL_FP32, st);
     mars::launch_rstats(ctx->d_tabs, w2 + ctx->wl_rstats.off, ctx->wl_rstats.n, st);
-    if (kernel == MARS_HIP_KERNEL_MFMA_I8)
+    if (kernel == MARS_HIP_KERNEL_MFMA_I8 && ctx->walk_prepared <= 0.0)
         mars::launch_seed(ctx->d_tabs, w2 + ctx->wl_seed.off, ctx->wl_seed.n, ctx->common, st);
 }
 
 // Sizes whose MFMA search first runs a threshold pre-pass over every
 // kPrepassStride-th position tile: their stage-1 bound lets many candidates
 // through until the per-range thresholds are close to final, so tightening
-// them on a sample first cuts the exact fits of the full pass.
+// them on a sample first cuts the exact fits of the full pass.  (Not for
+// walk-only batches, whose few searched groups the pre-pass would delay more
+// than it saves.)
 constexpr uint32_t kPrepassMinSize = 16, kPrepassStride = 8;
 
 int enqueue_search(mars_hip_ctx *ctx, const Group &g, int kernel, hipStream_t st)
@@ -546,7 +579,7 @@ int enqueue_search(mars_hip_ctx *ctx, const Group &g, int kernel, hipStream_t st
     const int k = kernel == MARS_HIP_KERNEL_MFMA_I8 ? 0 : 1;
     Common c = ctx->common;
     c.sample = 1;
-    if (k == 0 && g.size >= kPrepassMinSize) {
+    if (k == 0 && g.size >= kPrepassMinSize && ctx->walk_prepared <= 0.0) {
         Common pre = c;
         pre.sample = kPrepassStride;
         if (mars::launch_search(kernel, g.size, ctx->d_tabs, ctx->d_work3 + g.search[k].off,
@@ -591,6 +624,41 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
     hipStream_t st = ctx->stream;
     if (ev) HIP_TRY(ctx, hipEventRecord(ev[0], st));
     enqueue_prep(ctx, kernel, st);
+    if (ctx->walk_prepared > 0.0) {
+        // Walk-only batch: the sizes in turn, largest first, each group's
+        // ranges activated from its parent group's merged winners, then
+        // seeded, searched and merged.  That chain keeps few blocks busy, so
+        // it runs on a side stream beside the Fisher search; with ev it runs
+        // serially on the context stream, group i's interval covering all
+        // four steps.
+        if (ev) HIP_TRY(ctx, hipEventRecord(ev[1], st));
+        hipStream_t ws = st;
+        if (!ev) {
+            HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, st));
+            ws = ctx->side[0];
+            HIP_TRY(ctx, hipStreamWaitEvent(ws, ctx->fork_ev, 0));
+        }
+        const Work2 *w2 = ctx->d_work2;
+        for (size_t i = 0; i < ctx->groups.size(); i++) {
+            const Group &g = ctx->groups[i];
+            mars::launch_activate(ctx->d_tabs, w2 + g.activate.off, g.activate.n, ctx->walk_prepared,
+                                  ctx->params.min_size, ws);
+            if (kernel == MARS_HIP_KERNEL_MFMA_I8)
+                mars::launch_seed(ctx->d_tabs, w2 + g.seed.off, g.seed.n, ctx->common, ws);
+            RC_TRY(enqueue_search(ctx, g, kernel, ws));
+            enqueue_merge(ctx, g, kernel, ws);
+            if (ev) HIP_TRY(ctx, hipEventRecord(ev[2 + i], ws));
+        }
+        if (ev) HIP_TRY(ctx, hipEventRecord(ev[2 + ctx->groups.size()], st));
+        enqueue_fisher(ctx, st);
+        if (ev) HIP_TRY(ctx, hipEventRecord(ev[3 + ctx->groups.size()], st));
+        if (!ev) {
+            HIP_TRY(ctx, hipEventRecord(ctx->join_ev[0], ws));
+            HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join_ev[0], 0));
+        }
+        HIP_TRY(ctx, hipGetLastError());
+        return MARS_HIP_OK;
+    }
     if (!ev) {
         HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, st));
         const int nside = std::min<int>(kSideStreams, (int)ctx->groups.size());
@@ -632,6 +700,11 @@ int download(mars_hip_ctx *ctx)
                                     hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipMemcpyAsync(t.h_found.data(), t.d_found, t.nr, hipMemcpyDeviceToHost,
                                     ctx->stream));
+        if (t.d_active) {
+            t.h_active.resize(t.nr);
+            HIP_TRY(ctx, hipMemcpyAsync(t.h_active.data(), t.d_active, t.nr, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+        }
         if (!t.fisher) continue;
         t.h_fout.resize(t.nr);
         t.h_ffound.resize(t.nr);
@@ -685,7 +758,12 @@ int timed_step(mars_hip_ctx *ctx, mars_hip_timing *tm, std::vector<double> *grou
             tm->exact_fits_by_size[i] = cnt[8 + i + 1];
             tm->survivors += cnt[i + 1];
         }
-        for (const Table &t : ctx->tables) tm->evals += (uint64_t)t.nr * table_evals(t);
+        for (const Table &t : ctx->tables) {
+            // walk-only tables: the visited nodes (the same every step)
+            uint64_t nr = t.nr;
+            if (t.d_active) nr = std::count(t.h_active.begin(), t.h_active.end(), (uint8_t)1);
+            tm->evals += nr * table_evals(t);
+        }
         (void)hipEventElapsedTime(&ms, ev[2 + ng], ev[3 + ng]);
         tm->ms_fisher = ms;
         for (const Table &t : ctx->tables)
@@ -789,6 +867,33 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
             ctx->groups.push_back(std::move(g));
         }
     }
+    // Walk-only tables (Fisher batches whose exhaustive tables only serve a
+    // threshold walk, the RD warm-up): every range's parent, at the base
+    // stride (the walk's); other strides are never asked.
+    ctx->walk_prepared = 0.0;
+    if (ctx->walk_t > 0.0 && p->method == MARS_HIP_METHOD_FISHER && ctx->topk == 1) {
+        ctx->walk_prepared = ctx->walk_t;
+        for (Table &t : ctx->tables) {
+            t.h_parent.assign(t.nr, -2);
+            if (t.shift != p->shift) continue;
+            const uint32_t ps = 2 * t.size;
+            auto pt = ctx->table_of.find(std::make_tuple(t.plane, ps, t.shift));
+            const Table *u = ps <= p->max_size && pt != ctx->table_of.end() ? &ctx->tables[pt->second]
+                                                                           : nullptr;
+            t.parent_tab = u ? pt->second : 0;
+            for (uint32_t r = 0; r < t.nr; r++) {
+                // A parent absent from its size's table is one the walk splits
+                // without a search (larger than max_size or across the border).
+                int32_t par = -1;
+                if (u) {
+                    auto it = u->index.find(((uint64_t)(t.rows[r] & ~(ps - 1)) << 32) |
+                                            (t.cols[r] & ~(ps - 1)));
+                    if (it != u->index.end()) par = (int32_t)it->second;
+                }
+                t.h_parent[r] = par;
+            }
+        }
+    }
     // partial capacity: the most chunks any shape of the autotuner's space gives
     for (Group &g : ctx->groups)
         for (int k = 0; k < 2; k++) {
@@ -835,6 +940,10 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         plan.add(&t.d_out, sizeof(mars_hip_candidate) * (size_t)t.nr * t.topk);
         plan.add(&t.d_found, (size_t)t.nr);
         plan.add(&t.d_score, sizeof(double) * (size_t)t.nr);
+        if (ctx->walk_prepared > 0.0) {
+            plan.add(&t.d_active, (size_t)t.nr_pad);
+            plan.add(&t.d_parent, sizeof(int32_t) * (size_t)t.nr);
+        }
         if (t.fisher) {
             plan.add(&t.d_fcls, sizeof(uint16_t) * (size_t)t.P);
             plan.add(&t.d_fbuck, sizeof(uint32_t) * 2 * 72);
@@ -859,6 +968,9 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         HIP_TRY(ctx, hipMemcpyAsync(t.d_cols, t.cols.data(), 4 * (size_t)t.nr, hipMemcpyHostToDevice, ctx->stream));
         if (!t.iperm.empty())
             HIP_TRY(ctx, hipMemcpyAsync(t.d_iperm, t.iperm.data(), 4 * t.iperm.size(),
+                                        hipMemcpyHostToDevice, ctx->stream));
+        if (t.d_parent && t.nr)
+            HIP_TRY(ctx, hipMemcpyAsync(t.d_parent, t.h_parent.data(), 4 * (size_t)t.nr,
                                         hipMemcpyHostToDevice, ctx->stream));
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -926,6 +1038,14 @@ int lookup(const mars_hip_ctx *ctx, uint32_t plane, uint32_t method, uint32_t ro
                 *n_out = t.h_ffound[i] ? 1 : 0;
                 if (*n_out) out[0] = t.h_fout[i];
             } else {
+                if (t.d_active && !t.h_active[i]) {
+                    char buf[192];
+                    snprintf(buf, sizeof buf,
+                             "plane %u range (%u,%u) size %u shift %u: the prepared walk-only "
+                             "tables hold only the nodes the threshold walk visits",
+                             plane, row, col, size, shift);
+                    return fail(mctx, MARS_HIP_E_NOT_IN_TABLE, buf);
+                }
                 *evals = table_evals(t);
                 const uint32_t n = std::min<uint32_t>(k, t.h_found[i]);
                 for (uint32_t j = 0; j < n; j++) out[j] = t.h_out[(size_t)i * t.topk + j];
@@ -1022,6 +1142,14 @@ int mars_hip_set_ranked(mars_hip_ctx *ctx, uint32_t k)
     if (!ctx) return MARS_HIP_E_ARG;
     if (k < 1 || k > mars::kMaxRanked) return fail(ctx, MARS_HIP_E_ARG, "k must be 1..32");
     ctx->ranked = k;
+    return MARS_HIP_OK;
+}
+
+int mars_hip_set_walk_tables(mars_hip_ctx *ctx, double t_rms)
+{
+    if (!ctx) return MARS_HIP_E_ARG;
+    if (!(t_rms == t_rms)) return fail(ctx, MARS_HIP_E_ARG, "t_rms is NaN");
+    ctx->walk_t = t_rms > 0.0 ? t_rms : 0.0;
     return MARS_HIP_OK;
 }
 
@@ -1344,9 +1472,13 @@ int mars_hip_encode_plane_method(mars_hip_ctx *ctx, const uint8_t *px, uint32_t 
                               ? std::max<uint32_t>(1, std::min<uint32_t>(o->rd_candidates, mars::kMaxRanked))
                               : 1;
     const uint32_t keep = ctx->ranked;
+    const double keep_walk = ctx->walk_t;
     RC_TRY(mars_hip_set_ranked(ctx, want));
+    // With the Fisher retriever the exhaustive tables serve only the warm-up.
+    ctx->walk_t = method == MARS_HIP_METHOD_FISHER && p->has_lambda ? MARS_WARMUP_T_RMS : 0.0;
     const int prc = mars_hip_prepare(ctx, px, w, h, &hp, shifts, ns);
     ctx->ranked = keep;
+    ctx->walk_t = keep_walk;
     RC_TRY(prc);
     return mars_hip_encode_prepared(ctx, 0, p, o, out, out_len, stats);
 }
@@ -1414,11 +1546,16 @@ int mars_hip_encode_rgb(mars_hip_ctx *ctx, const uint8_t *rgb, uint32_t w, uint3
         const uint32_t ns = walk_shifts(chroma_params, o, shifts);
         const mars_hip_plane planes[2] = {{cb.data(), cw, ch}, {cr.data(), cw, ch}};
         const uint32_t keep = ctx->ranked;
+        const double keep_walk = ctx->walk_t;
         ctx->ranked = method == MARS_HIP_METHOD_EXHAUSTIVE && chroma_params->has_lambda
                           ? std::max<uint32_t>(1, std::min<uint32_t>(o->rd_candidates, mars::kMaxRanked))
                           : 1;
+        ctx->walk_t = method == MARS_HIP_METHOD_FISHER && chroma_params->has_lambda
+                          ? MARS_WARMUP_T_RMS
+                          : 0.0;
         rc = mars_hip_prepare_batch(ctx, planes, 2, &hp, shifts, ns);
         ctx->ranked = keep;
+        ctx->walk_t = keep_walk;
     }
     if (rc == 0) rc = mars_hip_encode_prepared(ctx, 0, chroma_params, o, &cbs, &cbl, &st.cb);
     if (rc == 0) rc = mars_hip_encode_prepared(ctx, 1, chroma_params, o, &crs, &crl, &st.cr);

@@ -923,7 +923,7 @@ int mars_encode_plane(const uint8_t *px, uint32_t w, uint32_t h, const mars_enco
     if (P.has_lambda) {
         // encode.rs:1139-1169 build_rate_snapshot: threshold walk at t_rms 8.
         Params wp = P;
-        wp.t_rms = 8.0;
+        wp.t_rms = MARS_WARMUP_T_RMS;
         wp.has_lambda = false;
         Ctx wctx = ctx;
         wctx.params = &wp;

@@ -120,6 +120,9 @@ struct TableG {
     GLB mars_hip_candidate *out;
     GLB uint8_t *found;
     GLB double *score;
+    GLB uint8_t *active;
+    const GLB int32_t *parent;
+    uint32_t parent_tab;
     uint32_t size, shift, P, P_pad, npc, nr, nr_pad, kmem;
     uint32_t nchunks[2], tiles_per_chunk[2];
     GLB uint16_t *fcls;
@@ -160,6 +163,9 @@ __device__ __forceinline__ TableG view(const TableDesc &t)
     g.out = glb(t.out);
     g.found = glb(t.found);
     g.score = glb(t.score);
+    g.active = glb(t.active);
+    g.parent = glb(t.parent);
+    g.parent_tab = t.parent_tab;
     g.size = t.size;
     g.shift = t.shift;
     g.P = t.P;

@@ -64,6 +64,13 @@ struct TableDesc {
     mars_hip_candidate *out; // [nr][topk]
     uint8_t *found;          // [nr] candidates found (0..topk)
     double *score;       // [nr] the search kernel's own score of the winner (FP32: binary32 rms)
+    // Walk-only tables (both null otherwise): only the ranges a threshold
+    // walk visits are searched.  active[r]: visited this step; parent[r]:
+    // the range's parent in table parent_tab, -1 when the parent splits
+    // without a search (visited whatever the results), -2 when no walk asks.
+    uint8_t *active;         // [nr_pad]
+    const int32_t *parent;   // [nr]
+    uint32_t parent_tab;
     uint32_t size, shift, P, P_pad, npc, nr, nr_pad, kmem;
     // Position chunking of the search: [0] MFMA kernel (32-position
     // tiles), [1] FP32 baseline (64-position tiles).  partial holds
@@ -106,6 +113,11 @@ void launch_possums(const TableDesc *t, const Work2 *work, uint32_t nwork, hipSt
 void launch_ranges(const TableDesc *t, const Work2 *work, uint32_t nwork, bool fp32,
                    hipStream_t st);
 void launch_rstats(const TableDesc *t, const Work2 *work, uint32_t nwork, hipStream_t st);
+// Walk-only tables, work item (table, block of 256 ranges): active[r] from
+// the parent's merged result (split when its rms > t_rms and its size >
+// min_size, the threshold walk's rule).
+void launch_activate(const TableDesc *t, const Work2 *work, uint32_t nwork, double t_rms,
+                     uint32_t min_size, hipStream_t st);
 // Work item (table, first of four ranges).
 void launch_seed(const TableDesc *t, const Work2 *work, uint32_t nwork, const Common &c,
                  hipStream_t st);

@@ -398,6 +398,35 @@ __device__ __forceinline__ uint32_t tau_bits(double sum, int64_t n)
     return __float_as_uint((float)(tau * (1.0 + 1e-7)));
 }
 
+// Walk-only tables, work item (table, block of 256 ranges): whether the
+// threshold walk (encode.rs walk, the RD warm-up's) visits range r, from its
+// parent's merged winner -- the parent splits when its best rms exceeds
+// t_rms (no candidate: +inf) and its size exceeds min_size; a parent the walk
+// splits without searching (larger than the largest range size, or across
+// the border) always passes its children on.
+__global__ void k_activate(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work,
+                           double t_rms, uint32_t min_size)
+{
+    const Work2 wk = work[blockIdx.x];
+    const TableG T = view(tabs[wk.x]);
+    const uint32_t r = wk.y * 256 + threadIdx.x;
+    if (r >= T.nr_pad) return;
+    uint8_t a = 0;
+    if (r < T.nr) {
+        const int32_t p = T.parent[r];
+        if (p == -1) {
+            a = 1;
+        } else if (p >= 0) {
+            const TableG U = view(tabs[T.parent_tab]);
+            const GLB double *rp = (const GLB double *)((const GLB char *)(U.out + p) +
+                                                        offsetof(mars_hip_candidate, rms));
+            const double rms = U.found[p] ? *rp : __longlong_as_double(0x7ff0000000000000ll);
+            a = U.active[p] && rms > t_rms && U.size > min_size;
+        }
+    }
+    T.active[r] = a;
+}
+
 // Correlation t1_x4 of one candidate from its range row rk (i8 r - 128,
 // 16-byte aligned) and its window dw of the contracted plane (row stride wc):
 // sizes <= 8 issue every load at once, larger sizes one window row at a time.
@@ -502,6 +531,7 @@ __global__ __launch_bounds__(256) void k_seed(const TableDesc *__restrict__ tabs
     const TableG T = view(tabs[wk.x]);
     const uint32_t wave = threadIdx.x >> 6, r = wk.y * 4 + wave;
     if (r >= T.nr) return; // the whole wave: no barrier below is block-wide
+    if (T.active && !T.active[r]) return; // walk-only: not visited this step
     switch (T.size) {
     case 2: seed_wave<2>(T, r, cm, s_tau[wave]); break;
     case 4: seed_wave<4>(T, r, cm, s_tau[wave]); break;
@@ -617,6 +647,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     const uint32_t chunk = wk.y, g = wk.z;
     const uint32_t nchunks = T.nchunks[0], tiles_per_chunk = T.tiles_per_chunk[0];
     const double nd = (double)n, inv_n = 1.0 / (double)n;
+    if (T.active) { // walk-only table: a group none of whose ranges is visited is skipped
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < 8; i++) any |= T.active[g * 8 + i] != 0; // nr_pad is a multiple of 8
+        if (!any) return;
+    }
 
     __shared__ __align__(16) uint32_t s_thr[8];
     __shared__ unsigned long long s_count[2];
@@ -1302,6 +1338,12 @@ __global__ void k_merge_exact(const TableDesc *__restrict__ tabs, const Work2 *_
     const TableG T = view(tabs[wk.x]);
     const uint32_t r = wk.y * blockDim.x + threadIdx.x;
     if (r >= T.nr) return;
+    if (T.active && !T.active[r]) { // walk-only, not visited: its partials are stale
+        T.score[r] = __longlong_as_double(0x7ff0000000000000ll);
+        st_cand(T.out, r, mars_hip_candidate{});
+        T.found[r] = 0;
+        return;
+    }
     const uint32_t nchunks = T.nchunks[0];
     double b = __longlong_as_double(0x7ff0000000000000ll);
     uint32_t k = 0xffffffffu;
@@ -1339,6 +1381,14 @@ __global__ void k_merge(const TableDesc *__restrict__ tabs, const Work2 *__restr
     const Work2 wk = work[blockIdx.x];
     const TableG T = view(tabs[wk.x]);
     const uint32_t r = wk.y, t = threadIdx.x, nchunks = T.nchunks[1];
+    if (T.active && !T.active[r]) { // walk-only, not visited (block-uniform)
+        if (t == 0) {
+            T.score[r] = __longlong_as_double(0x7ff0000000000000ll);
+            st_cand(T.out, r, mars_hip_candidate{});
+            T.found[r] = 0;
+        }
+        return;
+    }
     double b = __longlong_as_double(0x7ff0000000000000ll);
     uint32_t k = 0xffffffffu;
     int32_t t1 = 0;
@@ -1494,6 +1544,12 @@ void launch_sat(const PlaneDesc *planes, const Work2 *rows_work, uint32_t n_rows
 void launch_possums(const TableDesc *t, const Work2 *work, uint32_t n, hipStream_t st)
 {
     if (n) hipLaunchKernelGGL(k_possums, dim3(n), dim3(256), 0, st, t, work);
+}
+
+void launch_activate(const TableDesc *t, const Work2 *work, uint32_t n, double t_rms,
+                     uint32_t min_size, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_activate, dim3(n), dim3(256), 0, st, t, work, t_rms, min_size);
 }
 
 void launch_seed(const TableDesc *t, const Work2 *work, uint32_t n, const Common &c,

@@ -115,6 +115,11 @@ class HipContext:
         """Candidates per range the next prepare keeps (search_top_k's k, 1..32)."""
         self._check(self._L.mars_hip_set_ranked(self._ctx, k), "mars_hip_set_ranked")
 
+    def set_walk_tables(self, t_rms: float) -> None:
+        """t_rms > 0: the next Fisher prepare computes exhaustive winners only for
+        the nodes a threshold walk at t_rms visits (the RD warm-up's); 0: full."""
+        self._check(self._L.mars_hip_set_walk_tables(self._ctx, t_rms), "mars_hip_set_walk_tables")
+
     def prepare(self, image: np.ndarray, params: EncodeParams, shifts: Sequence[int],
                 method: int = N.METHOD_EXHAUSTIVE) -> None:
         img = np.ascontiguousarray(image, dtype=np.uint8)
