@@ -57,6 +57,11 @@ struct Shape {
 constexpr uint32_t kShapeTargets[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
 constexpr uint32_t kShapeMinTiles[] = {4, 8, 16};
 constexpr uint32_t kMaxTarget = 16384, kMinTiles = 4; // bounds of the autotuner's space
+// Walk-only batches: position chunks per range group of the MFMA search (its
+// work list holds only the visited groups, so it can split them finer than
+// a list of every group could afford).
+constexpr uint32_t kWalkChunks = 8;
+uint32_t walk_chunks(uint32_t ntiles) { return std::max<uint32_t>(1, std::min(ntiles, kWalkChunks)); }
 
 struct Plane {
     uint32_t w = 0, h = 0, wc = 0, hc = 0, pitch = 0;
@@ -124,6 +129,10 @@ struct Group {
     Slice search[2], merge[2];    // per kernel
     Slice merge_ranked;           // (table, range) items of the ranked merge
     Slice seed, activate;         // walk-only batches: (table, 4 ranges), (table, 256 ranges)
+    // Walk-only batches: the MFMA search's work list, compacted on the device
+    // to the chunks of the range groups the walk visits (capacity: every chunk).
+    Work3 *d_wq = nullptr;
+    uint32_t wq_cap = 0;
 };
 
 } // namespace
@@ -142,6 +151,7 @@ struct mars_hip_ctx {
     // Threshold of the walk the next Fisher prepare serves its exhaustive
     // tables to (0: full tables), and that of the prepared batch.
     double walk_t = 0.0, walk_prepared = 0.0;
+    uint32_t *d_wq_count = nullptr; // per group: items in its compacted work list
     bool prepared = false;
     mars_hip_params params{};
     std::vector<Plane> planes;
@@ -346,6 +356,10 @@ int layout(mars_hip_ctx *ctx)
                 Table &t = ctx->tables[ti];
                 chunking(tiles_of(t, k), ng, sh.target_blocks, sh.min_tiles, &t.nchunks[k],
                          &t.tpc[k]);
+                if (ctx->walk_prepared > 0.0 && k == 0) { // fixed chunks, listed on the device
+                    t.tpc[0] = std::max<uint32_t>(1, cdiv(tiles_of(t, 0), walk_chunks(tiles_of(t, 0))));
+                    t.nchunks[0] = std::max<uint32_t>(1, cdiv(tiles_of(t, 0), t.tpc[0]));
+                }
                 if (t.nchunks[k] > t.nch_cap[k]) { // outside the allocated shape space
                     t.tpc[k] = cdiv(tiles_of(t, k), t.nch_cap[k]);
                     t.nchunks[k] = cdiv(tiles_of(t, k), t.tpc[k]);
@@ -493,7 +507,8 @@ int layout(mars_hip_ctx *ctx)
             g.search[k].off = (uint32_t)w3.size();
             for (uint32_t ti : g.tables) {
                 const Table &t = ctx->tables[ti];
-                if (t.P == 0) continue;
+                // (walk-only batches list the MFMA search's items on the device)
+                if (t.P == 0 || (k == 0 && ctx->walk_prepared > 0.0)) continue;
                 for (uint32_t gr = 0; gr < groups_of(t, k); gr++)
                     for (uint32_t ch = 0; ch < t.nchunks[k]; ch++) w3.push_back(make_uint3(ti, ch, gr));
             }
@@ -639,13 +654,22 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
             HIP_TRY(ctx, hipStreamWaitEvent(ws, ctx->fork_ev, 0));
         }
         const Work2 *w2 = ctx->d_work2;
+        mars::launch_zero_u32(ctx->d_wq_count, (uint32_t)ctx->groups.size(), ws);
         for (size_t i = 0; i < ctx->groups.size(); i++) {
             const Group &g = ctx->groups[i];
             mars::launch_activate(ctx->d_tabs, w2 + g.activate.off, g.activate.n, ctx->walk_prepared,
-                                  ctx->params.min_size, ws);
+                                  ctx->params.min_size, g.d_wq, ctx->d_wq_count + i, ws);
             if (kernel == MARS_HIP_KERNEL_MFMA_I8)
                 mars::launch_seed(ctx->d_tabs, w2 + g.seed.off, g.seed.n, ctx->common, ws);
-            RC_TRY(enqueue_search(ctx, g, kernel, ws));
+            if (kernel == MARS_HIP_KERNEL_MFMA_I8) {
+                Common c = ctx->common;
+                c.sample = 1;
+                c.nwork = ctx->d_wq_count + i;
+                if (mars::launch_search(kernel, g.size, ctx->d_tabs, g.d_wq, g.wq_cap, c, ws) != 0)
+                    return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
+            } else {
+                RC_TRY(enqueue_search(ctx, g, kernel, ws));
+            }
             enqueue_merge(ctx, g, kernel, ws);
             if (ev) HIP_TRY(ctx, hipEventRecord(ev[2 + i], ws));
         }
@@ -905,6 +929,13 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
                 chunking(tiles_of(t, k), ng, kMaxTarget, kMinTiles, &t.nch_cap[k], &tpc);
                 // the ranked merge gives each chunk list one lane of a wave
                 if (ctx->topk > 1) t.nch_cap[k] = std::min<uint32_t>(t.nch_cap[k], 64);
+                if (ctx->walk_prepared > 0.0 && k == 0)
+                    t.nch_cap[0] = std::max(t.nch_cap[0], walk_chunks(tiles_of(t, 0)));
+            }
+            if (ctx->walk_prepared > 0.0 && k == 0) {
+                g.wq_cap = 0;
+                for (uint32_t ti : g.tables)
+                    g.wq_cap += groups_of(ctx->tables[ti], 0) * walk_chunks(tiles_of(ctx->tables[ti], 0));
             }
         }
 
@@ -953,6 +984,10 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
             plan.add(&t.d_ffound, (size_t)t.nr);
             plan.add(&t.d_fevals, sizeof(uint32_t) * (size_t)t.nr);
         }
+    }
+    if (ctx->walk_prepared > 0.0) {
+        for (Group &g : ctx->groups) plan.add(&g.d_wq, sizeof(Work3) * std::max<uint32_t>(g.wq_cap, 1));
+        plan.add(&ctx->d_wq_count, sizeof(uint32_t) * ctx->groups.size());
     }
     HIP_TRY(ctx, hipMalloc(&ctx->arena, plan.total));
     ctx->arena_bytes = plan.total;

@@ -99,6 +99,9 @@ struct Common {
     unsigned long long *counters; // [16]: [log2 s] stage-1 survivors, [8 + log2 s] stage-3 fits
     uint32_t sample;              // MFMA search: 1, or the tile stride of a threshold pre-pass
     uint32_t topk;                // candidates kept per range: 1, or a ranked table's k (<= 32)
+    // MFMA search over a list compacted on the device (walk-only batches):
+    // the number of valid work items; null for a static list.
+    const uint32_t *nwork;
 };
 
 // Work items: x = descriptor index, y/z = kernel-specific block coordinates.
@@ -116,8 +119,12 @@ void launch_rstats(const TableDesc *t, const Work2 *work, uint32_t nwork, hipStr
 // Walk-only tables, work item (table, block of 256 ranges): active[r] from
 // the parent's merged result (split when its rms > t_rms and its size >
 // min_size, the threshold walk's rule).
+// Also appends a search work item (table, chunk, range group) for every
+// chunk of every range group with a visited range to wq, counting in
+// *wq_count (zeroed by launch_zero_u32 beforehand).
 void launch_activate(const TableDesc *t, const Work2 *work, uint32_t nwork, double t_rms,
-                     uint32_t min_size, hipStream_t st);
+                     uint32_t min_size, Work3 *wq, uint32_t *wq_count, hipStream_t st);
+void launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 // Work item (table, first of four ranges).
 void launch_seed(const TableDesc *t, const Work2 *work, uint32_t nwork, const Common &c,
                  hipStream_t st);

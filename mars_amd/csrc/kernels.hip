@@ -404,13 +404,15 @@ __device__ __forceinline__ uint32_t tau_bits(double sum, int64_t n)
 // t_rms (no candidate: +inf) and its size exceeds min_size; a parent the walk
 // splits without searching (larger than the largest range size, or across
 // the border) always passes its children on.
-__global__ void k_activate(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work,
-                           double t_rms, uint32_t min_size)
+__global__ __launch_bounds__(256) void k_activate(const TableDesc *__restrict__ tabs,
+                                                  const Work2 *__restrict__ work, double t_rms,
+                                                  uint32_t min_size, Work3 *__restrict__ wq,
+                                                  uint32_t *__restrict__ wq_count)
 {
+    __shared__ uint8_t s_a[256];
     const Work2 wk = work[blockIdx.x];
     const TableG T = view(tabs[wk.x]);
     const uint32_t r = wk.y * 256 + threadIdx.x;
-    if (r >= T.nr_pad) return;
     uint8_t a = 0;
     if (r < T.nr) {
         const int32_t p = T.parent[r];
@@ -424,7 +426,28 @@ __global__ void k_activate(const TableDesc *__restrict__ tabs, const Work2 *__re
             a = U.active[p] && rms > t_rms && U.size > min_size;
         }
     }
-    T.active[r] = a;
+    if (r < T.nr_pad) T.active[r] = a;
+    s_a[threadIdx.x] = a;
+    __syncthreads();
+    // The block's 32 range groups of 8: each with a visited range gets all
+    // its chunks appended to the search's work list.
+    const uint32_t g = wk.y * 32 + threadIdx.x;
+    if (threadIdx.x < 32 && g * 8 < T.nr_pad && T.P > 0) {
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < 8; i++) any |= s_a[threadIdx.x * 8 + i] != 0;
+        if (any) {
+            const uint32_t nch = T.nchunks[0];
+            const uint32_t base = atomicAdd(wq_count, nch);
+            for (uint32_t ch = 0; ch < nch; ch++) wq[base + ch] = make_uint3(wk.x, ch, g);
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // Correlation t1_x4 of one candidate from its range row rk (i8 r - 128,
@@ -642,6 +665,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     constexpr bool INT_LIMIT = S <= 8; // every stage-1 quantity fits 32 bits
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5;
+    if (c.nwork && blockIdx.x >= *glb(c.nwork)) return; // past a compacted list's end
     const Work3 wk = work[blockIdx.x];
     const TableG T = view(tabs[wk.x]);
     const uint32_t chunk = wk.y, g = wk.z;
@@ -1547,9 +1571,16 @@ void launch_possums(const TableDesc *t, const Work2 *work, uint32_t n, hipStream
 }
 
 void launch_activate(const TableDesc *t, const Work2 *work, uint32_t n, double t_rms,
-                     uint32_t min_size, hipStream_t st)
+                     uint32_t min_size, Work3 *wq, uint32_t *wq_count, hipStream_t st)
 {
-    if (n) hipLaunchKernelGGL(k_activate, dim3(n), dim3(256), 0, st, t, work, t_rms, min_size);
+    if (n)
+        hipLaunchKernelGGL(k_activate, dim3(n), dim3(256), 0, st, t, work, t_rms, min_size, wq,
+                           wq_count);
+}
+
+void launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st)
+{
+    if (n) hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
 }
 
 void launch_seed(const TableDesc *t, const Work2 *work, uint32_t n, const Common &c,
