@@ -154,6 +154,23 @@ def test_parity_quadtree_512_sampled():
     _check_tables(img, ctx, "quad512", sample=192, seed=3)
 
 
+@pytest.mark.timeout(300)
+def test_parity_dense_2048_fixed16_sampled():
+    """The largest pool: 2048x2048 seed-7 plane, fixed 16x16 grid, stride 1
+    (16 384 ranges x 4 068 289 positions x 8); a sample of ranges against the
+    oracle, plus every range's winner a real position and the table sized."""
+    img = synth.plane(2048, 2048, synth.WORKLOAD_SEEDS["dense2048"])
+    print("dense2048 fixed 16x16 stride 1: GPU step", flush=True)
+    ctx = HipContext(0)
+    ctx.prepare(img, EncodeParams(min_size=16, max_size=16, shift=1), [1])
+    (size, shift, rows, cols, cands, P), = ctx.tables()
+    assert (size, shift, len(rows), P) == (16, 1, 16384, 4068289)
+    assert all(c is not None and c.dom_row <= 2048 - 32 and c.dom_col <= 2048 - 32 for c in cands)
+    print("dense2048: oracle sample", flush=True)
+    _check_tables(img, ctx, "dense2048", sample=6, seed=5)
+    ctx.close()
+
+
 def test_provider_lookup_and_errors():
     img = synth.plane(64, 64, 8)
     params = EncodeParams(min_size=4, max_size=16, shift=4)
