@@ -679,6 +679,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     }
 
     __shared__ __align__(16) uint32_t s_thr[8];
+    // Bumped whenever one of s_thr drops, so a wave checks one word per tile
+    // to know whether its stage-1 limits are stale.
+    __shared__ uint32_t s_epoch;
     __shared__ unsigned long long s_count[2];
     __shared__ int64_t s_T0[8], s_T2[8];
     __shared__ double s_A[8];
@@ -698,6 +701,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         uint32_t r = g * 8 + threadIdx.x;
         bool v = r < T.nr;
         s_thr[threadIdx.x] = v ? T.gthr[r] : 0x7f800000u;
+        if (threadIdx.x == 0) s_epoch = 0;
         int64_t t0 = v ? T.rstat[2 * r] : 0, t2 = v ? T.rstat[2 * r + 1] : 0;
         s_T0[threadIdx.x] = t0;
         s_T2[threadIdx.x] = t2;
@@ -742,12 +746,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // Stage-1 cache per range: sqrt(A - n tau) (negative when A - n tau <= 0)
     // and the threshold bits it was computed from.
     float sa[4];
-    uint32_t thc[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        sa[i] = -1.0f;
-        thc[i] = 0xffffffffu; // no threshold has these bits (NaN)
-    }
+    for (int i = 0; i < 4; i++) sa[i] = -1.0f;
+    uint32_t seen = 0xffffffffu; // s_epoch the limits were computed at (none yet)
+    // Lower a threshold, noting the drop for every wave.
+    auto lower = [&](uint32_t slot, uint32_t v) {
+        if (atomicMin(&s_thr[slot], v) > v) atomicAdd(&s_epoch, 1u);
+    };
 
     long a8[2][NK];
     v4i a16[2][NK];
@@ -854,14 +859,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                         L[pos] = TopRec{r_l, k_l, t_l, tau_bits(s_l, n)};
                         const uint32_t nn2 = nn < kk ? nn + 1 : kk;
                         s_topn[wave][sl_l] = nn2;
-                        if (nn2 == kk) atomicMin(&s_thr[sl_l], L[kk - 1].tau);
+                        if (nn2 == kk) lower(sl_l, L[kk - 1].tau);
                     }
                 }
             } else if (lane == 0 && better(r_l, k_l, wb[sl_l].rms, wb[sl_l].key)) {
                 wb[sl_l].rms = r_l;
                 wb[sl_l].key = k_l;
                 wb[sl_l].t1 = t_l;
-                atomicMin(&s_thr[sl_l], tau_bits(s_l, n));
+                lower(sl_l, tau_bits(s_l, n));
             }
         }
     };
@@ -918,13 +923,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // Exchange thresholds with the other blocks of this range group.
         if (exchange && (tcount & 15) == 15 && lane < 8) {
             uint32_t r = g * 8 + lane;
-            if (r < T.nr) atomicMin(&s_thr[lane], atomicMin((uint32_t *)&T.gthr[r], lds_load(&s_thr[lane])));
+            if (r < T.nr) lower(lane, atomicMin((uint32_t *)&T.gthr[r], lds_load(&s_thr[lane])));
         }
-        // The four thresholds of the lane's ranges, read before the MFMAs so
-        // the LDS latency overlaps them.
-        uint32_t thb[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) thb[i] = lds_load(&s_thr[4 * h + i]);
+        // Whether any threshold dropped since the limits were computed
+        // (read before the MFMAs so the LDS latency overlaps them).
+        const uint32_t ep = __builtin_amdgcn_readfirstlane(lds_load(&s_epoch));
         // (cur: this tile; nxt: filled for tile + 4).  Sizes with register
         // A fragments issue these loads after the first MFMAs: every load in
         // flight at an MFMA is then one the MFMA needs, so the compiler's
@@ -941,17 +944,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // INT_MIN, or INT_MAX for a padding slot
         const int32_t pmin = p < T.P ? (int32_t)0x80000000 : 0x7fffffff;
 
-        // Per-range sqrt(A - n tau), recomputed only when one of the wave's
-        // thresholds moved since the last tile (a wave-uniform branch).
+        // Per-range sqrt(A - n tau), recomputed only when a threshold dropped
+        // since the last time (a scalar branch).  Thresholds only drop, so
+        // one read a moment stale is a looser, still valid, limit.
         {
-            bool moved = false;
-#pragma unroll
-            for (int i = 0; i < 4; i++) moved |= thb[i] != thc[i];
-            if (__ballot(moved)) {
+            if (ep != seen) {
+                seen = ep;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    thc[i] = thb[i];
-                    const float thr = __uint_as_float(thb[i]);
+                    const float thr = __uint_as_float(lds_load(&s_thr[4 * h + i]));
                     float amt;
                     if constexpr (INT_LIMIT) {
                         amt = Alo[i] - (float)n * thr;
