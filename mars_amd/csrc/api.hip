@@ -1025,6 +1025,14 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         c.alfa_unit = p->max_alfa / std::ldexp(1.0, (int)p->bits_alfa);
         c.qa_max = (double)((1u << p->bits_alfa) - 1);
         c.qb_max = (double)((1u << p->bits_beta) - 1);
+        {
+            // Largest alfa2 of encode.rs:266 (qalfa <= qa_max), over 4,
+            // rounded up; 1 / (2 cap_u) rounded down and capped.
+            const double u = c.qa_max * c.alfa_unit * 0.25 * (1.0 + 1e-12);
+            c.cap_u = std::nextafter((float)u, INFINITY);
+            const double k = u > 0.0 ? 1.0 / (2.0 * (double)c.cap_u) : 8.0;
+            c.cap_k = std::nextafter((float)(k < 8.0 ? k : 8.0), 0.0f);
+        }
         const uint32_t nq = 1u << p->bits_alfa;
         std::vector<double> ys(nq);
         for (uint32_t q = 0; q < nq; q++) {

@@ -96,6 +96,10 @@ struct Common {
     double alfa_unit;     // max_alfa / 2^bits_alfa: alfa2 = qalfa * alfa_unit
     double qa_max, qb_max; // 2^bits_alfa - 1, 2^bits_beta - 1
     const double *yscale; // [2^bits_alfa] (2^bits_beta-1) / ((1 + alfa2) * 255)
+    // Stage-1 contrast cap: cap_u >= the largest quantised alfa / 4 (alfa
+    // applies to the domain's pixel average, D / 4), cap_k <= 1 / (2 cap_u),
+    // at most 8 so the integer limits of sizes <= 8 stay inside int32.
+    float cap_u, cap_k;
     unsigned long long *counters; // [16]: [log2 s] stage-1 survivors, [8 + log2 s] stage-3 fits
     uint32_t sample;              // MFMA search: 1, or the tile stride of a threshold pre-pass
     uint32_t topk;                // candidates kept per range: 1, or a ranked table's k (<= 32)

@@ -630,6 +630,10 @@ template <int S> struct Queue {
 // reached, inflated so that anything above it has a strictly larger binary64
 // rms), a candidate can only win if Ni >= sqrt((A - n tau) Di), i.e.
 //   t1_x4 > floor((S1*t0 + sqrt((A - n tau) Di)) / n) - 1.
+// Sizes <= 4 also use the cap on the slope: alfa2 <= qa_max * alfa_unit,
+// so with u = alfa2/4 <= U, n*SSE >= A - 2u*Ni + u^2*Di and, where
+// A - n tau > U^2 Di, the bound rises to the tangent (A - n tau + U^2 Di)/(2U)
+// (about a fifth of the plain bound's survivors at s = 4 remain).
 // The limit is computed once per (range, position) from the tile's four
 // thresholds (one LDS read issued before the MFMAs) and the position's
 // sqrt(Di), rounded so that it never excludes more than the exact bound;
@@ -663,6 +667,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     constexpr int LOG2S = S == 2 ? 1 : S == 4 ? 2 : S == 8 ? 3 : S == 16 ? 4 : 5;
     constexpr int LOG2N = 2 * LOG2S;
     constexpr bool INT_LIMIT = S <= 8; // every stage-1 quantity fits 32 bits
+    // Sizes <= 4 also bound stage 1 by the contrast cap: most of their
+    // Cauchy-Schwarz survivors sit on low-variance domains that only an
+    // alfa above the largest quantised one could fit.
+    constexpr bool CAP_LIMIT = S <= 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = lane >> 5;
     if (c.nwork && blockIdx.x >= *glb(c.nwork)) return; // past a compacted list's end
@@ -972,14 +980,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                 }
             }
         }
-        // Stage-1 limit per (range, position) on v = 4*hi + lo.
+        // Stage-1 limit per (range, position) on v = 4*hi + lo.  With
+        // X = n t1 - S1 t0, a = A - n tau and u = alfa / 4 <= U (cap_u), a
+        // candidate's SSE is at least (A - 2 u X + u^2 Di) / n, so it can
+        // reach tau only if X >= sqrt(a Di) when a <= U^2 Di, else X >=
+        // (a + U^2 Di) / (2U) = sqrt(a Di) + (sqrt(a) - U sqrt(Di))^2 / (2U).
+        // nt is that bound from sa <= sqrt(a) and sqDi <= sqrt(Di); it rises
+        // with both, and the (1 - 2e-5) in sa covers the rounding here.
         int32_t lim[4];
         const int32_t o128 = (int32_t)((uint32_t)S1p << 7);
+        const float usq = CAP_LIMIT ? c.cap_u * sqDi : 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const float nt = fabsf(sa[i]) * sqDi; // <= sqrt((A - n tau) Di) when sa >= 0
+            float nt = fabsf(sa[i]) * sqDi; // <= sqrt((A - n tau) Di) when sa >= 0
+            if constexpr (CAP_LIMIT) {
+                const float d = fmaxf(fabsf(sa[i]) - usq, 0.0f);
+                nt = fmaf(d * d, c.cap_k, nt);
+            }
             if constexpr (INT_LIMIT) {
-                // S1*t0 + nt < 2^31 for sizes <= 8, S1 and t0 < 2^24.
+                // S1*t0 + nt < 2^31 for sizes <= 8 (nt <= sqrt(A Di) + 8 A),
+                // S1 and t0 < 2^24.
                 const uint32_t X = __umul24((uint32_t)S1p, (uint32_t)T0[i]) + (uint32_t)nt;
                 lim[i] = (int32_t)(X >> LOG2N) - 1 - (offx[i] + o128);
             } else {
