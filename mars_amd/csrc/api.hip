@@ -62,6 +62,16 @@ constexpr uint32_t kMaxTarget = 16384, kMinTiles = 4; // bounds of the autotuner
 // a list of every group could afford).
 constexpr uint32_t kWalkChunks = 8;
 uint32_t walk_chunks(uint32_t ntiles) { return std::max<uint32_t>(1, std::min(ntiles, kWalkChunks)); }
+// Sizes whose MFMA search first runs a threshold pre-pass over every
+// kPrepassStride-th position tile: their stage-1 bound lets many candidates
+// through until the per-range thresholds are close to final, so tightening
+// them on a sample first cuts the exact fits of the full pass.  The pre-pass
+// runs one block per range group (all of the group's sampled tiles), so the
+// group starts cold once, not once per position chunk; the launch is small
+// and overlaps the other sizes' searches in the step.  (Not for walk-only
+// batches, whose few searched groups the pre-pass would delay more than it
+// saves.)
+constexpr uint32_t kPrepassMinSize = 16, kPrepassStride = 8;
 
 struct Plane {
     uint32_t w = 0, h = 0, wc = 0, hc = 0, pitch = 0;
@@ -127,6 +137,7 @@ struct Group {
     uint32_t size = 0, shift = 0;
     std::vector<uint32_t> tables; // longest searches first
     Slice search[2], merge[2];    // per kernel
+    Slice prepass;                // MFMA threshold pre-pass: (table, 0, range group)
     Slice merge_ranked;           // (table, range) items of the ranked merge
     Slice seed, activate;         // walk-only batches: (table, 4 ranges), (table, 256 ranges)
     // Walk-only batches: the MFMA search's work list, compacted on the device
@@ -513,6 +524,16 @@ int layout(mars_hip_ctx *ctx)
                     for (uint32_t ch = 0; ch < t.nchunks[k]; ch++) w3.push_back(make_uint3(ti, ch, gr));
             }
             g.search[k].n = (uint32_t)w3.size() - g.search[k].off;
+            if (k == 0) {
+                g.prepass.off = (uint32_t)w3.size();
+                if (g.size >= kPrepassMinSize && ctx->walk_prepared <= 0.0)
+                    for (uint32_t ti : g.tables) {
+                        const Table &t = ctx->tables[ti];
+                        if (t.P == 0) continue;
+                        for (uint32_t gr = 0; gr < groups_of(t, 0); gr++) w3.push_back(make_uint3(ti, 0, gr));
+                    }
+                g.prepass.n = (uint32_t)w3.size() - g.prepass.off;
+            }
             begin2(g.merge[k]);
             for (uint32_t ti : g.tables) {
                 const Table &t = ctx->tables[ti];
@@ -581,24 +602,16 @@ void enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st)
         mars::launch_seed(ctx->d_tabs, w2 + ctx->wl_seed.off, ctx->wl_seed.n, ctx->common, st);
 }
 
-// Sizes whose MFMA search first runs a threshold pre-pass over every
-// kPrepassStride-th position tile: their stage-1 bound lets many candidates
-// through until the per-range thresholds are close to final, so tightening
-// them on a sample first cuts the exact fits of the full pass.  (Not for
-// walk-only batches, whose few searched groups the pre-pass would delay more
-// than it saves.)
-constexpr uint32_t kPrepassMinSize = 16, kPrepassStride = 8;
-
 int enqueue_search(mars_hip_ctx *ctx, const Group &g, int kernel, hipStream_t st)
 {
     const int k = kernel == MARS_HIP_KERNEL_MFMA_I8 ? 0 : 1;
     Common c = ctx->common;
     c.sample = 1;
-    if (k == 0 && g.size >= kPrepassMinSize && ctx->walk_prepared <= 0.0) {
+    if (k == 0 && g.prepass.n > 0) {
         Common pre = c;
         pre.sample = kPrepassStride;
-        if (mars::launch_search(kernel, g.size, ctx->d_tabs, ctx->d_work3 + g.search[k].off,
-                                g.search[k].n, pre, st) != 0)
+        if (mars::launch_search(kernel, g.size, ctx->d_tabs, ctx->d_work3 + g.prepass.off,
+                                g.prepass.n, pre, st) != 0)
             return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
     }
     if (mars::launch_search(kernel, g.size, ctx->d_tabs, ctx->d_work3 + g.search[k].off,

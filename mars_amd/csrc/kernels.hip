@@ -795,11 +795,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     const int arow0 = (((am >> 2) & 1) * 4 + (am & 3)) * 8 + (am >> 3);
 
     // With c.sample > 1 (threshold pre-pass) the block covers every
-    // sample-th tile: tile index i below stands for tile i * sample.
+    // sample-th tile of all of its group's positions (whatever its chunk):
+    // tile index i below stands for tile i * sample.
     const uint32_t sample = c.sample;
+    const bool whole = sample > 1;
     const uint32_t ntiles = (T.P_pad / 32 + sample - 1) / sample;
-    const uint32_t tpc = (tiles_per_chunk + sample - 1) / sample;
-    const uint32_t t_begin = chunk * tpc;
+    const uint32_t tpc = whole ? ntiles : tiles_per_chunk;
+    const uint32_t t_begin = whole ? 0 : chunk * tpc;
     uint32_t t_end = t_begin + tpc;
     if (t_end > ntiles) t_end = ntiles;
     unsigned long long qtotal = 0, stage3 = 0; // stage-1 survivors (wave), stage-3 fits (lane)
@@ -921,7 +923,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
             }
         }
     }
-    const bool exchange = nchunks > 1;
+    const bool exchange = !whole && nchunks > 1;
 
     // One tile: `cur` holds its window fragments and position record, the
     // next tile's are fetched into `nxt`.  The loop below runs two tiles per
