@@ -15,7 +15,8 @@
 //   k_fisher_search   one wave per range, one lane per candidate of the
 //                     range's bucket, isometry MAPPING[range iso][domain iso]
 //                     (fisher.rs:62-75): exact integer t1 by i8 dot products
-//                     on the split planes, the verbatim binary64 fit, the
+//                     on the split planes, the verbatim binary64 fit (of
+//                     the candidates a least-squares bound cannot rule out), the
 //                     first strict minimum of rms in bucket order kept
 //                     (lib.rs:254-286).
 //
@@ -209,7 +210,9 @@ __device__ __forceinline__ void window_dots(const GLB int8_t *hi, const GLB int8
 // range's bucket.  Per candidate the search-order record gives the window
 // offset, S1 and the position index; t1 is exact from the split planes:
 //   t1 = 4 sum r'hi + sum r'lo + 512 sum r' + 128 S1;
-// then the reference's binary64 fit, and the minimum by (rms, position).
+// then the reference's binary64 fit, skipped when a certified lower bound of
+// the candidate's SSE exceeds the wave's best (shared after every round of
+// 64), and the minimum by (rms, position).
 template <uint32_t S>
 __device__ __forceinline__ void fisher_range(const TableG &T, uint32_t r, uint32_t lane,
                                              const Common &cm)
@@ -220,29 +223,66 @@ __device__ __forceinline__ void fisher_range(const TableG &T, uint32_t r, uint32
     const int64_t t0 = T.rstat[2 * r], t2 = T.rstat[2 * r + 1];
     const int32_t off_r = 512 * (int32_t)(t0 - 128 * (int64_t)K); // 512 sum r'
     const GLB int8_t *rng = T.rng8 + (size_t)r * 8 * T.kmem;
-    double best = __longlong_as_double(0x7ff0000000000000ll);
+    const double inf = __longlong_as_double(0x7ff0000000000000ll);
+    double best = inf;
     uint32_t bp = 0xffffffffu, bk = 0;
     int32_t bt1 = 0, bs1 = 0, bs2 = 0;
-    for (uint32_t c = lane; c < cnt; c += 64) {
+    // The wave's best SSE so far, inflated so that a candidate whose exact
+    // SSE exceeds it has a strictly larger binary64 rms (relative 1e-9 and
+    // absolute 1e-6 n cover the rounding of the fit's sum, /s0 and sqrt).
+    double tau = inf;
+    const double Ad = (double)((int64_t)K * t2 - t0 * t0); // A = n t2 - t0^2, exact
+    struct Cand {
+        uint32_t p, k;
+        int32_t t1, s1, s2;
+    };
+    // Exact t1 of candidate c of the bucket from the split planes.
+    auto gather = [&](uint32_t c) {
         const uint32_t e = T.flist[start + c];
         const uint32_t k = fisher::kMapping[isom][e & 7];
         const uint4 rec = ld_u4(T.posinfo, e >> 3); // {window offset, S1, -, p}
-        const uint32_t p = rec.w;
         int32_t ah = 0, al = 0;
         window_dots<S>(T.hi + rec.x, T.lo + rec.x, T.pitch, rng + (size_t)k * T.kmem, ah, al);
-        const int32_t s1 = (int32_t)rec.y, s2 = T.posx[2 * p + 1];
-        const int32_t t1 = 4 * ah + al + off_r + 128 * s1;
+        const int32_t s1 = (int32_t)rec.y;
+        return Cand{rec.w, k, 4 * ah + al + off_r + 128 * s1, s1, T.posx[2 * rec.w + 1]};
+    };
+    auto consider = [&](const Cand &x) {
+        // Free least-squares bound with alfa2 >= 0 (encode.rs:261-267):
+        // n SSE >= A - max(Ni, 0)^2 / Di, Ni = n t1 - S1 t0, Di = n S2 - S1^2
+        // (x4 / x16 units).  The candidate is skipped when that exceeds
+        // n tau: A Di - Ni+^2 > n tau Di, compared in binary64 with a margin
+        // of 1e-12 A Di, far above the products' rounding (5.5e-16 A Di);
+        // Ni^2 <= A Di, so a right side above A Di never skips.
+        const int64_t Ni = (int64_t)K * x.t1 - (int64_t)x.s1 * t0;
+        const int64_t Di = (int64_t)K * x.s2 - (int64_t)x.s1 * x.s1;
+        if (tau < inf && Di > 0) {
+            const double Dd = (double)Di, Np = Ni > 0 ? (double)Ni : 0.0;
+            const double ad = Ad * Dd;
+            if (ad - Np * Np > (double)K * tau * Dd + 1e-12 * ad) return;
+        }
         uint32_t qa, qb;
         double rms, sum;
-        fit_f64((int64_t)K, s1, s2, t0, t1, t2, cm.max_alfa, cm.bits_alfa, cm.bits_beta, &qa,
+        fit_f64((int64_t)K, x.s1, x.s2, t0, x.t1, t2, cm.max_alfa, cm.bits_alfa, cm.bits_beta, &qa,
                 &qb, &rms, &sum);
-        if (rms < best || (rms == best && p < bp)) {
+        if (rms < best || (rms == best && x.p < bp)) {
             best = rms;
-            bp = p;
-            bk = k;
-            bt1 = t1;
-            bs1 = s1;
-            bs2 = s2;
+            bp = x.p;
+            bk = x.k;
+            bt1 = x.t1;
+            bs1 = x.s1;
+            bs2 = x.s2;
+            const double tb = (sum > 0.0 ? sum : 0.0) * (1.0 + 1e-9) + 1e-6 * (double)K;
+            tau = tb < tau ? tb : tau;
+        }
+    };
+    for (uint32_t base = 0; base < cnt; base += 64) {
+        if (base + lane < cnt) consider(gather(base + lane));
+        if (base + 64 < cnt) { // share the wave's best before the next round
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double o = __shfl_xor(tau, off, 64);
+                tau = o < tau ? o : tau;
+            }
         }
     }
 #pragma unroll
