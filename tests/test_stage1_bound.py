@@ -13,6 +13,11 @@ from its own inflated SSE (the tightest threshold it can ever meet), with the
 kernel's operations and roundings.  Candidates whose quantised slope sits at
 the cap come within a few units of the limit, so the same check with a cap
 10% too small (an over-pruning bound) must fail.
+
+k_fisher_search skips a bucket candidate's binary64 fit when the free
+least-squares bound, A Di - max(Ni, 0)^2 > n tau Di (+ a 1e-12 A Di margin),
+rules it out against the wave's best inflated SSE; the same own-threshold
+check applies, and dropping the n factor must be caught.
 """
 
 import numpy as np
@@ -108,3 +113,28 @@ def test_every_candidate_passes_its_own_threshold(size, seed, max_alfa, bits_alf
     assert n_cand > 10000 and n_capped > 100, (n_cand, n_capped)
     assert tight > 0, "no candidate came near the capped limit"
     assert misses_bad > 0, "an over-pruning cap went unnoticed"
+
+
+def fisher_skips(m, rms, n_factor=True):
+    """k_fisher_search's test: skip the fit when A Di - Ni+^2 > n tau Di + 1e-12 A Di."""
+    n, S1, S2, t0, t1, t2 = m
+    tau = max(rms * rms * n, 0.0) * (1.0 + 1e-9) + 1e-6 * n
+    A = n * t2 - t0 * t0
+    Ni, Di = n * t1 - S1 * t0, n * S2 - S1 * S1
+    if Di <= 0:
+        return False
+    ad = float(A) * float(Di)
+    np_ = float(Ni) if Ni > 0 else 0.0
+    return ad - np_ * np_ > (float(n) if n_factor else 1.0) * tau * float(Di) + 1e-12 * ad
+
+
+@pytest.mark.parametrize("size,seed", [(4, 3), (8, 4), (2, 5), (16, 6)])
+def test_fisher_bound_never_skips_a_candidate_at_its_own_threshold(size, seed):
+    n_cand = caught = 0
+    for m in candidates(size, seed):
+        _, _, rms = O.fit_f64(m, 1.0, 4, 7)
+        assert not fisher_skips(m, rms), m
+        caught += fisher_skips(m, rms, n_factor=False)
+        n_cand += 1
+    assert n_cand >= 100
+    assert caught > 0, "a bound without the n factor went unnoticed"
