@@ -153,6 +153,7 @@ struct mars_hip_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side[kSideStreams] = {}; // default priority, like `stream`
     hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
+    hipEvent_t prep_fork_ev = nullptr, prep_join_ev = nullptr; // range operands beside the pool
     mutable std::mutex err_mu;
     std::string err;
     int kernel = MARS_HIP_KERNEL_MFMA_I8;
@@ -587,19 +588,32 @@ int layout(mars_hip_ctx *ctx)
 }
 
 // Pool and range preparation of every plane and table.
-void enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st)
+// The step's preparation.  The range operands and sums read only the
+// pixels, so with `fork` they run on a side stream beside the domain pool
+// (contraction, summed-area tables, position sums) and join before the seed,
+// which needs both; without, everything runs in order on st.
+int enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st, bool fork)
 {
     const Work2 *w2 = ctx->d_work2;
+    hipStream_t rs = st;
+    if (fork) {
+        rs = ctx->side[kSideStreams - 1];
+        HIP_TRY(ctx, hipEventRecord(ctx->prep_fork_ev, st));
+        HIP_TRY(ctx, hipStreamWaitEvent(rs, ctx->prep_fork_ev, 0));
+    }
+    mars::launch_ranges(ctx->d_tabs, w2 + ctx->wl_ranges.off, ctx->wl_ranges.n,
+                        kernel == MARS_HIP_KERNEL_FP32, rs);
+    mars::launch_rstats(ctx->d_tabs, w2 + ctx->wl_rstats.off, ctx->wl_rstats.n, rs);
+    if (fork) HIP_TRY(ctx, hipEventRecord(ctx->prep_join_ev, rs));
     mars::launch_contract(ctx->d_planes, w2 + ctx->wl_contract.off, ctx->wl_contract.n,
                           ctx->d_counters, kCounters, st);
     mars::launch_sat(ctx->d_planes, w2 + ctx->wl_sat_rows.off, ctx->wl_sat_rows.n,
                      w2 + ctx->wl_sat_cols.off, ctx->wl_sat_cols.n, st);
     mars::launch_possums(ctx->d_tabs, w2 + ctx->wl_possums.off, ctx->wl_possums.n, st);
-    mars::launch_ranges(ctx->d_tabs, w2 + ctx->wl_ranges.off, ctx->wl_ranges.n,
-                        kernel == MARS_HIP_KERNEL_FP32, st);
-    mars::launch_rstats(ctx->d_tabs, w2 + ctx->wl_rstats.off, ctx->wl_rstats.n, st);
+    if (fork) HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->prep_join_ev, 0));
     if (kernel == MARS_HIP_KERNEL_MFMA_I8 && ctx->walk_prepared <= 0.0)
         mars::launch_seed(ctx->d_tabs, w2 + ctx->wl_seed.off, ctx->wl_seed.n, ctx->common, st);
+    return MARS_HIP_OK;
 }
 
 int enqueue_search(mars_hip_ctx *ctx, const Group &g, int kernel, hipStream_t st)
@@ -651,7 +665,7 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
 {
     hipStream_t st = ctx->stream;
     if (ev) HIP_TRY(ctx, hipEventRecord(ev[0], st));
-    enqueue_prep(ctx, kernel, st);
+    RC_TRY(enqueue_prep(ctx, kernel, st, ev == nullptr));
     if (ctx->walk_prepared > 0.0) {
         // Walk-only batch: the sizes in turn, largest first, each group's
         // ranges activated from its parent group's merged winners, then
@@ -1147,7 +1161,9 @@ int mars_hip_open(int device, mars_hip_ctx **out)
     bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreate(&ctx->stream) == hipSuccess &&
               hipMalloc((void **)&ctx->d_counters, sizeof(unsigned long long) * kCounters) ==
                   hipSuccess &&
-              hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess;
+              hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&ctx->prep_fork_ev, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&ctx->prep_join_ev, hipEventDisableTiming) == hipSuccess;
     for (int j = 0; ok && j < kSideStreams; j++)
         ok = hipStreamCreate(&ctx->side[j]) == hipSuccess &&
              hipEventCreateWithFlags(&ctx->join_ev[j], hipEventDisableTiming) == hipSuccess;
@@ -1173,6 +1189,8 @@ void mars_hip_close(mars_hip_ctx *ctx)
         if (ctx->join_ev[j]) (void)hipEventDestroy(ctx->join_ev[j]);
     }
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->prep_fork_ev) (void)hipEventDestroy(ctx->prep_fork_ev);
+    if (ctx->prep_join_ev) (void)hipEventDestroy(ctx->prep_join_ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
