@@ -387,8 +387,8 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share of the search; --gpus {args.gpus} is not "
-              "supported (the search shards by range block with no exchange but a per-range "
-              "winner gather; see the multi-GPU paragraph of DESIGN.md)", file=sys.stderr)
+              "supported: it times the search on one GPU (the search would shard by range block "
+              "with no exchange but a per-range winner gather)", file=sys.stderr)
         return 2
 
     img, res = _gpu_leg(args.steps, args.warmup)

@@ -13,7 +13,8 @@
  * on the GPU), `mars_hip_search` is SearchProvider::search (a read-only,
  * thread-safe table lookup) of the prepared method -- the exhaustive search,
  * or IndexedSearchProvider with the Fisher retriever
- * (crates/mars-search/src/lib.rs:468-552).  The binding is shown in INTEGRATION.md.
+ * (crates/mars-search/src/lib.rs:468-552).  The binding is one extern block
+ * over these entry points, each taking plain pointers and sizes.
  *
  * Conventions: status 0 = ok, < 0 = error (mars_hip_last_error() says why).
  * Nothing unwinds across the ABI.  No CPU search fallback exists: a caller
