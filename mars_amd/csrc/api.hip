@@ -714,7 +714,9 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
         HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, st));
         const int nside = std::min<int>(kSideStreams, (int)ctx->groups.size());
         for (int j = 0; j < nside; j++) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[j], ctx->fork_ev, 0));
-        for (size_t i = 0; i < ctx->groups.size(); i++) {
+        // Smallest size (the most work) first, so its blocks are dispatched
+        // ahead of the shorter chains'.
+        for (size_t i = ctx->groups.size(); i-- > 0;) {
             hipStream_t ss = ctx->side[i % kSideStreams];
             RC_TRY(enqueue_search(ctx, ctx->groups[i], kernel, ss));
             enqueue_merge(ctx, ctx->groups[i], kernel, ss);
