@@ -1260,16 +1260,26 @@ int mars_hip_run(mars_hip_ctx *ctx, int iters, int use_graph, mars_hip_timing *t
         int rc = enqueue_step(ctx, ctx->kernel, nullptr);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-        if (rc) return rc;
-        if (e != hipSuccess) return fail(ctx, MARS_HIP_E_DEVICE, "graph capture failed");
-        ctx->graph = g;
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : fail(ctx, MARS_HIP_E_DEVICE, "graph capture failed");
+        }
+        ctx->graph = g; // owned by ctx from here: drop_graph() frees it on any later error
         HIP_TRY(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
         ctx->graph_kernel = ctx->kernel;
     }
-    hipEvent_t e0, e1;
-    HIP_TRY(ctx, hipEventCreate(&e0));
-    HIP_TRY(ctx, hipEventCreate(&e1));
-    HIP_TRY(ctx, hipEventRecord(e0, ctx->stream));
+    // The two timing events are released on every return path.
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    HIP_TRY(ctx, hipEventCreate(&ev.e[0]));
+    HIP_TRY(ctx, hipEventCreate(&ev.e[1]));
+    HIP_TRY(ctx, hipEventRecord(ev.e[0], ctx->stream));
     for (int it = 0; it < iters; it++) {
         if (use_graph) {
             HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
@@ -1277,13 +1287,11 @@ int mars_hip_run(mars_hip_ctx *ctx, int iters, int use_graph, mars_hip_timing *t
             RC_TRY(enqueue_step(ctx, ctx->kernel, nullptr));
         }
     }
-    HIP_TRY(ctx, hipEventRecord(e1, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ev.e[1], ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0;
-    HIP_TRY(ctx, hipEventElapsedTime(&ms, e0, e1));
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
     tm.ms_step = ms / iters;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     RC_TRY(download(ctx));
     if (timing) *timing = tm;
     return MARS_HIP_OK;

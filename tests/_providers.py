@@ -66,3 +66,35 @@ def oracle_cache(img, config):
         return memo[size][(row, col)]
 
     return OC.assemble(img, "synthetic", config, lookup)
+
+
+def _bits(x: float) -> int:
+    return C.c_int64.from_buffer_copy(C.c_double(x)).value
+
+
+def candidate_key(c):
+    """Every field of a winner: codes, rms bits and the six RawMoments."""
+    if c is None:
+        return None
+    m = c.m if hasattr(c, "m") else c  # oracle candidates keep RawMoments in .m
+    return (c.dom_row, c.dom_col, c.isometry, c.qalfa, c.qbeta, _bits(c.rms), m.s0, m.s1_x4,
+            m.s2_x16, m.t0, m.t1_x4, m.t2)
+
+
+def assert_tables_equal(tables, provider: TableProvider, label: str) -> int:
+    """Every (size, shift) table of a context (HipContext.tables() rows) equals the
+    oracle table a TableProvider computed, range by range; returns the ranges
+    compared.  Every node the provider holds must be in the context's tables."""
+    seen = set()
+    n = 0
+    for size, shift, rows, cols, cands, _ in tables:
+        for r, c, got in zip(rows, cols, cands):
+            ref, _ = provider.table[(r, c, size, shift)]
+            want = ref[0] if ref else None
+            assert candidate_key(got) == candidate_key(want), \
+                f"{label}: size {size} shift {shift} range ({r},{c}): " \
+                f"{candidate_key(got)} != {candidate_key(want)}"
+            seen.add((r, c, size, shift))
+            n += 1
+    assert seen == set(provider.table), f"{label}: the context misses oracle nodes"
+    return n

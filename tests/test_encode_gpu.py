@@ -14,16 +14,24 @@ import pytest
 from mars_amd import synth
 from mars_amd.codec import EncodeOptions, encode_plane, hip_encode_plane
 from mars_amd.provider import EncodeParams, HipContext
-from tests._providers import TableProvider
+from tests._providers import TableProvider, assert_tables_equal
 
 pytestmark = pytest.mark.gpu
 
 
-def _both(img, params, options, shifts):
+def _both(img, params, options, shifts, check_tables=True):
+    """Encode with the oracle's table and with the GPU table.  With
+    check_tables, every winner of the GPU table the encode read must also
+    equal the oracle's (a wrong winner that does not change a split or a
+    mode decision would leave the bytes alone)."""
     prov = TableProvider(img, params, shifts)
     ref, ref_st = encode_plane(img, params, options, prov.fn, None)
     ctx = HipContext(0)
     got, st = hip_encode_plane(ctx, img, params, options)
+    if check_tables:
+        n = assert_tables_equal(ctx.tables(), prov, f"{img.shape[1]}x{img.shape[0]}")
+        print(f"  {n} table winners equal the oracle's", flush=True)
+    ctx.close()
     return ref, ref_st, got, st
 
 

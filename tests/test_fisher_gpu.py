@@ -19,7 +19,7 @@ from mars_amd import codec, synth
 from mars_amd.codec import EncodeOptions, encode_plane, hip_encode_plane_method, hip_encode_rgb
 from mars_amd.provider import EncodeParams, HipContext
 from tests import _oracle as O
-from tests._providers import TableProvider
+from tests._providers import TableProvider, assert_tables_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -100,11 +100,13 @@ def test_fisher_batch_equals_single_planes():
     batch.close()
 
 
-def _oracle_encode(img, p, o, shifts):
+def _oracle_encode(img, p, o, shifts, keep=None):
     fp = O.FisherProvider(img, p.max_alfa, p.bits_alfa, p.bits_beta)
     warm = TableProvider(img, p, shifts) if p.lambda_ is not None else None
     data, st = encode_plane(img, p, o, fp.fn, fp.user, warm.fn if warm else None, None)
     fp.close()
+    if keep is not None:
+        keep.append((img, p, warm))
     return data, st
 
 
@@ -127,14 +129,29 @@ def test_fisher_encode_equals_the_oracle_chain(lam, modes, density, tr):
     assert got == ref
 
 
-def _oracle_rgb(rgb, yp, cp, o):
+def _oracle_rgb(rgb, yp, cp, o, keep=None):
     y, cb, cr = O.ycbcr(rgb)
     streams, stats = [], []
     for plane, p in ((y, yp), (O.downsample_box(cb), cp), (O.downsample_box(cr), cp)):
-        data, st = _oracle_encode(plane, p, o, [p.shift])
+        data, st = _oracle_encode(plane, p, o, [p.shift], keep)
         streams.append(data)
         stats.append(st)
     return codec.write_container(codec.COLOR_RGB_420, streams), stats
+
+
+def _check_every_winner(planes):
+    """Every winner of both methods on every plane of a colour encode: the
+    exhaustive tables (full, not the walk-only ones the encode reads) against
+    the oracle tables the warm-up provider computed, the Fisher tables against
+    the oracle's search_block_fitted.  The bytes alone would miss a wrong
+    winner that changes no decision."""
+    for name, (img, p, warm) in zip(("y", "cb", "cr"), planes):
+        ctx = HipContext(0)
+        ctx.prepare(img, p, [p.shift], method=FISHER)
+        n = assert_tables_equal(ctx.tables(), warm, name)
+        f = _check_tables(ctx, img)
+        ctx.close()
+        print(f"  {name}: {n} exhaustive and {f} Fisher winners equal the oracle's", flush=True)
 
 
 @pytest.mark.parametrize("w,h", [
@@ -150,7 +167,8 @@ def test_rgb420_fisher_encode_equals_the_oracle_chain(w, h):
     cp = EncodeParams(min_size=4, max_size=16, shift=4, lambda_=200.0)
     o = EncodeOptions(allowed_modes=(True, False, True, False))
     print(f"rgb {w}x{h}: oracle chain", flush=True)
-    ref, ref_stats = _oracle_rgb(rgb, yp, cp, o)
+    planes = []
+    ref, ref_stats = _oracle_rgb(rgb, yp, cp, o, planes)
     print("rgb: GPU", flush=True)
     ctx = HipContext(0)
     got, stats, nbytes = hip_encode_rgb(ctx, rgb, yp, cp, o, FISHER)
@@ -161,6 +179,7 @@ def test_rgb420_fisher_encode_equals_the_oracle_chain(w, h):
         assert (stats[k].search_evals, stats[k].warmup_evals) == (rs.search_evals,
                                                                   rs.warmup_evals), k
     assert got == ref
+    _check_every_winner(planes)
 
 
 def test_rgb444_exhaustive_equals_the_oracle_chain():

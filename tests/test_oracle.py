@@ -6,6 +6,10 @@ What fixes each oracle function, independently of its own code:
                      arithmetic (fractions) for qalfa / qbeta / SSE on random
                      moments -- a dropped term, a wrong sign or swapped s1/t0
                      fails those.
+  * fit_f32          exact rational arithmetic with per-operation binary32
+                     error bounds at s <= 4 (codes away from rounding
+                     boundaries, SSE of the returned codes); planted mistakes
+                     (a dropped SSE term, s1*t1 for s1*t0) fail it.
   * contract / domain_sums / cross_term / permute_range
                      the reference's own exactness construction
                      (encode.rs:2101-2205: integer moments == the mean-based
@@ -123,6 +127,61 @@ def test_fit_f64_against_exact_rationals(size):
         assert abs(rms * rms * n - float(psse)) <= 1e-7 * max(1.0, float(psse)) + 1e-6
         checked += 1
     assert checked == 300
+
+
+@pytest.mark.parametrize("size", [2, 4])
+def test_fit_f32_against_exact_rationals(size):
+    """or_fit_f32 (encode.rs:295-337, the FP32 baseline's fit) against exact
+    rationals.  At s <= 4 every moment is an integer below 2^24, exact in
+    binary32, so the only errors are the fit's own roundings; each is bounded
+    here from the operands (unit roundoff 2^-24 per operation, with margin).
+    Codes must be the exact ones wherever the exact rational lies farther
+    from a rounding boundary than that bound, and s0 * rms^2 must be the exact
+    SSE of the returned codes within the bound of the `sum` expression -- a
+    wrong alfa branch, a swapped operand or a dropped term is off by orders of
+    magnitude more."""
+    u = 2.0 ** -24
+    rng = np.random.default_rng(200 + size)
+    n = size * size
+    codes_checked = 0
+    for trial in range(600):
+        rr = rng.integers(0, 256, size=(size, size))
+        dd = rng.integers(0, 1021, size=(size, size))
+        if trial % 3 == 1:
+            a = rng.integers(1, 16) / 16.0
+            rr = np.clip(np.rint(a * dd / 4.0 + rng.integers(-20, 60)
+                                 + rng.normal(0, 2, size=(size, size))), 0, 255).astype(int)
+        elif trial % 3 == 2:
+            rr = np.full((size, size), int(rng.integers(0, 256)))
+        m = (n, int(dd.sum()), int((dd.astype(np.int64) ** 2).sum()), int(rr.sum()),
+             int((rr.astype(np.int64) * dd).sum()), int((rr.astype(np.int64) ** 2).sum()))
+        assert max(m) < 2 ** 24
+        qa, qb, rms = O.fit_f32(m)
+        eqa, eqb, _, x, y = _exact_fit(m)
+        s0, s1, s2, t0, t1, t2 = (m[0], m[1] / 4, m[2] / 16, m[3], m[4] / 4, m[5])
+        num, den = s0 * t1 - s1 * t0, s0 * s2 - s1 * s1
+        if den != 0 and num > 0:
+            ex = abs(float(x)) * 8 * u * ((abs(s0 * t1) + abs(s1 * t0)) / abs(num)
+                                          + (s0 * s2 + s1 * s1) / abs(den) + 2)
+            if abs(float(x) - (np.floor(float(x)) + 0.5)) > 8 * ex + 1e-9:
+                assert qa == eqa, (m, qa, eqa)
+                codes_checked += 1
+        elif num <= 0 or den == 0:
+            assert qa == 0, m
+        if qa == eqa:
+            a2 = qa / 16.0
+            eb = 8 * u * ((abs(t0) + abs(a2 * s1)) / s0 + a2 * 255 + abs(float(y)))
+            ey = eb / ((1 + a2) * 255) * 127 + 8 * u * abs(float(y))
+            if abs(float(y) - (np.floor(float(y)) + 0.5)) > 8 * ey + 1e-9:
+                assert qb == eqb, (m, qb, eqb)
+        a2 = Fraction(qa, 16)
+        b2 = Fraction(qb, 127) * ((1 + a2) * 255) - (a2 * 255 if a2 > 0 else 0)
+        terms = [t2, 2 * a2 * Fraction(m[4], 4), 2 * b2 * t0, a2 * a2 * Fraction(m[2], 16),
+                 2 * a2 * b2 * Fraction(m[1], 4), s0 * b2 * b2]
+        sse = terms[0] - terms[1] - terms[2] + terms[3] + terms[4] + terms[5]
+        bound = 32 * u * sum(abs(float(t)) for t in terms) + 4 * u * s0 * rms * rms + 1e-6
+        assert abs(s0 * rms * rms - float(sse)) <= bound, (m, rms, float(sse) / s0)
+    assert codes_checked > 250
 
 
 def test_fit_f64_perfect_affine_block():
@@ -266,3 +325,25 @@ def test_table_is_thread_count_invariant():
     for x, y in zip(a, b):
         assert (x.rms, x.dom_row, x.dom_col, x.isometry) == (y.rms, y.dom_row, y.dom_col,
                                                              y.isometry)
+
+
+def test_winner_consistency_check_pins_moments_and_fit():
+    """The table check used at sizes the oracle cannot search in full
+    (tests/_checks.py) accepts the oracle's own winners and rejects a winner
+    whose isometry, position or rms was altered."""
+    from mars_amd import synth
+    from tests._checks import winner_consistency
+
+    img = synth.plane(64, 48, 9)
+    rows = [r for r in range(0, 48, 8) for _ in range(0, 64, 8)]
+    cols = [c for _ in range(0, 48, 8) for c in range(0, 64, 8)]
+    ref, _ = O.search_table(img, 8, 2, rows, cols, threads=4)
+    assert winner_consistency(img, 8, rows, cols, ref) == len(rows)
+    for field, delta in (("isometry", 1), ("dom_col", 2), ("rms", 1e-12)):
+        bad = list(ref)
+        c = O.OrCandidate.from_buffer_copy(ref[5])
+        v = getattr(c, field)
+        setattr(c, field, (v + delta) % 8 if field == "isometry" else v + delta)
+        bad[5] = c
+        with pytest.raises(AssertionError):
+            winner_consistency(img, 8, rows, cols, bad)

@@ -50,3 +50,32 @@ def test_dense_stride1_mfma_exact_and_fp32_error_bounded():
         # binary32 moments (t1 and s2 are inexact above 2^24) and the binary32
         # fit's cancellation in `sum`: a bounded error in pixel-value units
         assert max(err) < 0.25
+
+
+def test_fp32_scores_are_the_binary32_fit_of_exact_moments():
+    """The plain FP32 baseline (encode.rs:295-337 fit_f32 on binary32 moments).
+    Up to s = 8 every partial sum of its binary32 correlation is an integer
+    below 2^24, so its moments are exact and its own score of each winner must
+    equal the oracle's or_fit_f32 of that winner's exact moments bit for bit
+    (the table's moments are exact integers).  Above, t1 and S2 round in
+    binary32; the deviation there is only reported."""
+    img = synth.plane(128, 128, 46)
+    ctx = HipContext(0, N.KERNEL_FP32)
+    ctx.prepare(img, EncodeParams(min_size=2, max_size=32, shift=2), [2])
+    scores = ctx.table_scores()
+    exact = 0
+    for size, shift, rows, cols, cands, P in ctx.tables():
+        sc = scores[(size, shift)]
+        dev = 0.0
+        for c, s in zip(cands, sc):
+            m = (c.s0, c.s1_x4, c.s2_x16, c.t0, c.t1_x4, c.t2)
+            _, _, r32 = O.fit_f32(m)
+            r32 = float(np.float32(r32))
+            if size <= 8:
+                assert s == r32, f"size {size}: kernel score {s!r} != fit_f32 {r32!r} of {m}"
+                exact += 1
+            dev = max(dev, abs(s - r32))
+        print(f"size {size}: {len(rows)} winners, max |score - fit_f32(exact)| {dev:.3e}",
+              flush=True)
+    ctx.close()
+    assert exact > 1000

@@ -14,6 +14,7 @@ from mars_amd import _native as N
 from mars_amd import synth
 from mars_amd.provider import EncodeParams, HipContext, HipSearchProvider, SearchRequest
 from tests import _oracle as O
+from tests._checks import winner_consistency
 
 pytestmark = pytest.mark.gpu
 
@@ -171,6 +172,26 @@ def test_parity_dense_2048_fixed16_sampled():
     ctx.close()
 
 
+@pytest.mark.timeout(600)
+def test_parity_dense_2048_fixed8_sampled():
+    """Config 5's 8x8 grid: the 2048x2048 seed-7 plane, stride 1 (65 536 ranges
+    x 4 133 089 positions x 8 isometries).  Every winner is checked for a legal
+    position and moments and fit consistent with the pixels; a sample of
+    ranges equals the oracle's exhaustive search."""
+    img = synth.plane(2048, 2048, synth.WORKLOAD_SEEDS["dense2048"])
+    print("dense2048 fixed 8x8 stride 1: GPU step", flush=True)
+    ctx = HipContext(0)
+    ctx.prepare(img, EncodeParams(min_size=8, max_size=8, shift=1), [1])
+    (size, shift, rows, cols, cands, P), = ctx.tables()
+    assert (size, shift, len(rows), P) == (8, 1, 65536, 4133089)
+    assert all(c is not None for c in cands)
+    print("dense2048 8x8: winner consistency", flush=True)
+    winner_consistency(img, size, rows, cols, cands)
+    print("dense2048 8x8: oracle sample", flush=True)
+    _check_tables(img, ctx, "dense2048-8", sample=16, seed=8)
+    ctx.close()
+
+
 def test_provider_lookup_and_errors():
     img = synth.plane(64, 64, 8)
     params = EncodeParams(min_size=4, max_size=16, shift=4)
@@ -213,3 +234,34 @@ def test_fp32_baseline_runs_and_mostly_agrees():
                 diff += 1
     print(f"fp32 baseline: {diff}/{tot} ranges differ from the binary64 reference", flush=True)
     assert diff <= 0.05 * tot
+
+
+def test_concurrent_lookups_equal_serial():
+    """SearchProvider::search is called from rayon workers in data-dependent
+    order (encode.rs:1442-1468, 2033-2051): 16 threads looking up every node of
+    a prepared 512x512 table in shuffled orders get the serial answers (ctypes
+    releases the GIL around each call, so the lookups overlap)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    img = synth.plane(512, 512, synth.WORKLOAD_SEEDS["quadtree512"])
+    ctx = HipContext(0)
+    ctx.prepare(img, EncodeParams(min_size=4, max_size=32, shift=4), [4, 8])
+    nodes = [(r, c, size, shift) for size, shift, rows, cols, _, _ in ctx.tables()
+             for r, c in zip(rows, cols)]
+    nodes += [(1, 1, 4, 4), (0, 0, 4, 12)]  # not in the table: error path under threads too
+
+    def look(order):
+        out = {}
+        for i in order:
+            rc, cand, ev = ctx.search_raw(*nodes[i])
+            out[i] = (rc, _key(cand), ev)
+        return out
+
+    serial = look(range(len(nodes)))
+    assert serial[len(nodes) - 1][0] == N.E_NOT_IN_TABLE
+    orders = [np.random.default_rng(t).permutation(len(nodes)) for t in range(16)]
+    with ThreadPoolExecutor(16) as pool:
+        results = list(pool.map(look, orders))
+    for res in results:
+        assert res == serial
+    ctx.close()
