@@ -188,6 +188,40 @@ int mars_hip_set_ranked(mars_hip_ctx *ctx, uint32_t k);
  */
 int mars_hip_set_walk_tables(mars_hip_ctx *ctx, double t_rms);
 
+/* Device scratch of mars_hip_search_grid_noalloc, per GPU, for the life of
+ * the process. */
+#define MARS_HIP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/*
+ * The fixed-grid search of mars_hip_prepare (min_size == max_size) without
+ * any allocation, for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * No counterpart in the reference (whose searches run on the CPU).
+ *
+ * Input: d_px, a w x h u8 plane in device memory.  Output: the winner of
+ * every size x size range block of the grid inside the plane (row r, column
+ * c of the grid at index r * (w / size) + c; the border blocks a quadtree
+ * walk would split are not part of the grid) in device memory: d_out[i] the
+ * reference Candidate (8-byte aligned), d_found[i] = 1, or 0 with a zeroed
+ * record when no domain position is legal.  Same results as
+ * mars_hip_prepare with min_size == max_size and p->shift as the one stride,
+ * by the split-INT8 MFMA search (whatever mars_hip_set_kernel chose).
+ *
+ * Enqueues its work on `stream` (a hipStream_t; null: the context's stream)
+ * and returns; the results are there once the stream reaches that point, and
+ * d_px must stay valid until then.  Every buffer of the step lives in a
+ * static device array of MARS_HIP_NOALLOC_SCRATCH_BYTES per GPU; a plane
+ * whose step needs more is refused with MARS_HIP_E_UNSUPPORTED before
+ * anything runs (a 1024 x 1024 plane fits at any size and stride >= 2).
+ * Every call on a GPU shares that array and starts by clearing the part it
+ * uses: calls that overlap on one GPU (two streams, two threads, two
+ * contexts of one process) race and must be serialised by the caller.
+ * Calls no hipMalloc/hipFree and allocates no host memory on success.
+ */
+int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_t w, uint32_t h,
+                                 const mars_hip_params *p, mars_hip_candidate *d_out,
+                                 uint8_t *d_found, void *stream);
+
 /* Last error message of this context ("" if none). */
 const char *mars_hip_last_error(const mars_hip_ctx *ctx);
 

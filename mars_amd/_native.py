@@ -23,6 +23,7 @@ EXPORTS = (
     "mars_hip_table_info", "mars_hip_table_read", "mars_hip_table_scores", "mars_hip_fit_f64",
     "mars_hip_copy_probe", "mars_hip_search_method", "mars_hip_table_read_method",
     "mars_hip_set_ranked", "mars_hip_table_read_ranked", "mars_hip_set_walk_tables",
+    "mars_hip_search_grid_noalloc",
 )
 
 
@@ -100,6 +101,8 @@ def lib() -> C.CDLL:
     L.mars_hip_set_walk_tables.argtypes = [C.c_void_p, C.c_double]
     L.mars_hip_table_read_ranked.argtypes = [C.c_void_p, u32, P(u32), P(CCandidate),
                                              P(C.c_uint8)]
+    L.mars_hip_search_grid_noalloc.argtypes = [C.c_void_p, C.c_void_p, u32, u32, P(Params),
+                                               C.c_void_p, C.c_void_p, C.c_void_p]
     for name in EXPORTS:
         if name not in ("mars_hip_close", "mars_hip_last_error"):
             getattr(L, name).restype = C.c_int

@@ -32,6 +32,43 @@
 #include "mars_codec.h"
 #include "mars_hip.h"
 
+// Device memory of mars_hip_search_grid_noalloc: every buffer of its step
+// beyond the caller's input plane and output records (pool, range operands,
+// position records, partial winners, descriptors, work lists, counters).
+// Each GPU the library runs on holds its own copy until the process ends.
+// Host code never passes this name to a kernel or copy (on the host it is a
+// stand-in, not a device address): mars_hip_open looks up each device's
+// address once with hipGetSymbolAddress and the entry point uses that.
+static __device__ __align__(256) unsigned char scratch[MARS_HIP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+// Device address of `scratch` per HIP device (null until looked up).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// Look up `scratch` on the current device (`device`), once per device.
+bool bind_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return false;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return true;
+    unsigned char *ptr = nullptr;
+    if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return false;
+    g_scratch_ptr[device] = ptr;
+    return true;
+}
+
+unsigned char *scratch_of(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+} // namespace
+
 namespace {
 
 using mars::Common;
@@ -826,6 +863,31 @@ int timed_step(mars_hip_ctx *ctx, mars_hip_timing *tm, std::vector<double> *grou
     return rc;
 }
 
+// Step constants of one set of search params (the yscale table, counters,
+// sample and topk are the caller's to set).  Stage 2 is exact only when
+// max_alfa is a power of two: then the alfa scaling of encode.rs:264 is exact
+// in binary64.
+void fill_common(const mars_hip_params &p, Common &c)
+{
+    int e = 0;
+    const double mant = std::frexp(p.max_alfa, &e);
+    c = Common{};
+    c.max_alfa = p.max_alfa;
+    c.bits_alfa = p.bits_alfa;
+    c.bits_beta = p.bits_beta;
+    c.stage2 = (mant == 0.5 && p.bits_alfa <= 8 && p.bits_beta <= 12) ? 1 : 0;
+    c.alfa_c = std::ldexp(1.0, (int)p.bits_alfa + 2) / p.max_alfa;
+    c.alfa_unit = p.max_alfa / std::ldexp(1.0, (int)p.bits_alfa);
+    c.qa_max = (double)((1u << p.bits_alfa) - 1);
+    c.qb_max = (double)((1u << p.bits_beta) - 1);
+    // Largest alfa2 of encode.rs:266 (qalfa <= qa_max), over 4, rounded up;
+    // 1 / (2 cap_u) rounded down and capped.
+    const double u = c.qa_max * c.alfa_unit * 0.25 * (1.0 + 1e-12);
+    c.cap_u = std::nextafter((float)u, INFINITY);
+    const double k = u > 0.0 ? 1.0 / (2.0 * (double)c.cap_u) : 8.0;
+    c.cap_k = std::nextafter((float)(k < 8.0 ? k : 8.0), 0.0f);
+}
+
 bool valid_params(const mars_hip_params *p)
 {
     return p->bits_alfa >= 1 && p->bits_alfa <= 16 && p->bits_beta >= 1 && p->bits_beta <= 16 &&
@@ -1039,29 +1101,9 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
-    // Stage-2 constants: exact only when max_alfa is a power of two (then the
-    // alfa scaling of encode.rs:264 is exact in binary64).
     {
-        int e = 0;
-        double mant = std::frexp(p->max_alfa, &e);
         Common &c = ctx->common;
-        c = Common{};
-        c.max_alfa = p->max_alfa;
-        c.bits_alfa = p->bits_alfa;
-        c.bits_beta = p->bits_beta;
-        c.stage2 = (mant == 0.5 && p->bits_alfa <= 8 && p->bits_beta <= 12) ? 1 : 0;
-        c.alfa_c = std::ldexp(1.0, (int)p->bits_alfa + 2) / p->max_alfa;
-        c.alfa_unit = p->max_alfa / std::ldexp(1.0, (int)p->bits_alfa);
-        c.qa_max = (double)((1u << p->bits_alfa) - 1);
-        c.qb_max = (double)((1u << p->bits_beta) - 1);
-        {
-            // Largest alfa2 of encode.rs:266 (qalfa <= qa_max), over 4,
-            // rounded up; 1 / (2 cap_u) rounded down and capped.
-            const double u = c.qa_max * c.alfa_unit * 0.25 * (1.0 + 1e-12);
-            c.cap_u = std::nextafter((float)u, INFINITY);
-            const double k = u > 0.0 ? 1.0 / (2.0 * (double)c.cap_u) : 8.0;
-            c.cap_k = std::nextafter((float)(k < 8.0 ? k : 8.0), 0.0f);
-        }
+        fill_common(*p, c);
         const uint32_t nq = 1u << p->bits_alfa;
         std::vector<double> ys(nq);
         for (uint32_t q = 0; q < nq; q++) {
@@ -1173,6 +1215,11 @@ int mars_hip_open(int device, mars_hip_ctx **out)
         mars_hip_close(ctx);
         return MARS_HIP_E_DEVICE;
     }
+    ctx->err.reserve(512); // error messages then fit without a host allocation
+    // The no-alloc search's scratch address on this device; a failed lookup
+    // (status checked in bind_scratch) leaves it null and only that entry
+    // point then refuses to run.
+    (void)bind_scratch(device);
     *out = ctx;
     return MARS_HIP_OK;
 }
@@ -1480,6 +1527,189 @@ int mars_hip_fit_f64(mars_hip_ctx *ctx, const int64_t *moments, uint32_t n, doub
     (void)hipFree(dqa);
     (void)hipFree(dqb);
     (void)hipFree(dr);
+    return MARS_HIP_OK;
+}
+
+int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_t w, uint32_t h,
+                                 const mars_hip_params *p, mars_hip_candidate *d_out,
+                                 uint8_t *d_found, void *stream)
+{
+    if (!ctx || !d_px || !p || !d_out || !d_found) return fail(ctx, MARS_HIP_E_ARG, "null argument");
+    if (w == 0 || h == 0 || w > 16384 || h > 16384) return fail(ctx, MARS_HIP_E_ARG, "bad plane size");
+    if (!valid_params(p) || p->shift == 0) return fail(ctx, MARS_HIP_E_ARG, "bad params");
+    if (p->method != MARS_HIP_METHOD_EXHAUSTIVE)
+        return fail(ctx, MARS_HIP_E_ARG, "the no-alloc search is exhaustive only");
+    const uint32_t s = p->max_size;
+    if (p->min_size != s || s < 2 || s > kMaxSize || (s & (s - 1)))
+        return fail(ctx, MARS_HIP_E_ARG,
+                    "the no-alloc search takes one range size: min_size == max_size, a power of "
+                    "two 2..32");
+    unsigned char *const base = scratch_of(ctx->device);
+    if (!base)
+        return fail(ctx, MARS_HIP_E_DEVICE,
+                    "hipGetSymbolAddress found no scratch address for this device");
+
+    // Geometry, as prepare_batch lays out one table of one plane.
+    const uint32_t shift = p->shift, gw = w / s, gh = h / s, nr = gw * gh;
+    if (nr == 0) return MARS_HIP_OK;
+    const uint32_t nr_pad = (nr + 7) / 8 * 8, K = s * s, kmem = K < 16 ? 16 : K;
+    uint32_t npr = 0, npc = 0;
+    if (h >= 2 * s && w >= 2 * s) {
+        npr = (h - 2 * s) / shift + 1;
+        npc = (w - 2 * s) / shift + 1;
+    }
+    const uint64_t P64 = (uint64_t)npr * npc;
+    if (P64 > 0xffffffffu / 8) return fail(ctx, MARS_HIP_E_UNSUPPORTED, "too many positions");
+    const uint32_t P = (uint32_t)P64, P_pad = (P + 31) / 32 * 32;
+    const uint32_t wc = w / 2, hc = h / 2, pitch = ((wc + 15) / 16) * 16 + 16;
+    const size_t copy_stride = (size_t)pitch * std::max<uint32_t>(hc, 1);
+    const size_t nc = (size_t)wc * hc, nsat = (size_t)(wc + 1) * (hc + 1);
+    // search order of sizes >= 16 (search_order)
+    const uint32_t order_m = (s >= 16 && P > 0 && 32 % shift == 0 && 32 / shift > 1) ? 32 / shift : 0;
+    // chunking of the search with the context's shape for this size
+    const uint32_t ngroups = nr_pad / 8;
+    const Shape &sh = ctx->shape[ceil_log2(s) - 1];
+    uint32_t nchunks = 1, tpc = 1;
+    chunking(cdiv(P, 32), ngroups, sh.target_blocks, sh.min_tiles, &nchunks, &tpc);
+    const bool prepass = s >= kPrepassMinSize && P > 0;
+    const uint32_t n_contract = std::max<uint32_t>(1, cdiv(nc, 256));
+    const uint32_t n_sat_rows = hc + 1, n_sat_cols = cdiv(wc + 1, 16), n_possums = cdiv(P_pad, 256);
+    const uint32_t n_ranges = cdiv((uint64_t)nr_pad * 8 * kmem / 4, 256);
+    const uint32_t n_rstats = cdiv(nr_pad, 4), n_seed = cdiv(nr, 4), n_merge = cdiv(nr, 128);
+    const uint32_t n2 = std::max({n_contract, n_sat_rows, n_sat_cols, n_possums, n_ranges, n_rstats,
+                                  n_seed, n_merge});
+    const uint64_t nsearch = (uint64_t)ngroups * nchunks;
+    if (n2 >= (1u << 24) || nsearch >= (1u << 24))
+        return fail(ctx, MARS_HIP_E_UNSUPPORTED, "plane too large for one step");
+    const uint32_t nq = 1u << p->bits_alfa;
+
+    // Carve the scratch (no allocation: fixed offsets from its base).
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *q = base + off;
+        off += (std::max<size_t>(bytes, 1) + kAlign - 1) / kAlign * kAlign;
+        return (void *)q;
+    };
+    mars::GridSetup g{};
+    g.plane_dst = (PlaneDesc *)take(sizeof(PlaneDesc));
+    g.table_dst = (TableDesc *)take(sizeof(TableDesc));
+    g.w2 = (Work2 *)take(sizeof(Work2) * n2);
+    g.search = (Work3 *)take(sizeof(Work3) * std::max<uint64_t>(nsearch, 1));
+    g.prepass = prepass ? (Work3 *)take(sizeof(Work3) * ngroups) : nullptr;
+    auto *counters = (unsigned long long *)take(sizeof(unsigned long long) * kCounters);
+    g.yscale = (double *)take(sizeof(double) * nq);
+    g.rows = (uint32_t *)take(4 * (size_t)nr);
+    g.cols = (uint32_t *)take(4 * (size_t)nr);
+    PlaneDesc &pd = g.plane;
+    pd.px = d_px;
+    pd.d = (int32_t *)take(nc * sizeof(int32_t));
+    pd.df = (float *)take(nc * sizeof(float));
+    pd.lo = (int8_t *)take(16 * copy_stride);
+    pd.hi = (int8_t *)take(16 * copy_stride);
+    pd.sat1 = (int64_t *)take(nsat * sizeof(int64_t));
+    pd.sat2 = (int64_t *)take(nsat * sizeof(int64_t));
+    pd.w = w;
+    pd.h = h;
+    pd.wc = wc;
+    pd.hc = hc;
+    pd.pitch = pitch;
+    pd.copy_stride = (uint32_t)copy_stride;
+    TableDesc &td = g.table;
+    td.px = pd.px;
+    td.d = pd.d;
+    td.df = pd.df;
+    td.lo = pd.lo;
+    td.hi = pd.hi;
+    td.sat1 = pd.sat1;
+    td.sat2 = pd.sat2;
+    td.w = w;
+    td.wc = wc;
+    td.pitch = pitch;
+    td.copy_stride = (uint32_t)copy_stride;
+    td.rows = g.rows;
+    td.cols = g.cols;
+    td.rng8 = (int8_t *)take((size_t)nr_pad * 8 * kmem);
+    td.rngf = nullptr; // the FP32 baseline's operand: not used here
+    td.rstat = (int32_t *)take(sizeof(int32_t) * 2 * nr_pad);
+    td.rpos = (int32_t *)take(sizeof(int32_t) * nr_pad);
+    td.gthr = (uint32_t *)take(sizeof(uint32_t) * nr_pad);
+    td.posinfo = (uint4 *)take(sizeof(uint4) * (size_t)P_pad);
+    g.iperm = order_m ? (uint32_t *)take(sizeof(uint32_t) * (size_t)P) : nullptr;
+    td.iperm = g.iperm;
+    td.posx = (int32_t *)take(2 * sizeof(int32_t) * (size_t)P);
+    td.posd = (double *)take(2 * sizeof(double) * (size_t)P);
+    td.partial = (Partial *)take(sizeof(Partial) * (size_t)nr_pad * nchunks);
+    td.out = d_out;
+    td.found = d_found;
+    td.score = (double *)take(sizeof(double) * (size_t)nr);
+    td.size = s;
+    td.shift = shift;
+    td.P = P;
+    td.P_pad = P_pad;
+    td.npc = npc;
+    td.nr = nr;
+    td.nr_pad = nr_pad;
+    td.kmem = kmem;
+    td.nchunks[0] = nchunks;
+    td.tiles_per_chunk[0] = tpc;
+    td.nchunks[1] = td.tiles_per_chunk[1] = 1;
+    if (off > (size_t)MARS_HIP_NOALLOC_SCRATCH_BYTES) {
+        char buf[160];
+        snprintf(buf, sizeof buf,
+                 "%ux%u plane, size %u, stride %u needs %zu bytes of scratch; the no-alloc "
+                 "search has %zu",
+                 w, h, s, shift, off, (size_t)MARS_HIP_NOALLOC_SCRATCH_BYTES);
+        return fail(ctx, MARS_HIP_E_UNSUPPORTED, buf);
+    }
+    g.n2 = n2;
+    g.ngroups = ngroups;
+    g.nchunks = nchunks;
+    g.grid_cols = gw;
+    g.nr = nr;
+    g.size = s;
+    g.P = P;
+    g.npc = npc;
+    g.order_m = order_m;
+    g.nq = nq;
+    g.bits_alfa = p->bits_alfa;
+    g.max_alfa = p->max_alfa;
+    Common c;
+    fill_common(*p, c);
+    g.qb_max = c.qb_max;
+    c.yscale = g.yscale;
+    c.counters = counters;
+    c.sample = 1;
+    c.topk = 1;
+
+    // The step: everything the shared kernels read starts from zero, as a
+    // freshly prepared arena does (the previous call's contents are stale).
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    const PlaneDesc *planes = g.plane_dst;
+    const TableDesc *tabs = g.table_dst;
+    const Work2 *w2 = g.w2;
+    mars::launch_zero_bytes(base, off, st);
+    mars::launch_grid_setup(g, st);
+    if (P > 0) {
+        mars::launch_ranges(tabs, w2, n_ranges, false, st);
+        mars::launch_rstats(tabs, w2, n_rstats, st);
+    }
+    mars::launch_contract(planes, w2, n_contract, counters, kCounters, st);
+    mars::launch_sat(planes, w2, n_sat_rows, w2, n_sat_cols, st);
+    mars::launch_possums(tabs, w2, n_possums, st);
+    if (P > 0) {
+        mars::launch_seed(tabs, w2, n_seed, c, st);
+        if (prepass) {
+            Common pre = c;
+            pre.sample = kPrepassStride;
+            if (mars::launch_search(MARS_HIP_KERNEL_MFMA_I8, s, tabs, g.prepass, ngroups, pre, st))
+                return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
+        }
+        if (mars::launch_search(MARS_HIP_KERNEL_MFMA_I8, s, tabs, g.search, (uint32_t)nsearch, c, st))
+            return fail(ctx, MARS_HIP_E_UNSUPPORTED, "no search kernel for this size");
+    }
+    mars::launch_merge(tabs, w2, n_merge, true, c, st);
+    HIP_TRY(ctx, hipGetLastError());
     return MARS_HIP_OK;
 }
 

@@ -147,6 +147,32 @@ void launch_merge_ranked(const TableDesc *t, const Work2 *work, uint32_t nwork, 
 void launch_fisher(const TableDesc *t, const Work2 *dom_work, uint32_t n_dom,
                    const Work2 *rng_work, uint32_t n_rng, const Work2 *tab_work, uint32_t n_tab,
                    const Work2 *range_work, uint32_t n_range, const Common &c, hipStream_t st);
+// Fixed-grid search into caller-free memory (mars_hip_search_grid_noalloc):
+// what prepare_batch builds on the host and copies over -- the descriptors,
+// the work lists, the range coordinates, the search order and the quantiser
+// scale table -- is written on the device by one kernel from these values.
+struct GridSetup {
+    PlaneDesc plane;
+    TableDesc table;
+    PlaneDesc *plane_dst;
+    TableDesc *table_dst;
+    Work2 *w2;        // [n2]: (0, i), the one list every Work2 launch reads a prefix of
+    uint32_t n2;
+    Work3 *search;    // [ngroups * nchunks]: (0, chunk, range group), group-major
+    Work3 *prepass;   // [ngroups]: (0, 0, range group), or null
+    uint32_t ngroups, nchunks;
+    uint32_t *rows, *cols; // [nr]: the grid's ranges, row-major
+    uint32_t grid_cols, nr, size;
+    uint32_t *iperm;  // [P]: search slot of each position, or null (natural order)
+    uint32_t P, npc, order_m;
+    double *yscale;   // [nq]: (2^bits_beta - 1) / ((1 + alfa2) * 255) per alfa code
+    uint32_t nq, bits_alfa;
+    double max_alfa, qb_max;
+};
+void launch_grid_setup(const GridSetup &g, hipStream_t st);
+// Zero `bytes` (a multiple of 16) at p with vector stores.
+void launch_zero_bytes(void *p, size_t bytes, hipStream_t st);
+
 void launch_copy_probe(const void *src, void *dst, size_t bytes, hipStream_t st);
 void launch_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, uint32_t bits_alfa,
                          uint32_t bits_beta, uint32_t *qa, uint32_t *qb, double *rms,

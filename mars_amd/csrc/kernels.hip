@@ -1519,7 +1519,67 @@ __global__ void k_fit_selftest(const int64_t *m, uint32_t n, double max_alfa, ui
             &rms[i], &sum);
 }
 
+// Set-up of a fixed-grid search whose buffers the caller did not allocate:
+// one grid-stride pass writes every list, coordinate and table that
+// prepare_batch would build on the host (same values, so the shared kernels
+// behave exactly as for a prepared table of one plane and one size).
+__global__ void k_grid_setup(GridSetup g)
+{
+    const uint32_t nsearch = g.ngroups * g.nchunks;
+    const uint32_t nperm = g.iperm ? g.P : 0;
+    const uint32_t total = max(max(max(g.n2, nsearch), max(g.nr, nperm)), max(g.nq, 1u));
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        if (i == 0) {
+            *g.plane_dst = g.plane;
+            *g.table_dst = g.table;
+        }
+        if (i < g.n2) g.w2[i] = make_uint2(0u, i);
+        if (i < nsearch) g.search[i] = make_uint3(0u, i % g.nchunks, i / g.nchunks);
+        if (g.prepass && i < g.ngroups) g.prepass[i] = make_uint3(0u, 0u, i);
+        if (i < g.nr) {
+            const uint32_t gi = i / g.grid_cols;
+            g.rows[i] = gi * g.size;
+            g.cols[i] = (i - gi * g.grid_cols) * g.size;
+        }
+        if (i < nperm) {
+            // Positions ordered by domain row, then by column residue modulo
+            // order_m, then by column: a residue class c starts after the
+            // ceil((npc - c') / m) columns of every smaller residue c'.
+            const uint32_t pi = i / g.npc, pj = i - pi * g.npc, m = g.order_m;
+            const uint32_t c = pj % m;
+            uint32_t off = 0;
+            for (uint32_t c2 = 0; c2 < c; c2++) off += (g.npc - c2 + m - 1) / m;
+            g.iperm[i] = pi * g.npc + off + pj / m;
+        }
+        if (i < g.nq) {
+            const double a2 = (double)i / (double)(1u << g.bits_alfa) * g.max_alfa;
+            g.yscale[i] = g.qb_max / ((1.0 + fabs(a2)) * 255.0);
+        }
+    }
+}
+
+__global__ void k_zero_bytes(uint4 *p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // ---------------------------------------------------------------- launchers
+
+void launch_grid_setup(const GridSetup &g, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_grid_setup, dim3(256), dim3(256), 0, st, g);
+}
+
+void launch_zero_bytes(void *p, size_t bytes, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    if (n == 0) return;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero_bytes, dim3((uint32_t)blocks), dim3(256), 0, st, (uint4 *)p, n);
+}
 
 template <int S, int KCAP>
 static void launch_mfma(const TableDesc *t, const Work3 *work, uint32_t n, const Common &c,
