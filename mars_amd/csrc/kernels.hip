@@ -335,15 +335,16 @@ __device__ __forceinline__ void load_b(const GLB int8_t *plane, uint32_t boff, u
         }
         *b8 = (long)x;
     } else if constexpr (S == 4) {
+        // (one 32-bit offset per load: the plane base stays a scalar operand)
         const uint32_t o = boff + 2 * h * pitch;
         const uint32_t r0 = *(const GLB uint32_t *)(plane + o);
-        const uint32_t r1 = *(const GLB uint32_t *)(plane + o + pitch);
+        const uint32_t r1 = *(const GLB uint32_t *)(plane + (o + pitch));
         *b8 = (long)((uint64_t)r0 | ((uint64_t)r1 << 32));
     } else if constexpr (S == 8) {
         typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
         const uint32_t o = boff + (4 * kk + 2 * h) * pitch;
         const u32x2 r0 = *(const GLB u32x2 *)(plane + o);
-        const u32x2 r1 = *(const GLB u32x2 *)(plane + o + pitch);
+        const u32x2 r1 = *(const GLB u32x2 *)(plane + (o + pitch));
         v4i t;
         t.x = (int)r0.x;
         t.y = (int)r0.y;
@@ -734,7 +735,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // for a padding range and, through S1 = -2^21 in the padding position
     // record, -2^28 for a padding position put the limit above every v; no
     // combination leaves int32.
-    int32_t offx[4];
+    // noff = -1 - (that offset), so a limit is one three-term add.
+    int32_t noff[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const uint32_t r = g * 8 + 4 * h + i;
@@ -745,17 +747,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         A[i] = (double)(n * t2 - t0 * t0);
         Alo[i] = (float)A[i] * (1.0f - 3e-7f); // <= A after both roundings
         rmin[i] = rv ? (int32_t)0x80000000 : 0x7fffffff;
-        // sa < 0 until the first threshold
-        offx[i] = (int32_t)(512 * (t0 - 128 * n)) + (rv ? (1 << 27) : -(1 << 28));
+        // no pruning until the first threshold
+        noff[i] = -1 - ((int32_t)(512 * (t0 - 128 * n)) + (rv ? (1 << 27) : -(1 << 28)));
     }
     // t1 = 4*hi + lo + off + 128*S1 with off = 512 * (t0 - 128 n) (< 2^27 in
     // magnitude), recomputed from t0 where needed rather than kept live.
     auto off_of = [&](int i) { return (T0[i] - (int32_t)(128 * n)) * 512; };
-    // Stage-1 cache per range: sqrt(A - n tau) (negative when A - n tau <= 0)
-    // and the threshold bits it was computed from.
+    // Stage-1 cache per range: sqrt(A - n tau), and the threshold bits it
+    // was computed from.  A - n tau <= 0 (no pruning) is sa < 0 for sizes
+    // >= 16; sizes <= 8 fold it into noff and keep sa = 1 (positive, so the
+    // limit needs no absolute value).
     float sa[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) sa[i] = -1.0f;
+    for (int i = 0; i < 4; i++) sa[i] = INT_LIMIT ? 1.0f : -1.0f;
     uint32_t seen = 0xffffffffu; // s_epoch the limits were computed at (none yet)
     // Lower a threshold, noting the drop for every wave.
     auto lower = [&](uint32_t slot, uint32_t v) {
@@ -973,12 +977,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                     // sqrt within 1 ulp; (1 - 2e-5) keeps nt below
                     // sqrt((A - n tau) Di) after the product's rounding
                     // (sqDi is sqrt(Di) rounded down).
-                    // negative: A - n tau <= 0, no pruning for this range
-                    sa[i] = amt > 0.0f ? __builtin_amdgcn_sqrtf(amt) * (1.0f - 2e-5f) : -1.0f;
+                    const bool prune = amt > 0.0f; // else A - n tau <= 0: no pruning
+                    sa[i] = prune ? __builtin_amdgcn_sqrtf(amt) * (1.0f - 2e-5f)
+                                  : (INT_LIMIT ? 1.0f : -1.0f);
                     if constexpr (INT_LIMIT)
-                        offx[i] = off_of(i) + (rmin[i] == (int32_t)0x80000000
-                                                ? (sa[i] < 0.0f ? (1 << 27) : 0)
-                                                : -(1 << 28));
+                        noff[i] = -1 - (off_of(i) + (rmin[i] == (int32_t)0x80000000
+                                                         ? (prune ? 0 : (1 << 27))
+                                                         : -(1 << 28)));
                 }
             }
         }
@@ -994,16 +999,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         const float usq = CAP_LIMIT ? c.cap_u * sqDi : 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            float nt = fabsf(sa[i]) * sqDi; // <= sqrt((A - n tau) Di) when sa >= 0
+            const float sab = INT_LIMIT ? sa[i] : fabsf(sa[i]);
+            float nt = sab * sqDi; // <= sqrt((A - n tau) Di) when pruning
             if constexpr (CAP_LIMIT) {
-                const float d = fmaxf(fabsf(sa[i]) - usq, 0.0f);
+                const float d = fmaxf(sab - usq, 0.0f);
                 nt = fmaf(d * d, c.cap_k, nt);
             }
             if constexpr (INT_LIMIT) {
                 // S1*t0 + nt < 2^31 for sizes <= 8 (nt <= sqrt(A Di) + 8 A),
                 // S1 and t0 < 2^24.
                 const uint32_t X = __umul24((uint32_t)S1p, (uint32_t)T0[i]) + (uint32_t)nt;
-                lim[i] = (int32_t)(X >> LOG2N) - 1 - (offx[i] + o128);
+                lim[i] = (int32_t)(X >> LOG2N) + noff[i] - o128;
             } else {
                 const double X = ((double)S1p * (double)T0[i] + (double)nt) * inv_n;
                 const int32_t L = (int32_t)floor(X) - 1 - (off_of(i) + o128);
@@ -1011,16 +1017,66 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
             }
         }
 
-        // Epilogue of one row tile: one compare per value, issued 8 at a
-        // time into SGPR masks that the scalar unit ORs, so a group costs one
-        // branch; only a group where some lane passes goes value by value and
-        // appends (wave ballot + mbcnt).  The queue is drained after every
-        // DRAIN values.
+        // Epilogue of one row tile.  Sizes >= 16: one compare per value,
+        // issued 8 at a time into SGPR masks that the scalar unit ORs, so a
+        // group costs one branch; only a group where some lane passes goes
+        // value by value and appends (wave ballot + mbcnt).  The queue is
+        // drained after every DRAIN values.  Sizes <= 8: see below.
         constexpr int DRAIN = Queue<S>::DRAIN;
         constexpr int GW = DRAIN < 8 ? DRAIN : 8; // values per compare group
         static_assert(16 % DRAIN == 0 && DRAIN % GW == 0, "groups tile a row tile's 16 values");
         static_assert(64 * DRAIN + 64 <= Queue<S>::CAP, "a drain interval fits the queue");
         auto epilogue = [&](const v16i &ah, const v16i &al, const int tt) {
+            if constexpr (DRAIN == 16) {
+                // Sizes <= 8: one compare per (range, position).  The limit
+                // does not depend on the isometry, so the row tile's 4 values
+                // of a range are tested by their maximum (a v_max3 and a
+                // v_max); only a range some lane passes is tested value by
+                // value.  Against a compare per value this also drops most
+                // of the scalar mask ORs and per-value branches (s = 4 launch
+                // 0.367 -> 0.330 ms at 512 x 512).
+                int32_t v[16];
+#pragma unroll
+                for (int t = 0; t < 16; t++) v[t] = 4 * ah[t] + al[t];
+                unsigned long long mr[4], any = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int32_t mx = max(max(max(v[i], v[i + 4]), v[i + 8]), v[i + 12]);
+                    mr[i] = __ballot(mx > lim[i]);
+                    any |= mr[i];
+                }
+                if (any != 0) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        if (!mr[i]) continue;
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            const int t = 4 * j + i;
+                            const bool sv = v[t] > lim[i];
+                            const unsigned long long m = __ballot(sv);
+                            if (m) {
+                                const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
+                                    (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                                if (sv)
+                                    q[qn + pre] = make_uint2((uint32_t)(v[t] + off_of(i) + o128) |
+                                                                 ((uint32_t)(4 * h + i) << 28),
+                                                             p * 8 + 4 * tt + j);
+                                qn += (uint32_t)__popcll(m);
+                                qtotal += (uint32_t)__popcll(m);
+                            }
+                        }
+                    }
+                    if (qn >= 64) {
+                        uint32_t b0 = 0;
+                        for (; qn - b0 >= 64; b0 += 64) process(b0, 64);
+                        const uint32_t rem = qn - b0;
+                        const uint2 tail = q[b0 + ((uint32_t)lane < rem ? lane : 0)];
+                        if ((uint32_t)lane < rem) q[lane] = tail;
+                        qn = rem;
+                    }
+                }
+                return;
+            }
 #pragma unroll
             for (int grp = 0; grp < 16 / DRAIN; grp++) {
 #pragma unroll
