@@ -41,7 +41,10 @@ MIN_SIZE, MAX_SIZE, SHIFT = 4, 32, 4
 WARMUP_T_RMS = 8.0  # the RD warm-up's threshold walk (encode.rs:1139-1169)
 
 
-def _gpu_leg(steps: int, warmup: int):
+def _gpu_leg(steps: int, warmup: int, cpu_leg=None):
+    """Both kernels on the quadtree512 workload.  cpu_leg(evals_by_size,
+    gpu_run), when given, runs while the MFMA context is open, so its CPU
+    repeats can interleave with GPU runs of the same step."""
     from mars_amd import _native as N
     from mars_amd import synth
     from mars_amd.provider import EncodeParams, HipContext
@@ -64,6 +67,7 @@ def _gpu_leg(steps: int, warmup: int):
         tabs = ctx.tables()
         evals_by_size = {str(s): len(rows) * P * 8 for s, sh, rows, cols, c, P in tabs}
         res[name] = dict(ms_step=t.ms_step, evals=t.evals, ms_pool=float(np.mean([b.ms_pool for b in br])),
+                         ms_pool_build=float(np.median([b.ms_pool_build for b in br])),
                          ms_search=float(np.mean([b.ms_search for b in br])),
                          ms_by_size=by_size, evals_by_size=evals_by_size,
                          survivors=int(br[0].survivors),
@@ -75,6 +79,9 @@ def _gpu_leg(steps: int, warmup: int):
                                  for i in range(6) if tune.ms[i] > 0},
                          winners=[[(c.dom_row, c.dom_col, c.isometry, c.qalfa, c.qbeta)
                                    if c is not None else None for c in tt[4]] for tt in tabs])
+        if cpu_leg is not None and kern == N.KERNEL_MFMA_I8:
+            res["cpu"] = cpu_leg(evals_by_size,
+                                 lambda: ctx.run(iters=steps, use_graph=True).ms_step)
         ctx.close()
     return img, res
 
@@ -93,7 +100,9 @@ def _batch_leg(name: str, specs, steps: int):
     tune = ctx.autotune(reps=1)
     ctx.run(iters=1, use_graph=True)
     t = ctx.run(iters=steps, use_graph=True)
+    pool_ms = float(np.median([ctx.run(iters=1, use_graph=False).ms_pool_build for _ in range(3)]))
     ctx.close()
+    pool_b = sum(_pool_bytes(w, h, [4, 8, 16, 32], SHIFT) for w, h, _, _ in specs)
     per_ms = 0.0
     for img in imgs:
         c = HipContext(0)
@@ -109,6 +118,8 @@ def _batch_leg(name: str, specs, steps: int):
             "one_step_per_image": {"ms_per_batch": per_ms,
                                    "images_per_s": len(imgs) * 1e3 / per_ms},
             "speedup_vs_one_step_per_image": per_ms / t.ms_step,
+            "pool_build": {"ms": pool_ms, "bytes": pool_b, "GB_per_s": pool_b / (pool_ms * 1e-3) / 1e9,
+                           "frac_of_hbm_peak": pool_b / (pool_ms * 1e-3) / PEAK_HBM},
             "autotuned_shapes": {str(2 << i): [int(tune.target_blocks[i]), int(tune.min_tiles[i])]
                                  for i in range(6) if tune.ms[i] > 0}}
 
@@ -214,41 +225,102 @@ def _ranked_leg(k: int, steps: int, warmup: int) -> int:
     return 0
 
 
-def _cpu_leg(img, budget_s: float = 12.0):
-    """The oracle (C restatement of the reference CPU search) on the host cores,
-    on a bounded sample of the same workload: ranges drawn from every size."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _host_threads() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _median_mad(xs):
+    med = float(np.median(xs))
+    return med, float(np.median([abs(x - med) for x in xs]))
+
+
+def _cpu_leg(img, evals_by_size, gpu_run, reps: int = 5, per_size_s: float = 0.4):
+    """The oracle (C restatement of the reference CPU search) on the host, in
+    the reference's protocol (crates/mars-bench/src/gpu_search.rs:350-408):
+    runs A/B interleaved with GPU runs, `reps` repeats, median + MAD, one
+    thread and all host threads.  Each repeat times a fixed seeded sample of
+    every size's ranges; the rate of the workload is the per-size rates
+    combined with the workload's own per-size eval counts (a step is 76% s = 4
+    evals, the CPU's fastest size per eval): sum(E_s) / sum(E_s / rate_s)."""
     from tests import _oracle as O
 
-    threads = os.cpu_count() or 1
+    threads = _host_threads()
     nodes = O.nodes(W, H, MIN_SIZE, MAX_SIZE)
     rng = np.random.default_rng(0)
-    evals = 0
-    t0 = time.perf_counter()
-    used = {}
-    rounds = 0
-    while time.perf_counter() - t0 < budget_s and rounds < 1000:
-        for size in sorted(nodes):
-            pick = rng.choice(len(nodes[size]), size=threads, replace=False)
-            rows = [nodes[size][i][0] for i in pick]
-            cols = [nodes[size][i][1] for i in pick]
-            _, ev = O.search_table(img, size, SHIFT, rows, cols, threads=threads)
-            evals += ev
-            used[size] = used.get(size, 0) + len(rows)
-        rounds += 1
-    dt = time.perf_counter() - t0
-    return dict(value=evals / dt, unit="evals/s", cores=threads, kind="port",
-                sample=f"{sum(used.values())} ranges ({', '.join(f'{v} of size {k}' for k, v in sorted(used.items()))}) "
-                       f"of the 512x512 quadtree workload, {evals:.3g} evals in {dt:.1f} s, "
-                       f"oracle/liboracle.so over {threads} threads")
+    sizes = sorted(nodes)
+
+    def timed(size, idx, nthreads):
+        rows = [nodes[size][i][0] for i in idx]
+        cols = [nodes[size][i][1] for i in idx]
+        t0 = time.perf_counter()
+        _, ev = O.search_table(img, size, SHIFT, rows, cols, threads=nthreads)
+        return ev, time.perf_counter() - t0
+
+    # sample sizes: about per_size_s of work per size and mode
+    sample = {}
+    for size in sizes:
+        ev, dt = timed(size, [0], 1)
+        n1 = int(max(1, min(len(nodes[size]), round(per_size_s / max(dt, 1e-6)))))
+        nall = int(min(len(nodes[size]), n1 * threads))
+        sample[size] = (rng.choice(len(nodes[size]), size=n1, replace=False),
+                        rng.choice(len(nodes[size]), size=nall, replace=False))
+    mix = {1: [], threads: []}
+    per = {1: {s_: [] for s_ in sizes}, threads: {s_: [] for s_ in sizes}}
+    gpu_ms = []
+    for _ in range(reps):
+        gpu_ms.append(gpu_run())  # A/B: a GPU run before each CPU repeat
+        for mode, k in ((1, 0), (threads, 1)):
+            denom = 0.0
+            for size in sizes:
+                ev, dt = timed(size, sample[size][k], mode)
+                rate = ev / dt
+                per[mode][size].append(rate)
+                denom += evals_by_size[str(size)] / rate
+            mix[mode].append(sum(evals_by_size[str(s_)] for s_ in sizes) / denom)
+    st_med, st_mad = _median_mad(mix[1])
+    mt_med, mt_mad = _median_mad(mix[threads])
+    g_med, g_mad = _median_mad(gpu_ms)
+    share = {str(s_): evals_by_size[str(s_)] / sum(evals_by_size[str(x)] for x in sizes) for s_ in sizes}
+    return dict(
+        value=mt_med, unit="evals/s", cores=threads, kind="port",
+        sample=(f"{reps} repeats A/B-interleaved with GPU runs; each repeat times seeded samples "
+                f"of every size's ranges of the 512x512 quadtree workload "
+                f"({', '.join(f'{len(sample[s_][0])}/{len(sample[s_][1])} of size {s_}' for s_ in sizes)} "
+                f"ranges, one thread/all threads) and combines the per-size rates with the "
+                f"workload's eval shares ({', '.join(f's{k} {v:.1%}' for k, v in share.items())}); "
+                f"oracle/liboracle.so, {_cpu_model()}"),
+        mad=mt_mad, runs=mix[threads],
+        single_thread={"value": st_med, "mad": st_mad, "runs": mix[1], "cores": 1},
+        per_size_evals_per_s={"1": {str(s_): float(np.median(v)) for s_, v in per[1].items()},
+                              str(threads): {str(s_): float(np.median(v)) for s_, v in per[threads].items()}},
+        cpu_model=_cpu_model(),
+        gpu_interleaved={"ms_per_step_median": g_med, "mad": g_mad, "runs": gpu_ms},
+        gpu_vs_cpu_all_threads=None)
 
 
-def _precision(sample: int) -> int:
+def _precision_study(sample: int, time_steps: bool):
     """Dense-stride precision study: a 2048x2048 plane (seed 7), stride 1, the
     fixed 8x8 and 16x16 grids (2.70e12 evals), searched by the split-INT8
-    MFMA kernel and by the plain FP32 kernel.  Reports, per size, the FP32
-    kernel's error against binary64 -- its own binary32 score of the winner
-    it picked, and that winner after the binary64 re-score, against the exact
-    winner -- and checks both against the oracle on a sample of ranges."""
+    MFMA kernel and by the plain FP32 kernel.  Per size: the FP32 kernel's
+    error against binary64 -- its own binary32 score of the winner it picked,
+    and that winner after the binary64 re-score, against the exact winner --
+    and both kernels against the oracle on a sample of ranges.  With
+    time_steps, each kernel's step is also timed (a graph replay; the FP32
+    step alone takes about 14 s)."""
     from mars_amd import _native as N
     from mars_amd import synth
     from mars_amd.provider import EncodeParams, HipContext
@@ -263,16 +335,18 @@ def _precision(sample: int) -> int:
         t0 = time.perf_counter()
         ctx.prepare(img, params, [1])
         prep_s = time.perf_counter() - t0
-        t = ctx.run(iters=1, use_graph=True)
-        runs[name] = dict(ms_step=t.ms_step, evals=t.evals, prepare_s=prep_s,
-                          ms_by_size={str(2 << i): t.ms_by_size[i] for i in range(6) if t.ms_by_size[i] > 0},
-                          tables={(tt[0], tt[1]): tt for tt in ctx.tables()},
-                          scores=ctx.table_scores())
+        r = dict(prepare_s=prep_s, tables={(tt[0], tt[1]): tt for tt in ctx.tables()},
+                 scores=ctx.table_scores())
+        if time_steps:
+            t = ctx.run(iters=1, use_graph=True)
+            r.update(ms_step=t.ms_step, evals=t.evals,
+                     ms_by_size={str(2 << i): t.ms_by_size[i] for i in range(6) if t.ms_by_size[i] > 0})
+        runs[name] = r
         ctx.close()
     codes = lambda c: None if c is None else (c.dom_row, c.dom_col, c.isometry, c.qalfa, c.qbeta)
     per_size = {}
     rng = np.random.default_rng(0)
-    threads = min(os.cpu_count() or 1, 16)
+    threads = min(_host_threads(), 16)
     for key in sorted(runs["mfma_i8"]["tables"]):
         size, shift, rows, cols, exact, P = runs["mfma_i8"]["tables"][key]
         fp = runs["fp32"]["tables"][key][4]
@@ -294,8 +368,6 @@ def _precision(sample: int) -> int:
                                                   c.rms, c.t1_x4)
         per_size[str(size)] = {
             "ranges": len(rows), "positions": P, "evals": len(rows) * P * 8,
-            "mfma_ms": runs["mfma_i8"]["ms_by_size"].get(str(size)),
-            "fp32_ms": runs["fp32"]["ms_by_size"].get(str(size)),
             "fp32_ranges_with_other_codes": code_diff,
             "fp32_ranges_with_other_rms": sum(1 for d in rms_diff if d != 0.0),
             "fp32_rescored_rms_err_max": max(rms_diff), "fp32_rescored_rms_err_mean": float(np.mean(rms_diff)),
@@ -309,13 +381,24 @@ def _precision(sample: int) -> int:
                                                                       (ref[j].dom_row, ref[j].dom_col, ref[j].isometry,
                                                                        ref[j].qalfa, ref[j].qbeta))),
         }
+        if time_steps:
+            per_size[str(size)].update(mfma_ms=runs["mfma_i8"]["ms_by_size"].get(str(size)),
+                                       fp32_ms=runs["fp32"]["ms_by_size"].get(str(size)))
+    out = {"workload": "dense2048: 2048x2048 u8 plane (seed 7), stride 1, fixed 8x8 and 16x16 grids",
+           "per_size": per_size}
+    if time_steps:
+        out.update(mfma_ms_per_step=runs["mfma_i8"]["ms_step"], fp32_ms_per_step=runs["fp32"]["ms_step"],
+                   evals_per_step=runs["mfma_i8"]["evals"],
+                   mfma_evals_per_s=runs["mfma_i8"]["evals"] / (runs["mfma_i8"]["ms_step"] * 1e-3),
+                   fp32_evals_per_s=runs["fp32"]["evals"] / (runs["fp32"]["ms_step"] * 1e-3))
+    return out
+
+
+def _precision(sample: int) -> int:
+    study = _precision_study(sample, True)
     line = {"metric": "precision study: split-INT8 MFMA vs FP32 kernel, error vs binary64",
-            "config": {"workload": "dense2048: 2048x2048 u8 plane (seed 7), stride 1, fixed 8x8 and 16x16 grids"},
-            "mfma_ms_per_step": runs["mfma_i8"]["ms_step"], "fp32_ms_per_step": runs["fp32"]["ms_step"],
-            "evals_per_step": runs["mfma_i8"]["evals"],
-            "mfma_evals_per_s": runs["mfma_i8"]["evals"] / (runs["mfma_i8"]["ms_step"] * 1e-3),
-            "fp32_evals_per_s": runs["fp32"]["evals"] / (runs["fp32"]["ms_step"] * 1e-3),
-            "per_size": per_size}
+            "config": {"workload": study.pop("workload")}}
+    line.update(study)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -339,6 +422,38 @@ def _traffic_probe() -> int:
 
 
 COPY_PROBE_BYTES = 256 << 20
+CLOCK_HZ = 2.4e9    # MI355X peak engine clock
+N_SIMD = 1024       # 256 CUs x 4 SIMDs
+
+
+def _pool_bytes(w: int, h: int, sizes, shift: int) -> int:
+    """Algorithmic bytes of one plane's pool build: the u8 plane read, the
+    contracted plane as u16, and two u32 window sums per domain position of
+    every searched size."""
+    b = w * h + (w // 2) * (h // 2) * 2
+    for s_ in sizes:
+        if h >= 2 * s_ and w >= 2 * s_:
+            b += ((h - 2 * s_) // shift + 1) * ((w - 2 * s_) // shift + 1) * 8
+    return b
+
+
+def _profiled_counter(kernel: str, counter: str):
+    """One counter of `kernel` from the committed instruction-counter summary
+    (profiles/quadtree512_search_counters.md), or None."""
+    path = os.path.join(ROOT, "profiles", "quadtree512_search_counters.md")
+    if not os.path.exists(path):
+        return None
+    col = None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if cells and cells[0] == "counter":
+            col = cells.index(kernel) if kernel in cells else None
+        elif col is not None and cells and cells[0] == counter:
+            try:
+                return float(cells[col])
+            except (ValueError, IndexError):
+                return None
+    return None
 
 
 def _profiled_traffic(kernel: str):
@@ -372,8 +487,10 @@ def main() -> int:
                     help="one eager MFMA step + a 256 MiB copy kernel (for rocprofv3 --pmc)")
     ap.add_argument("--precision", action="store_true",
                     help="the 2048x2048 dense-stride precision study instead of the benchmark")
-    ap.add_argument("--precision-sample", type=int, default=24,
-                    help="ranges per size checked against the oracle in --precision")
+    ap.add_argument("--precision-sample", type=int, default=8,
+                    help="ranges per size checked against the oracle in the precision study")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 2048x2048 precision key of the default line")
     args = ap.parse_args()
     if args.traffic_probe:
         return _traffic_probe()
@@ -391,7 +508,11 @@ def main() -> int:
               "with no exchange but a per-range winner gather)", file=sys.stderr)
         return 2
 
-    img, res = _gpu_leg(args.steps, args.warmup)
+    def cpu_hook(evals_by_size, gpu_run):
+        from mars_amd import synth
+        return _cpu_leg(synth.plane(W, H, SEED), evals_by_size, gpu_run)
+
+    img, res = _gpu_leg(args.steps, args.warmup, None if args.no_cpu else cpu_hook)
     mf, fp = res["mfma_i8"], res["fp32"]
     evals = mf["evals"]
     value = evals / (mf["ms_step"] * 1e-3)
@@ -415,7 +536,17 @@ def main() -> int:
                                          "seeds 100..163), quadtree 32->4, stride 4",
                                          synth.batch_specs(), 3)}
     colour = None if args.no_colour else _colour_leg(args.steps)
-    cpu = None if args.no_cpu else _cpu_leg(img)
+    cpu = res.get("cpu")
+    if cpu is not None:
+        cpu["gpu_vs_cpu_all_threads"] = value / cpu["value"]
+    precision = None if args.no_precision else _precision_study(args.precision_sample, False)
+    # VALU issue of the dominant launch: its SQ_INSTS_VALU (committed
+    # counter summary) at 4 cycles per wave64 instruction on one SIMD,
+    # against every SIMD's cycles over the launch measured here.
+    valu = _profiled_counter(f"k_search_mfma<{s}>", "SQ_INSTS_VALU")
+    valu_frac = valu * 4 / (N_SIMD * CLOCK_HZ * dom_ms * 1e-3) if valu else None
+    pool_b = _pool_bytes(W, H, [int(k) for k in mf["evals_by_size"]], SHIFT)
+    pool_gbs = pool_b / (mf["ms_pool_build"] * 1e-3) / 1e9
     line = {
         "metric": "R×D×iso evals/s, split-INT8 MFMA vs FP32 kernel; batched img/s @512²",
         "value": value,
@@ -450,7 +581,17 @@ def main() -> int:
                      "peak": PEAK_INT8_OPS / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_INT8_OPS,
                      "traffic": _profiled_traffic(f"k_search_mfma<{s}>"),
                      "op": "int8 ops (2*s^2 per eval)",
-                     "avg_launch_ms": dom_ms},
+                     "avg_launch_ms": dom_ms,
+                     "valu_frac": valu_frac,
+                     "valu_frac_basis": "SQ_INSTS_VALU of the launch (profiles/quadtree512_search_counters.md) "
+                                        "x 4 cycles / (1024 SIMDs x 2.4 GHz x avg_launch_ms)",
+                     "pool": {"bound": "hbm", "achieved": pool_gbs, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                              "frac": pool_gbs * 1e9 / PEAK_HBM, "bytes": pool_b,
+                              "ms": mf["ms_pool_build"],
+                              "kernels": "k_contract, k_sat_rows, k_sat_cols, k_possums (serial)",
+                              "basis": "W*H read + (W/2)(H/2) u16 contracted plane + 8 B per domain "
+                                       "position of every size"}},
+        "precision": precision,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -77,7 +77,7 @@ typedef struct {
 /* Per-run device timings (milliseconds, HIP events on the context stream). */
 typedef struct {
     double ms_step;     /* whole step: pool build + range prep + search + merge */
-    double ms_pool;     /* contraction + per-size window sums */
+    double ms_pool;     /* preparation: pool build, range operands and sums, threshold seed */
     double ms_search;   /* search kernels, all sizes */
     double ms_by_size[6]; /* search kernel per size 2,4,8,16,32,64 (index log2(size)-1) */
     uint64_t evals;     /* range x position x isometry candidates per step (walk-only
@@ -87,6 +87,8 @@ typedef struct {
     uint64_t exact_fits_by_size[6]; /* candidates that took the verbatim binary64 fit */
     double ms_fisher;      /* Fisher classification, buckets and bucket scans */
     uint64_t fisher_evals; /* Fisher candidates fitted per step */
+    double ms_pool_build;  /* the domain pool alone: contraction, split planes, summed-area
+                              tables, per-position window sums (within ms_pool) */
 } mars_hip_timing;
 
 /* One plane of a batch: w x h u8 pixels, row-major. */

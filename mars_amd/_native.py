@@ -46,7 +46,7 @@ class Timing(C.Structure):
                 ("ms_by_size", C.c_double * 6), ("evals", C.c_uint64),
                 ("survivors", C.c_uint64), ("survivors_by_size", C.c_uint64 * 6),
                 ("exact_fits_by_size", C.c_uint64 * 6), ("ms_fisher", C.c_double),
-                ("fisher_evals", C.c_uint64)]
+                ("fisher_evals", C.c_uint64), ("ms_pool_build", C.c_double)]
 
 
 class Plane(C.Structure):

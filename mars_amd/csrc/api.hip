@@ -643,7 +643,8 @@ int layout(mars_hip_ctx *ctx)
 // pixels, so with `fork` they run on a side stream beside the domain pool
 // (contraction, summed-area tables, position sums) and join before the seed,
 // which needs both; without, everything runs in order on st.
-int enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st, bool fork)
+int enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st, bool fork,
+                 hipEvent_t *pool_ev = nullptr)
 {
     const Work2 *w2 = ctx->d_work2;
     hipStream_t rs = st;
@@ -656,11 +657,13 @@ int enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st, bool fork)
                         kernel == MARS_HIP_KERNEL_FP32, rs);
     mars::launch_rstats(ctx->d_tabs, w2 + ctx->wl_rstats.off, ctx->wl_rstats.n, rs);
     if (fork) HIP_TRY(ctx, hipEventRecord(ctx->prep_join_ev, rs));
+    if (pool_ev) HIP_TRY(ctx, hipEventRecord(pool_ev[0], st));
     mars::launch_contract(ctx->d_planes, w2 + ctx->wl_contract.off, ctx->wl_contract.n,
                           ctx->d_counters, kCounters, st);
     mars::launch_sat(ctx->d_planes, w2 + ctx->wl_sat_rows.off, ctx->wl_sat_rows.n,
                      w2 + ctx->wl_sat_cols.off, ctx->wl_sat_cols.n, st);
     mars::launch_possums(ctx->d_tabs, w2 + ctx->wl_possums.off, ctx->wl_possums.n, st);
+    if (pool_ev) HIP_TRY(ctx, hipEventRecord(pool_ev[1], st));
     if (fork) HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->prep_join_ev, 0));
     if (kernel == MARS_HIP_KERNEL_MFMA_I8 && ctx->walk_prepared <= 0.0)
         mars::launch_seed(ctx->d_tabs, w2 + ctx->wl_seed.off, ctx->wl_seed.n, ctx->common, st);
@@ -706,7 +709,7 @@ void enqueue_fisher(mars_hip_ctx *ctx, hipStream_t st)
                         w2 + ctx->wl_frange.off, ctx->wl_frange.n, ctx->common, st);
 }
 
-// Enqueue one whole step.  With ev (4 + n_groups events: before the pool,
+// Enqueue one whole step.  With ev (6 + n_groups events: before the pool,
 // after the preparation, after each group's search, after the merges, after
 // the Fisher search) the step runs serially on the context stream so each
 // phase can be timed; without, each group's search and merge run on a side
@@ -716,7 +719,7 @@ int enqueue_step(mars_hip_ctx *ctx, int kernel, hipEvent_t *ev)
 {
     hipStream_t st = ctx->stream;
     if (ev) HIP_TRY(ctx, hipEventRecord(ev[0], st));
-    RC_TRY(enqueue_prep(ctx, kernel, st, ev == nullptr));
+    RC_TRY(enqueue_prep(ctx, kernel, st, ev == nullptr, ev ? ev + 4 + ctx->groups.size() : nullptr));
     if (ctx->walk_prepared > 0.0) {
         // Walk-only batch: the sizes in turn, largest first, each group's
         // ranges activated from its parent group's merged winners, then
@@ -830,7 +833,7 @@ uint64_t table_evals(const Table &t) { return (uint64_t)t.P * 8; }
 int timed_step(mars_hip_ctx *ctx, mars_hip_timing *tm, std::vector<double> *group_ms)
 {
     const size_t ng = ctx->groups.size();
-    std::vector<hipEvent_t> ev(4 + ng);
+    std::vector<hipEvent_t> ev(6 + ng);
     for (auto &e : ev) HIP_TRY(ctx, hipEventCreate(&e));
     int rc = enqueue_step(ctx, ctx->kernel, ev.data());
     if (rc == 0) {
@@ -842,6 +845,8 @@ int timed_step(mars_hip_ctx *ctx, mars_hip_timing *tm, std::vector<double> *grou
         *tm = mars_hip_timing{};
         (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
         tm->ms_pool = ms;
+        (void)hipEventElapsedTime(&ms, ev[4 + ng], ev[5 + ng]);
+        tm->ms_pool_build = ms;
     }
     if (group_ms) group_ms->assign(ng, 0.0);
     for (size_t i = 0; rc == 0 && i < ng; i++) {
