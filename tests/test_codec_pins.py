@@ -2,7 +2,9 @@
 
 crates/mars-codec/tests/search_provider.rs:13-67 and 179-227: four encodes of a
 closed-form 16x16 plane whose `.mars` bytes (length + SHA-256) and search eval
-counts the reference pins.  The chain here (walks, RD, rate snapshot, format,
+counts the reference pins; and the CLI-A stream (crates/mars-cli/tests/
+cli_a_gate.rs:55-56, 224-249), a 128x128 RD encode with all four leaf modes in
+a MARC grey container.  The chain here (walks, RD, rate snapshot, format,
 rANS) runs with the oracle's exhaustive search as its provider, so these pin
 both the oracle search and the host-side chain.  CPU only.
 """
@@ -66,3 +68,34 @@ def test_mars_header_layout():
     wrapped = wrap_gray(data)
     assert wrapped[:4] == b"MARC" and wrapped[4:7] == bytes([0, 0, 1])
     assert int.from_bytes(wrapped[7:11], "little") == len(data) and wrapped[11:] == data
+
+
+# crates/mars-cli/tests/cli_a_gate.rs:55-56, 224-249: `encmars` on a 128x128
+# gentle gradient (100 + (r + c) % 5), lambda 200, modes 0-3, t_rms 0 and the
+# CLI defaults (sizes 4..16, shift 4, bits 4/7, max_alfa 1, fixed residual
+# step 8, rd_candidates 1) writes a MARC grey container with this SHA-256 when
+# --adaptive-density is omitted; with it the stream must differ.
+CLI_A_STREAM_SHA256 = "55ccaacf9b9c0e952673992404cfdf8ebb21e8503ae3d45a67b4e82a5eed3544"
+
+
+def cli_a_gradient() -> np.ndarray:
+    r, c = np.mgrid[0:128, 0:128]
+    return (100 + (r + c) % 5).astype(np.uint8)
+
+
+def cli_a_params():
+    return EncodeParams(min_size=4, max_size=16, shift=4, bits_alfa=4, bits_beta=7, max_alfa=1.0,
+                        t_rms=0.0, zero_threshold=0, lambda_=200.0)
+
+
+def test_cli_a_default_stream():
+    img = cli_a_gradient()
+    streams = []
+    for density in (False, True):
+        prov = O.Provider(img)
+        data, _ = encode_plane(img, cli_a_params(),
+                               EncodeOptions(allowed_modes=(True, True, True, True),
+                                             adaptive_density=density), prov.fn, prov.user)
+        streams.append(wrap_gray(data))
+    assert hashlib.sha256(streams[0]).hexdigest() == CLI_A_STREAM_SHA256
+    assert streams[1] != streams[0]

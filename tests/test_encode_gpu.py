@@ -85,3 +85,19 @@ def test_quantiser_settings_odd_geometry(max_alfa, bits_alfa, bits_beta, zero, l
     ref, ref_st, got, st = _both(img, p, o, [4])
     assert (st.search_evals, st.warmup_evals) == (ref_st.search_evals, ref_st.warmup_evals)
     assert got == ref
+
+
+def test_cli_a_stream_from_the_gpu_table():
+    """The reference's CLI-A stream (cli_a_gate.rs:55-56): the same encode with the
+    GPU search table as provider gives the pinned MARC bytes."""
+    import hashlib as H
+
+    from mars_amd.codec import wrap_gray
+    from tests.test_codec_pins import CLI_A_STREAM_SHA256, cli_a_gradient, cli_a_params
+
+    img = cli_a_gradient()
+    ctx = HipContext(0)
+    data, _ = hip_encode_plane(ctx, img, cli_a_params(),
+                               EncodeOptions(allowed_modes=(True, True, True, True)))
+    ctx.close()
+    assert H.sha256(wrap_gray(data)).hexdigest() == CLI_A_STREAM_SHA256
