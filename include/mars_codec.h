@@ -28,6 +28,9 @@
  *                          (color.rs:400-489, RGB arm, no colour regions)
  *                          with each plane encoded through
  *                          mars_hip_encode_plane_method.
+ *   mars_read, mars_write  mars_format::read / write (mars_format.rs:232-388)
+ *   mars_ifs_parse         ifs::parse (ifs.rs:183-278)
+ *   mars_decode_iterative  ifs::decode_iterative (ifs.rs:281-309, 406-484)
  *
  * Status: 0 ok, < 0 error (MARS_HIP_E_* codes of mars_hip.h).
  */
@@ -159,6 +162,76 @@ int mars_hip_encode_plane_method(mars_hip_ctx *ctx, const uint8_t *px, uint32_t 
                                  const mars_encode_params *p, const mars_encode_options *o,
                                  uint32_t method, uint8_t **out, size_t *out_len,
                                  mars_encode_stats *stats);
+
+/* ---- reading streams back: the `.mars` v0 reader, the legacy `.ifs` parser
+ * and the iterative decoder (all CPU) ---- */
+
+/* Typed errors of the readers, mars_format.rs:141-181 MarsFormatError and
+ * ifs.rs:130-156 IfsError (one code per variant; the field values the Rust
+ * variants carry are not reported). */
+#define MARS_FMT_E_BAD_MAGIC (-101)
+#define MARS_FMT_E_INVALID_QSTEP (-102)
+#define MARS_FMT_E_UNSUPPORTED_VERSION (-103)
+#define MARS_FMT_E_TRUNCATED (-104)
+#define MARS_FMT_E_DEGENERATE_HEADER (-105)
+#define MARS_FMT_E_SECTION_OUT_OF_BOUNDS (-106)
+#define MARS_FMT_E_MISSING_TREE (-107)
+#define MARS_FMT_E_INVALID_TREE_PAYLOAD (-108) /* lib.rs:292-297 InvalidCompressedData */
+#define MARS_FMT_E_DOMAIN_OUT_OF_BOUNDS (-109)
+#define MARS_FMT_E_TRAILING_BITS (-110)         /* .ifs: 8 or more bits after the tree */
+#define MARS_FMT_E_NONZERO_PADDING (-111)       /* .ifs */
+/* A context alphabet above 2^20 (bits_alfa or bits_beta above 20): the
+ * format allows up to 24, but every decoded symbol scans its alphabet; the
+ * reference panics at 24. */
+#define MARS_FMT_E_UNSUPPORTED (-112)
+
+/* mars_format.rs:59-64 Header: the ifs.rs Header geometry plus the residual
+ * step as binary32 bits (quant.rs:18-64 ResidualQstep). */
+typedef struct {
+    uint32_t min_size, max_size, shift, bits_alfa, bits_beta, int_max_alfa;
+    uint32_t width, height;
+    uint32_t residual_qstep_bits;
+    uint32_t _pad;
+} mars_header;
+
+/* ifs.rs:100-117 Leaf.  A mode-3 leaf's size*size residual levels are
+ * residual[residual_off .. residual_off + residual_len) of the array that
+ * comes with the leaves (residual_len = 0 for the other modes). */
+typedef struct {
+    uint32_t row, col, size, mode;
+    uint32_t qalfa, qbeta, isometry, dom_row, dom_col;
+    int32_t qgx, qgy;
+    uint32_t residual_off, residual_len;
+} mars_leaf;
+
+/*
+ * mars_format::read (mars_format.rs:302-388): decode a `.mars` v0 plane
+ * stream into its header and leaves (traversal order: NW, SW, NE, SE).
+ * Arbitrary input returns 0 or a MARS_FMT_E_* code, never crashes, and does
+ * work bounded by the header (at most 2^24 leaf positions).  *leaves and
+ * *residual are malloc'd (free with mars_free; null when empty).
+ */
+int mars_read(const uint8_t *data, size_t len, mars_header *hdr, mars_leaf **leaves,
+              size_t *n_leaves, int32_t **residual, size_t *n_residual);
+
+/* mars_format::write (mars_format.rs:232-297) of a header and leaves. */
+int mars_write(const mars_header *hdr, const mars_leaf *leaves, size_t n_leaves,
+               const int32_t *residual, size_t n_residual, uint8_t **out, size_t *out_len);
+
+/* ifs::parse (ifs.rs:183-278): a legacy Mars 1 `.ifs` bitstream. */
+int mars_ifs_parse(const uint8_t *data, size_t len, mars_header *hdr, mars_leaf **leaves,
+                   size_t *n_leaves);
+
+/*
+ * ifs::decode_iterative (ifs.rs:281-309, decode_leaf 406-484): `iterations`
+ * fixed-point iterations from a flat 128 image into out (width * height
+ * bytes).  Leaves are checked first (inside the plane, domain inside the
+ * plane, mode 0..3, isometry 0..7, residual length size^2 for mode 3);
+ * MARS_HIP_E_ARG otherwise.
+ */
+int mars_decode_iterative(const mars_header *hdr, const mars_leaf *leaves, size_t n_leaves,
+                          const int32_t *residual, size_t n_residual, uint32_t iterations,
+                          uint8_t *out);
 
 /* color.rs:338-345 ColorEncodeStats, with each plane's encode counters. */
 typedef struct {

@@ -58,6 +58,36 @@ SEARCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                         C.POINTER(C.c_uint64))
 
 
+# Typed errors of the readers (mars_format.rs:141-181 MarsFormatError, ifs.rs:130-156).
+FMT_E_BAD_MAGIC, FMT_E_INVALID_QSTEP, FMT_E_UNSUPPORTED_VERSION, FMT_E_TRUNCATED = -101, -102, -103, -104
+FMT_E_DEGENERATE_HEADER, FMT_E_SECTION_OUT_OF_BOUNDS, FMT_E_MISSING_TREE = -105, -106, -107
+FMT_E_INVALID_TREE_PAYLOAD, FMT_E_DOMAIN_OUT_OF_BOUNDS, FMT_E_TRAILING_BITS = -108, -109, -110
+FMT_E_NONZERO_PADDING, FMT_E_UNSUPPORTED = -111, -112
+
+
+class CHeader(C.Structure):
+    """mars_format.rs:59-64 Header (geometry + residual step bits)."""
+    _fields_ = [(n, C.c_uint32) for n in ("min_size", "max_size", "shift", "bits_alfa",
+                                          "bits_beta", "int_max_alfa", "width", "height",
+                                          "residual_qstep_bits", "_pad")]
+
+
+class CLeaf(C.Structure):
+    """ifs.rs:100-117 Leaf (residual levels in a separate array)."""
+    _fields_ = [(n, C.c_uint32) for n in ("row", "col", "size", "mode", "qalfa", "qbeta",
+                                          "isometry", "dom_row", "dom_col")] + \
+               [("qgx", C.c_int32), ("qgy", C.c_int32), ("residual_off", C.c_uint32),
+                ("residual_len", C.c_uint32)]
+
+
+class FormatError(ValueError):
+    """A reader rejected its input; .code is the MARS_FMT_E_* status."""
+
+    def __init__(self, code: int):
+        super().__init__(f"stream rejected with status {code}")
+        self.code = code
+
+
 @dataclass(frozen=True)
 class EncodeOptions:
     """encode.rs:76-101 (no lambda regions; one RD candidate)."""
@@ -116,6 +146,18 @@ def _lib():
         L.mars_hip_encode_rgb.restype = C.c_int
         L.mars_free.argtypes = [C.c_void_p]
         L.mars_free.restype = None
+        L.mars_read.argtypes = [P(C.c_uint8), C.c_size_t, P(CHeader), P(P(CLeaf)),
+                                P(C.c_size_t), P(P(C.c_int32)), P(C.c_size_t)]
+        L.mars_read.restype = C.c_int
+        L.mars_write.argtypes = [P(CHeader), P(CLeaf), C.c_size_t, P(C.c_int32), C.c_size_t,
+                                 P(P(C.c_uint8)), P(C.c_size_t)]
+        L.mars_write.restype = C.c_int
+        L.mars_ifs_parse.argtypes = [P(C.c_uint8), C.c_size_t, P(CHeader), P(P(CLeaf)),
+                                     P(C.c_size_t)]
+        L.mars_ifs_parse.restype = C.c_int
+        L.mars_decode_iterative.argtypes = [P(CHeader), P(CLeaf), C.c_size_t, P(C.c_int32),
+                                            C.c_size_t, C.c_uint32, P(C.c_uint8)]
+        L.mars_decode_iterative.restype = C.c_int
         L._codec_bound = True
     return L
 
@@ -293,3 +335,91 @@ def hip_encode_rgb(ctx: HipContext, rgb: np.ndarray, y_params: EncodeParams,
     ctx._check(rc, "mars_hip_encode_rgb")
     stats = {"y": _stats(st.y), "cb": _stats(st.cb), "cr": _stats(st.cr)}
     return _take(L, buf, n), stats, (st.y_bytes, st.cb_bytes, st.cr_bytes)
+
+
+# ---- reading streams back ---------------------------------------------------------
+
+def _leaf_tuple(c: CLeaf, res):
+    """(row, col, size, mode, qalfa, qbeta, isometry, dom_row, dom_col, qgx, qgy, residual)."""
+    r = tuple(res[c.residual_off:c.residual_off + c.residual_len]) if c.residual_len else ()
+    return (c.row, c.col, c.size, c.mode, c.qalfa, c.qbeta, c.isometry, c.dom_row, c.dom_col,
+            c.qgx, c.qgy, r)
+
+
+def _header_dict(h: CHeader) -> dict:
+    return {n: getattr(h, n) for n, _ in CHeader._fields_ if n != "_pad"}
+
+
+def read_mars(data: bytes):
+    """mars_format::read: (header dict, leaves as tuples -- see _leaf_tuple) or FormatError."""
+    L = _lib()
+    buf = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(data or b"\0")
+    hdr = CHeader()
+    lp = C.POINTER(CLeaf)()
+    rp = C.POINTER(C.c_int32)()
+    n, nr = C.c_size_t(), C.c_size_t()
+    rc = L.mars_read(buf, len(data), C.byref(hdr), C.byref(lp), C.byref(n), C.byref(rp),
+                     C.byref(nr))
+    if rc != N.OK:
+        raise FormatError(rc)
+    try:
+        res = [rp[i] for i in range(nr.value)]
+        leaves = [_leaf_tuple(lp[i], res) for i in range(n.value)]
+    finally:
+        L.mars_free(C.cast(lp, C.c_void_p))
+        L.mars_free(C.cast(rp, C.c_void_p))
+    return _header_dict(hdr), leaves
+
+
+def _c_leaves(leaves):
+    arr = (CLeaf * max(len(leaves), 1))()
+    res = []
+    for i, t in enumerate(leaves):
+        row, col, size, mode, qa, qb, iso, dr, dc, gx, gy, r = t
+        arr[i] = CLeaf(row, col, size, mode, qa, qb, iso, dr, dc, gx, gy, len(res), len(r))
+        res.extend(r)
+    rarr = (C.c_int32 * max(len(res), 1))(*res)
+    return arr, rarr, len(res)
+
+
+def write_mars(header: dict, leaves) -> bytes:
+    """mars_format::write of a header dict and leaf tuples (as read_mars returns them)."""
+    L = _lib()
+    h = CHeader(**header)
+    arr, rarr, nres = _c_leaves(leaves)
+    out = C.POINTER(C.c_uint8)()
+    n = C.c_size_t()
+    rc = L.mars_write(C.byref(h), arr, len(leaves), rarr, nres, C.byref(out), C.byref(n))
+    if rc != N.OK:
+        raise FormatError(rc)
+    return _take(L, out, n)
+
+
+def ifs_parse(data: bytes):
+    """ifs::parse of a legacy Mars 1 `.ifs` bitstream: (header dict, leaf tuples)."""
+    L = _lib()
+    buf = (C.c_uint8 * max(len(data), 1)).from_buffer_copy(data or b"\0")
+    hdr = CHeader()
+    lp = C.POINTER(CLeaf)()
+    n = C.c_size_t()
+    rc = L.mars_ifs_parse(buf, len(data), C.byref(hdr), C.byref(lp), C.byref(n))
+    if rc != N.OK:
+        raise FormatError(rc)
+    try:
+        leaves = [_leaf_tuple(lp[i], []) for i in range(n.value)]
+    finally:
+        L.mars_free(C.cast(lp, C.c_void_p))
+    return _header_dict(hdr), leaves
+
+
+def decode_iterative(header: dict, leaves, iterations: int) -> np.ndarray:
+    """ifs::decode_iterative: `iterations` fixed-point iterations from flat 128."""
+    L = _lib()
+    h = CHeader(**header)
+    arr, rarr, nres = _c_leaves(leaves)
+    out = np.zeros((header["height"], header["width"]), np.uint8)
+    rc = L.mars_decode_iterative(C.byref(h), arr, len(leaves), rarr, nres, iterations,
+                                 out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc != N.OK:
+        raise ValueError(f"mars_decode_iterative failed with status {rc}")
+    return out

@@ -122,6 +122,24 @@ struct AdaptiveModel {
         left_and_prob(symbol, &l, &p);
         return -std::log2((double)p / (double)PRECISION_TOTAL);
     }
+    // lib.rs:104-121: the symbol whose fixed-point interval holds `quantile`
+    // (< 2^24, which the last symbol's right end always reaches).
+    void symbol_for_quantile(uint32_t quantile, uint32_t *sym, uint32_t *left, uint32_t *prob) const
+    {
+        uint64_t cum = 0;
+        const uint32_t n = (uint32_t)counts.size();
+        for (uint32_t i = 0; i < n; i++) {
+            const uint64_t next = cum + counts[i];
+            const uint32_t l = scaled(cum) + i, r = scaled(next) + i + 1;
+            if (quantile < r || i + 1 == n) {
+                *sym = i;
+                *left = l;
+                *prob = r - l;
+                return;
+            }
+            cum = next;
+        }
+    }
 };
 
 typedef std::pair<uint8_t, uint32_t> ContextKey;
@@ -350,6 +368,365 @@ int format_write(const Header &h, const std::vector<Leaf> &leaves, std::vector<u
     for (int i = 0; i < 4; i++) out.push_back((uint8_t)(tl >> (8 * i)));
     out.insert(out.end(), tree.begin(), tree.end());
     return MARS_HIP_OK;
+}
+
+// ---------------------------------------------------------------- read side
+
+double dead_zone_dequantize(int32_t level, double step, double dz);
+std::vector<double> inverse_dct2d(const std::vector<double> &coef, size_t size);
+uint32_t qstep_bits(double step);
+
+// The decode half of the stack rANS coder above (constriction's AnsCoder
+// from_compressed / decode_symbol, as the reference's Decoder uses it,
+// lib.rs:292-334): bytes become little-endian words (a trailing partial word
+// is dropped); the state is read from the end of the words -- the first word
+// popped must be nonzero, and a second is shifted in below it while the
+// state is below 2^32; each symbol takes quantile = state mod 2^24, then
+// state = (state >> 24) * p + (quantile - left), and a word is shifted in
+// whenever the state falls below 2^32 and words remain.
+struct AnsDecoder {
+    std::vector<uint32_t> bulk;
+    uint64_t state = 0;
+    bool init(const uint8_t *p, size_t n)
+    {
+        bulk.resize(n / 4);
+        for (size_t i = 0; i < bulk.size(); i++)
+            bulk[i] = (uint32_t)p[4 * i] | ((uint32_t)p[4 * i + 1] << 8) |
+                      ((uint32_t)p[4 * i + 2] << 16) | ((uint32_t)p[4 * i + 3] << 24);
+        state = 0;
+        if (bulk.empty()) return true;
+        const uint32_t first = bulk.back();
+        bulk.pop_back();
+        if (first == 0) return false; // never produced by the encoder
+        state = first;
+        while (!bulk.empty() && state < (1ull << 32)) {
+            state = (state << 32) | bulk.back();
+            bulk.pop_back();
+        }
+        return true;
+    }
+    uint32_t decode(const AdaptiveModel &m)
+    {
+        const uint32_t quantile = (uint32_t)(state & (PRECISION_TOTAL - 1));
+        uint32_t sym, left, prob;
+        m.symbol_for_quantile(quantile, &sym, &left, &prob);
+        state = (state >> PRECISION) * prob + (quantile - left);
+        if (state < (1ull << 32) && !bulk.empty()) {
+            state = (state << 32) | bulk.back();
+            bulk.pop_back();
+        }
+        return sym;
+    }
+};
+
+// The reader's largest alphabet.  The format allows bits_alfa/bits_beta up
+// to 24 (mars_format.rs:157), but the reference's AdaptiveModel::new
+// asserts an alphabet below 2^24 (so a bits_beta = 24 stream panics there)
+// and every decode scans the whole alphabet; this reader refuses alphabets
+// above 2^20 (bits above 20) with MARS_FMT_E_UNSUPPORTED instead, which keeps
+// the work per symbol bounded on arbitrary input.
+constexpr uint32_t kMaxReadAlphabet = 1u << 20;
+
+// lib.rs:312-334 Decoder: one model per context, created on first use with
+// that event's alphabet.
+struct EntropyDecoder {
+    AnsDecoder ans;
+    Models models;
+    bool bad = false; // an alphabet this reader does not take
+    uint32_t next(ContextKey ctx, uint32_t alphabet)
+    {
+        auto it = models.find(ctx);
+        if (it == models.end()) {
+            if (alphabet < 2 || alphabet > kMaxReadAlphabet) {
+                bad = true;
+                return 0;
+            }
+            it = models.emplace(ctx, AdaptiveModel(alphabet)).first;
+        }
+        const uint32_t s = ans.decode(it->second);
+        it->second.update(s);
+        return s;
+    }
+};
+
+int64_t unzigzag64(uint32_t z) { return (int64_t)(z >> 1) ^ -(int64_t)(z & 1); }
+
+// mars_format.rs:596-718 walk_read.
+int walk_read(const Header &h, uint32_t row, uint32_t col, uint32_t size, EntropyDecoder &dec,
+              Predictor &pred, std::vector<Leaf> &leaves)
+{
+    if (row >= h.height || col >= h.width) return MARS_HIP_OK;
+    const uint32_t half = size / 2;
+    const bool forced = size > h.max_size || row + size > h.height || col + size > h.width;
+    const uint32_t sc = (uint32_t)__builtin_ctz(size);
+    bool split = forced;
+    if (!forced && size > h.min_size) split = dec.next({FIELD_SPLIT, sc}, 2) == 1;
+    if (dec.bad) return MARS_FMT_E_UNSUPPORTED;
+    if (split) {
+        int rc = walk_read(h, row, col, half, dec, pred, leaves);
+        if (!rc) rc = walk_read(h, row + half, col, half, dec, pred, leaves);
+        if (!rc) rc = walk_read(h, row, col + half, half, dec, pred, leaves);
+        if (!rc) rc = walk_read(h, row + half, col + half, half, dec, pred, leaves);
+        return rc;
+    }
+    Leaf leaf{};
+    leaf.row = row;
+    leaf.col = col;
+    leaf.size = size;
+    const uint32_t mode = dec.next({FIELD_MODE, sc}, MODE_ALPHABET);
+    leaf.mode = (uint8_t)mode;
+    if (mode <= 1) {
+        leaf.qbeta = dec.next({FIELD_QBETA_DC, sc}, 1u << h.bits_beta);
+        if (mode == 1) {
+            const uint32_t ga = (uint32_t)(2 * (int64_t)AFFINE_GRAD_CLAMP + 1);
+            leaf.qgx = (int32_t)unzigzag64(dec.next({FIELD_GX, sc}, ga));
+            leaf.qgy = (int32_t)unzigzag64(dec.next({FIELD_GY, sc}, ga));
+        }
+        if (dec.bad) return MARS_FMT_E_UNSUPPORTED;
+        leaves.push_back(std::move(leaf));
+        return MARS_HIP_OK;
+    }
+    leaf.qalfa = dec.next({FIELD_QALFA, sc}, (1u << h.bits_alfa) - 1) + 1;
+    leaf.qbeta = dec.next({FIELD_QBETA, sc}, 1u << h.bits_beta);
+    leaf.isometry = (uint8_t)dec.next({FIELD_ISOMETRY, sc}, 8);
+    const int64_t rr = 1ll << h.bits_coord_row(), cr = 1ll << h.bits_coord_col();
+    const int64_t rd = unzigzag64(dec.next({FIELD_DOM_ROW, sc}, (uint32_t)(2 * rr)));
+    const int64_t cd = unzigzag64(dec.next({FIELD_DOM_COL, sc}, (uint32_t)(2 * cr)));
+    if (dec.bad) return MARS_FMT_E_UNSUPPORTED;
+    const int64_t ru = pred.prev_row_units + rd, cu = pred.prev_col_units + cd;
+    pred.prev_row_units = ru;
+    pred.prev_col_units = cu;
+    // (In 64 bits: the reference multiplies in u32, which a corrupt stream's
+    // accumulated deltas can overflow.)
+    if (ru < 0 || cu < 0 || ru > 0xffffffffll || cu > 0xffffffffll)
+        return MARS_FMT_E_DOMAIN_OUT_OF_BOUNDS;
+    const uint64_t dr = (uint64_t)h.shift * (uint64_t)ru, dc = (uint64_t)h.shift * (uint64_t)cu;
+    if (dr + 2ull * size > h.height || dc + 2ull * size > h.width) return MARS_FMT_E_DOMAIN_OUT_OF_BOUNDS;
+    leaf.dom_row = (uint32_t)dr;
+    leaf.dom_col = (uint32_t)dc;
+    if (mode == 3) {
+        // residual.rs:64-80 decode_values
+        const size_t n = (size_t)size * size;
+        leaf.residual.resize(n);
+        for (size_t i = 0; i < n; i++) {
+            const uint32_t band = std::min<uint32_t>((uint32_t)((i * BANDS) / n), BANDS - 1);
+            if (dec.next({FIELD_RESID_NZ, sc * BANDS + band}, 2) == 1) {
+                const uint32_t z = dec.next({FIELD_RESID_MAG, sc * BANDS + band},
+                                            (uint32_t)(2 * LEVEL_CLAMP + 1));
+                leaf.residual[i] = (int32_t)(z >> 1) ^ -(int32_t)(z & 1);
+            }
+        }
+    }
+    leaves.push_back(std::move(leaf));
+    return MARS_HIP_OK;
+}
+
+bool header_fields_valid(const Header &h)
+{
+    return h.width != 0 && h.height != 0 && h.shift != 0 && h.bits_alfa >= 2 && h.bits_alfa <= 24 &&
+           h.bits_beta >= 1 && h.bits_beta <= 24 && h.min_size != 0 && h.max_size >= h.min_size;
+}
+
+bool leaf_count_within_bound(const Header &h)
+{
+    const uint64_t vs = h.virtual_size(), ms = h.min_size;
+    return (vs / ms) * (vs / ms) <= (1ull << 24);
+}
+
+// mars_format.rs:302-388 read.
+int format_read(const uint8_t *data, size_t len, Header &h, std::vector<Leaf> &leaves)
+{
+    leaves.clear();
+    if (len < 24) return MARS_FMT_E_TRUNCATED;
+    if (memcmp(data, "MARS", 4) != 0) return MARS_FMT_E_BAD_MAGIC;
+    if (data[4] != 0) return MARS_FMT_E_UNSUPPORTED_VERSION;
+    uint32_t qb = (uint32_t)data[20] | ((uint32_t)data[21] << 8) | ((uint32_t)data[22] << 16) |
+                  ((uint32_t)data[23] << 24);
+    float qf;
+    memcpy(&qf, &qb, 4);
+    const double qd = (double)qf;
+    if (!(std::isfinite(qd) && qd >= 1.0 && qd <= 65535.0)) return MARS_FMT_E_INVALID_QSTEP;
+    h = Header{};
+    h.qstep_bits = qb;
+    h.min_size = data[9];
+    h.max_size = data[10];
+    h.shift = data[11];
+    h.bits_alfa = data[12];
+    h.bits_beta = data[13];
+    h.int_max_alfa = data[14];
+    h.width = (uint32_t)data[15] | ((uint32_t)data[16] << 8);
+    h.height = (uint32_t)data[17] | ((uint32_t)data[18] << 8);
+    const uint32_t nsec = data[19];
+    if (!header_fields_valid(h) || !leaf_count_within_bound(h)) return MARS_FMT_E_DEGENERATE_HEADER;
+    size_t off = 24;
+    const uint8_t *tree = nullptr;
+    size_t tree_len = 0;
+    bool have_tree = false;
+    for (uint32_t s = 0; s < nsec; s++) {
+        if (off + 5 > len) return MARS_FMT_E_TRUNCATED;
+        const uint8_t id = data[off];
+        const size_t sl = (size_t)data[off + 1] | ((size_t)data[off + 2] << 8) |
+                          ((size_t)data[off + 3] << 16) | ((size_t)data[off + 4] << 24);
+        off += 5;
+        if (sl > len - off) return MARS_FMT_E_SECTION_OUT_OF_BOUNDS;
+        if (id == 1) { // the last TREE section wins, as in the reference
+            tree = data + off;
+            tree_len = sl;
+            have_tree = true;
+        }
+        off += sl;
+    }
+    if (!have_tree) return MARS_FMT_E_MISSING_TREE;
+    EntropyDecoder dec;
+    if (!dec.ans.init(tree, tree_len)) return MARS_FMT_E_INVALID_TREE_PAYLOAD;
+    Predictor pred;
+    const int rc = walk_read(h, 0, 0, h.virtual_size(), dec, pred, leaves);
+    if (rc) leaves.clear();
+    return rc;
+}
+
+// ifs.rs:120-278: the legacy raw `.ifs` bitstream (Mars 1): bits leave each
+// byte most significant first, values are assembled least significant bit
+// first; header 4+4+7+7+6+12+12+8 bits; the tree NW, SW, NE, SE; 0-7 zero
+// padding bits.
+struct BitReader {
+    const uint8_t *data;
+    size_t len, pos = 0;
+    bool ok = true;
+    uint32_t read(uint32_t n)
+    {
+        uint32_t v = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            if ((pos >> 3) >= len) {
+                ok = false;
+                return 0;
+            }
+            const uint32_t b = (data[pos >> 3] >> (7 - (pos & 7))) & 1u;
+            pos++;
+            v |= b << i;
+        }
+        return v;
+    }
+};
+
+int ifs_walk(BitReader &r, const Header &h, uint32_t row, uint32_t col, uint32_t size,
+             std::vector<Leaf> &leaves)
+{
+    if (row >= h.height || col >= h.width) return MARS_HIP_OK;
+    const bool forced = size > h.max_size || row + size > h.height || col + size > h.width;
+    bool split = forced;
+    if (!forced && size > h.min_size) {
+        split = r.read(1) == 1;
+        if (!r.ok) return MARS_FMT_E_TRUNCATED;
+    }
+    if (split) {
+        const uint32_t half = size / 2;
+        int rc = ifs_walk(r, h, row, col, half, leaves);
+        if (!rc) rc = ifs_walk(r, h, row + half, col, half, leaves);
+        if (!rc) rc = ifs_walk(r, h, row, col + half, half, leaves);
+        if (!rc) rc = ifs_walk(r, h, row + half, col + half, half, leaves);
+        return rc;
+    }
+    Leaf l{};
+    l.row = row;
+    l.col = col;
+    l.size = size;
+    l.qalfa = r.read(h.bits_alfa);
+    l.qbeta = r.read(h.bits_beta);
+    if (l.qalfa != 0) {
+        l.isometry = (uint8_t)r.read(3);
+        const uint64_t dr = (uint64_t)h.shift * r.read(h.bits_coord_row());
+        const uint64_t dc = (uint64_t)h.shift * r.read(h.bits_coord_col());
+        if (!r.ok) return MARS_FMT_E_TRUNCATED;
+        if (dr + 2ull * size > h.height || dc + 2ull * size > h.width)
+            return MARS_FMT_E_DOMAIN_OUT_OF_BOUNDS;
+        l.dom_row = (uint32_t)dr;
+        l.dom_col = (uint32_t)dc;
+    }
+    if (!r.ok) return MARS_FMT_E_TRUNCATED;
+    l.mode = l.qalfa == 0 ? 0 : 2;
+    leaves.push_back(std::move(l));
+    return MARS_HIP_OK;
+}
+
+int ifs_parse(const uint8_t *data, size_t len, Header &h, std::vector<Leaf> &leaves)
+{
+    leaves.clear();
+    BitReader r{data, len};
+    h = Header{};
+    h.bits_alfa = r.read(4);
+    h.bits_beta = r.read(4);
+    h.min_size = r.read(7);
+    h.max_size = r.read(7);
+    h.shift = r.read(6);
+    h.width = r.read(12);
+    h.height = r.read(12);
+    h.int_max_alfa = r.read(8);
+    h.qstep_bits = qstep_bits(8.0); // ResidualQstep::LEGACY
+    if (!r.ok) return MARS_FMT_E_TRUNCATED;
+    if (h.width == 0 || h.height == 0 || h.shift == 0) return MARS_FMT_E_DEGENERATE_HEADER;
+    // (min_size 0 or max_size < min_size would split forever: the reference
+    // recurses until the bitstream runs out; here it is a degenerate header.)
+    if (h.min_size == 0 || h.max_size < h.min_size) return MARS_FMT_E_DEGENERATE_HEADER;
+    int rc = ifs_walk(r, h, 0, 0, h.virtual_size(), leaves);
+    if (rc) {
+        leaves.clear();
+        return rc;
+    }
+    const size_t left = len * 8 - r.pos;
+    if (left >= 8) rc = MARS_FMT_E_TRAILING_BITS;
+    else if (left > 0 && (data[len - 1] & ((1u << left) - 1)) != 0) rc = MARS_FMT_E_NONZERO_PADDING;
+    if (rc) leaves.clear();
+    return rc;
+}
+
+// ifs.rs:406-484 decode_leaf: one leaf of the next iteration from the previous
+// one (double-buffered, so leaves are independent).
+void decode_leaf(const Header &h, const Leaf &leaf, const uint8_t *img, size_t stride,
+                 uint8_t *next)
+{
+    const size_t size = leaf.size;
+    auto to_u8 = [](double v) -> uint8_t {
+        if (!(v == v)) return 0; // Rust's `as u8` of NaN
+        v = std::min(std::max(v, 0.0), 255.0);
+        return (uint8_t)v;
+    };
+    if (leaf.mode == 1) {
+        const double b0 = (double)leaf.qbeta / (double)((1u << h.bits_beta) - 1) * 255.0;
+        const double gx = (double)leaf.qgx / AFFINE_GRAD_SCALE;
+        const double gy = (double)leaf.qgy / AFFINE_GRAD_SCALE;
+        for (size_t u = 0; u < size; u++)
+            for (size_t v = 0; v < size; v++)
+                next[(leaf.row + u) * stride + leaf.col + v] =
+                    to_u8(std::round(b0 + gx * (double)u + gy * (double)v));
+        return;
+    }
+    const double max_alfa = (double)h.int_max_alfa / 256.0 * 8.0;
+    const double alfa = (double)leaf.qalfa / (double)(1u << h.bits_alfa) * max_alfa;
+    double beta = (double)leaf.qbeta / (double)((1u << h.bits_beta) - 1) * ((1.0 + std::fabs(alfa)) * 255.0);
+    if (alfa > 0.0) beta -= alfa * 255.0;
+    std::vector<double> res;
+    if (leaf.mode == 3 && !leaf.residual.empty()) {
+        std::vector<double> levels(leaf.residual.size());
+        for (size_t k = 0; k < levels.size(); k++)
+            levels[k] = dead_zone_dequantize(leaf.residual[k], h.qstep(), 0.5);
+        res = inverse_dct2d(levels, size);
+    }
+    for (size_t u = 0; u < size; u++)
+        for (size_t v = 0; v < size; v++) {
+            double d = 0.0;
+            if (leaf.qalfa != 0) {
+                const size_t dr = leaf.dom_row + 2 * u, dc = leaf.dom_col + 2 * v;
+                d = ((double)img[dr * stride + dc] + (double)img[(dr + 1) * stride + dc] +
+                     (double)img[dr * stride + dc + 1] + (double)img[(dr + 1) * stride + dc + 1]) /
+                    4.0;
+            }
+            uint32_t i, j;
+            iso_map(leaf.isometry, (uint32_t)u, (uint32_t)v, (uint32_t)size, &i, &j);
+            double value = (0.5 + d * alfa) + beta;
+            if (!res.empty()) value += res[u * size + v];
+            next[(leaf.row + i) * stride + leaf.col + j] = to_u8(value);
+        }
 }
 
 // ---------------------------------------------------------------- rate (rate.rs:60-85)
@@ -1053,5 +1430,201 @@ int mars_downsample_box(const uint8_t *px, uint32_t w, uint32_t h, uint8_t *out)
 }
 
 void mars_free(void *p) { free(p); }
+
+} // extern "C"
+
+namespace {
+
+void to_c_header(const Header &h, mars_header *o)
+{
+    *o = mars_header{};
+    o->min_size = h.min_size;
+    o->max_size = h.max_size;
+    o->shift = h.shift;
+    o->bits_alfa = h.bits_alfa;
+    o->bits_beta = h.bits_beta;
+    o->int_max_alfa = h.int_max_alfa;
+    o->width = h.width;
+    o->height = h.height;
+    o->residual_qstep_bits = h.qstep_bits;
+}
+
+Header from_c_header(const mars_header &c)
+{
+    Header h{};
+    h.min_size = c.min_size;
+    h.max_size = c.max_size;
+    h.shift = c.shift;
+    h.bits_alfa = c.bits_alfa;
+    h.bits_beta = c.bits_beta;
+    h.int_max_alfa = c.int_max_alfa;
+    h.width = c.width;
+    h.height = c.height;
+    h.qstep_bits = c.residual_qstep_bits;
+    return h;
+}
+
+// Leaves (and their residual levels) into malloc'd C arrays.
+int export_leaves(const std::vector<Leaf> &leaves, mars_leaf **out, size_t *n_out,
+                  int32_t **residual, size_t *n_residual)
+{
+    *out = nullptr;
+    *n_out = 0;
+    if (residual) *residual = nullptr;
+    if (n_residual) *n_residual = 0;
+    size_t nres = 0;
+    for (const Leaf &l : leaves) nres += l.residual.size();
+    mars_leaf *L = leaves.empty() ? nullptr : (mars_leaf *)malloc(sizeof(mars_leaf) * leaves.size());
+    int32_t *R = nres && residual ? (int32_t *)malloc(sizeof(int32_t) * nres) : nullptr;
+    if ((!leaves.empty() && !L) || (nres && residual && !R)) {
+        free(L);
+        free(R);
+        return MARS_HIP_E_ARG;
+    }
+    size_t off = 0;
+    for (size_t i = 0; i < leaves.size(); i++) {
+        const Leaf &l = leaves[i];
+        mars_leaf &c = L[i];
+        c = mars_leaf{};
+        c.row = l.row;
+        c.col = l.col;
+        c.size = l.size;
+        c.mode = l.mode;
+        c.qalfa = l.qalfa;
+        c.qbeta = l.qbeta;
+        c.isometry = l.isometry;
+        c.dom_row = l.dom_row;
+        c.dom_col = l.dom_col;
+        c.qgx = l.qgx;
+        c.qgy = l.qgy;
+        c.residual_off = (uint32_t)off;
+        c.residual_len = (uint32_t)l.residual.size();
+        if (R && !l.residual.empty()) memcpy(R + off, l.residual.data(), 4 * l.residual.size());
+        off += l.residual.size();
+    }
+    *out = L;
+    *n_out = leaves.size();
+    if (residual) *residual = R;
+    if (n_residual) *n_residual = R ? nres : 0;
+    return MARS_HIP_OK;
+}
+
+// C leaves back to Leaf, checked against the plane (decode_leaf indexes the
+// previous image through every field checked here).
+int import_leaves(const Header &h, const mars_leaf *in, size_t n, const int32_t *residual,
+                  size_t n_residual, bool check_plane, std::vector<Leaf> &out)
+{
+    out.clear();
+    out.reserve(n);
+    for (size_t i = 0; i < n; i++) {
+        const mars_leaf &c = in[i];
+        Leaf l{};
+        l.row = c.row;
+        l.col = c.col;
+        l.size = c.size;
+        l.mode = (uint8_t)c.mode;
+        l.qalfa = c.qalfa;
+        l.qbeta = c.qbeta;
+        l.isometry = (uint8_t)c.isometry;
+        l.dom_row = c.dom_row;
+        l.dom_col = c.dom_col;
+        l.qgx = c.qgx;
+        l.qgy = c.qgy;
+        if (c.mode > 3 || c.isometry > 7 || c.size == 0) return MARS_HIP_E_ARG;
+        if (c.residual_len) {
+            if ((uint64_t)c.residual_off + c.residual_len > n_residual || !residual) return MARS_HIP_E_ARG;
+            l.residual.assign(residual + c.residual_off, residual + c.residual_off + c.residual_len);
+        }
+        if (check_plane) {
+            if ((uint64_t)c.row + c.size > h.height || (uint64_t)c.col + c.size > h.width)
+                return MARS_HIP_E_ARG;
+            if (c.qalfa != 0 && c.mode != 1 &&
+                ((uint64_t)c.dom_row + 2ull * c.size > h.height ||
+                 (uint64_t)c.dom_col + 2ull * c.size > h.width))
+                return MARS_HIP_E_ARG;
+            if (c.mode == 3 && c.residual_len != 0 && (uint64_t)c.residual_len != (uint64_t)c.size * c.size)
+                return MARS_HIP_E_ARG;
+        }
+        out.push_back(std::move(l));
+    }
+    return MARS_HIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int mars_read(const uint8_t *data, size_t len, mars_header *hdr, mars_leaf **leaves,
+              size_t *n_leaves, int32_t **residual, size_t *n_residual)
+{
+    if ((!data && len) || !hdr || !leaves || !n_leaves) return MARS_HIP_E_ARG;
+    *leaves = nullptr;
+    *n_leaves = 0;
+    if (residual) *residual = nullptr;
+    if (n_residual) *n_residual = 0;
+    Header h{};
+    std::vector<Leaf> ls;
+    const int rc = format_read(data, len, h, ls);
+    if (rc) return rc;
+    to_c_header(h, hdr);
+    return export_leaves(ls, leaves, n_leaves, residual, n_residual);
+}
+
+int mars_write(const mars_header *hdr, const mars_leaf *leaves, size_t n_leaves,
+               const int32_t *residual, size_t n_residual, uint8_t **out, size_t *out_len)
+{
+    if (!hdr || (!leaves && n_leaves) || !out || !out_len) return MARS_HIP_E_ARG;
+    *out = nullptr;
+    *out_len = 0;
+    const Header h = from_c_header(*hdr);
+    std::vector<Leaf> ls;
+    int rc = import_leaves(h, leaves, n_leaves, residual, n_residual, false, ls);
+    if (rc) return rc;
+    std::vector<uint8_t> bytes;
+    rc = format_write(h, ls, bytes);
+    if (rc) return rc;
+    uint8_t *buf = (uint8_t *)malloc(std::max<size_t>(bytes.size(), 1));
+    if (!buf) return MARS_HIP_E_ARG;
+    memcpy(buf, bytes.data(), bytes.size());
+    *out = buf;
+    *out_len = bytes.size();
+    return MARS_HIP_OK;
+}
+
+int mars_ifs_parse(const uint8_t *data, size_t len, mars_header *hdr, mars_leaf **leaves,
+                   size_t *n_leaves)
+{
+    if ((!data && len) || !hdr || !leaves || !n_leaves) return MARS_HIP_E_ARG;
+    *leaves = nullptr;
+    *n_leaves = 0;
+    Header h{};
+    std::vector<Leaf> ls;
+    const int rc = ifs_parse(data, len, h, ls);
+    if (rc) return rc;
+    to_c_header(h, hdr);
+    return export_leaves(ls, leaves, n_leaves, nullptr, nullptr);
+}
+
+int mars_decode_iterative(const mars_header *hdr, const mars_leaf *leaves, size_t n_leaves,
+                          const int32_t *residual, size_t n_residual, uint32_t iterations,
+                          uint8_t *out)
+{
+    if (!hdr || (!leaves && n_leaves) || !out) return MARS_HIP_E_ARG;
+    const Header h = from_c_header(*hdr);
+    if (h.width == 0 || h.height == 0 || h.bits_alfa > 31 || h.bits_beta > 31)
+        return MARS_HIP_E_ARG;
+    std::vector<Leaf> ls;
+    const int rc = import_leaves(h, leaves, n_leaves, residual, n_residual, true, ls);
+    if (rc) return rc;
+    const size_t n = (size_t)h.width * h.height;
+    std::vector<uint8_t> img(n, 128), next(n);
+    for (uint32_t it = 0; it < iterations; it++) {
+        std::fill(next.begin(), next.end(), 0);
+        for (const Leaf &l : ls) decode_leaf(h, l, img.data(), h.width, next.data());
+        img.swap(next);
+    }
+    memcpy(out, img.data(), n);
+    return MARS_HIP_OK;
+}
 
 } // extern "C"

@@ -12,7 +12,7 @@ import hashlib
 import pytest
 
 from mars_amd import synth
-from mars_amd.codec import EncodeOptions, encode_plane, hip_encode_plane
+from mars_amd.codec import EncodeOptions, encode_plane, hip_encode_plane, read_mars, write_mars
 from mars_amd.provider import EncodeParams, HipContext
 from tests._providers import TableProvider, assert_tables_equal
 
@@ -32,6 +32,9 @@ def _both(img, params, options, shifts, check_tables=True):
         n = assert_tables_equal(ctx.tables(), prov, f"{img.shape[1]}x{img.shape[0]}")
         print(f"  {n} table winners equal the oracle's", flush=True)
     ctx.close()
+    # the stream the GPU path wrote reads back and re-writes to itself
+    hdr, leaves = read_mars(got)
+    assert write_mars(hdr, leaves) == got
     return ref, ref_st, got, st
 
 
