@@ -88,6 +88,10 @@ def test_ifs_rejects_damaged_streams():
     with pytest.raises(FormatError) as e:
         codec.ifs_parse(data + b"\0")
     assert e.value.code == codec.FMT_E_TRAILING_BITS
+    # this stream ends with padding bits (ifs.rs:205-214): they must be zero
+    with pytest.raises(FormatError) as e:
+        codec.ifs_parse(data[:-1] + bytes([data[-1] | 1]))
+    assert e.value.code == codec.FMT_E_NONZERO_PADDING
     # width 0: the 12-bit field after 4+4+7+7+6 = 28 header bits
     bad = bytearray(data)
     for bit in range(28, 40):
