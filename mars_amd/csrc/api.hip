@@ -115,6 +115,16 @@ constexpr uint32_t kPrepassMinSize = 16;
 // other sizes' searches (step 0.675 -> 0.661 ms).
 uint32_t prepass_stride(uint32_t s) { return s; }
 
+// Sizes whose search correlates in binary16 (the kernel's Geo<S>::F16).
+bool uses_f16(uint32_t s) { return s == 4 || s == 8; }
+// The binary16 plane: copies 0..7 of the split planes' layout (all that
+// sizes 4 and 8 address), 2-byte elements.
+size_t f16_plane_bytes(size_t copy_stride) { return 8 * copy_stride * sizeof(uint16_t); }
+// Window offsets are 32-bit: byte offsets into the 16 split-plane copies and
+// element offsets into the binary16 plane (byte offset twice that, below
+// 16 copy strides).
+bool copies_addressable(size_t copy_stride) { return 16 * copy_stride < ((size_t)1 << 32); }
+
 struct Plane {
     uint32_t w = 0, h = 0, wc = 0, hc = 0, pitch = 0;
     size_t copy_stride = 0;
@@ -123,6 +133,7 @@ struct Plane {
     int32_t *d_d = nullptr;
     float *d_df = nullptr;
     int8_t *d_lo = nullptr, *d_hi = nullptr;
+    uint16_t *d_wf = nullptr; // binary16 plane, when a table has size 4 or 8
     int64_t *d_sat1 = nullptr, *d_sat2 = nullptr;
 };
 
@@ -135,6 +146,7 @@ struct Table {
     uint32_t nchunks[2] = {1, 1}, tpc[2] = {1, 1}; // current chunking
     uint32_t *d_rows = nullptr, *d_cols = nullptr;
     int8_t *d_rng8 = nullptr;
+    uint16_t *d_rngh = nullptr; // binary16 range operand (sizes 4 and 8)
     float *d_rngf = nullptr;
     int32_t *d_rstat = nullptr;
     int32_t *d_rpos = nullptr;
@@ -434,8 +446,8 @@ int layout(mars_hip_ctx *ctx)
     std::vector<PlaneDesc> pd(ctx->planes.size());
     for (size_t i = 0; i < pd.size(); i++) {
         const Plane &p = ctx->planes[i];
-        pd[i] = PlaneDesc{p.d_px, p.d_d, p.d_df, p.d_lo, p.d_hi, p.d_sat1, p.d_sat2,
-                          p.w,    p.h,   p.wc,   p.hc,   p.pitch, (uint32_t)p.copy_stride};
+        pd[i] = PlaneDesc{p.d_px, p.d_d, p.d_df,  p.d_lo,  p.d_hi,  p.d_wf, p.d_sat1,
+                          p.d_sat2, p.w, p.h, p.wc, p.hc, p.pitch, (uint32_t)p.copy_stride};
     }
     std::vector<TableDesc> td(ctx->tables.size());
     for (size_t i = 0; i < td.size(); i++) {
@@ -447,6 +459,7 @@ int layout(mars_hip_ctx *ctx)
         d.df = p.d_df;
         d.lo = p.d_lo;
         d.hi = p.d_hi;
+        d.wf = p.d_wf;
         d.sat1 = p.d_sat1;
         d.sat2 = p.d_sat2;
         d.w = p.w;
@@ -456,6 +469,7 @@ int layout(mars_hip_ctx *ctx)
         d.rows = t.d_rows;
         d.cols = t.d_cols;
         d.rng8 = t.d_rng8;
+        d.rngh = t.d_rngh;
         d.rngf = t.d_rngf;
         d.rstat = t.d_rstat;
         d.rpos = t.d_rpos;
@@ -949,6 +963,8 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         pl.hc = pl.h / 2;
         pl.pitch = ((pl.wc + 15) / 16) * 16 + 16;
         pl.copy_stride = (size_t)pl.pitch * std::max<uint32_t>(pl.hc, 1);
+        if (!copies_addressable(pl.copy_stride))
+            return fail(ctx, MARS_HIP_E_UNSUPPORTED, "plane too large for 32-bit window offsets");
         pl.host.assign(in[pi].px, in[pi].px + (size_t)pl.w * pl.h);
         NodeMap nodes;
         collect(0, 0, 1u << ceil_log2(std::max(pl.w, pl.h)), pl.w, pl.h, *p, nodes);
@@ -1053,13 +1069,19 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
     ArenaPlan plan;
     plan.add(&ctx->d_planes, sizeof(PlaneDesc) * std::max<size_t>(ctx->planes.size(), 1));
     plan.add(&ctx->d_tabs, sizeof(TableDesc) * std::max<size_t>(ctx->tables.size(), 1));
-    for (Plane &pl : ctx->planes) {
+    std::vector<char> plane_f16(ctx->planes.size(), 0);
+    for (const Table &t : ctx->tables)
+        if (uses_f16(t.size)) plane_f16[t.plane] = 1;
+    for (size_t pi = 0; pi < ctx->planes.size(); pi++) {
+        Plane &pl = ctx->planes[pi];
         const size_t nc = (size_t)pl.wc * pl.hc, nsat = (size_t)(pl.wc + 1) * (pl.hc + 1);
         plan.add(&pl.d_px, (size_t)pl.w * pl.h);
         plan.add(&pl.d_d, nc * sizeof(int32_t));
         plan.add(&pl.d_df, nc * sizeof(float));
         plan.add(&pl.d_lo, 16 * pl.copy_stride);
         plan.add(&pl.d_hi, 16 * pl.copy_stride);
+        pl.d_wf = nullptr;
+        if (plane_f16[pi]) plan.add(&pl.d_wf, f16_plane_bytes(pl.copy_stride));
         plan.add(&pl.d_sat1, nsat * sizeof(int64_t));
         plan.add(&pl.d_sat2, nsat * sizeof(int64_t));
     }
@@ -1068,6 +1090,8 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         plan.add(&t.d_rows, 4 * (size_t)t.nr);
         plan.add(&t.d_cols, 4 * (size_t)t.nr);
         plan.add(&t.d_rng8, (size_t)t.nr_pad * 8 * t.kmem);
+        t.d_rngh = nullptr;
+        if (uses_f16(t.size)) plan.add(&t.d_rngh, sizeof(uint16_t) * t.nr_pad * 8 * t.kmem);
         plan.add(&t.d_rngf, (size_t)t.nr_pad / 4 * K * 32 * sizeof(float));
         plan.add(&t.d_rstat, sizeof(int32_t) * 2 * t.nr_pad);
         plan.add(&t.d_rpos, sizeof(int32_t) * t.nr_pad);
@@ -1582,6 +1606,8 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
     const uint32_t P = (uint32_t)P64, P_pad = (P + 31) / 32 * 32;
     const uint32_t wc = w / 2, hc = h / 2, pitch = ((wc + 15) / 16) * 16 + 16;
     const size_t copy_stride = (size_t)pitch * std::max<uint32_t>(hc, 1);
+    if (!copies_addressable(copy_stride))
+        return fail(ctx, MARS_HIP_E_UNSUPPORTED, "plane too large for 32-bit window offsets");
     const size_t nc = (size_t)wc * hc, nsat = (size_t)(wc + 1) * (hc + 1);
     // the search order of search_order()
     const uint32_t wb = row_load_bytes(s);
@@ -1627,6 +1653,7 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
     pd.df = (float *)take(nc * sizeof(float));
     pd.lo = (int8_t *)take(16 * copy_stride);
     pd.hi = (int8_t *)take(16 * copy_stride);
+    pd.wf = uses_f16(s) ? (uint16_t *)take(f16_plane_bytes(copy_stride)) : nullptr;
     pd.sat1 = (int64_t *)take(nsat * sizeof(int64_t));
     pd.sat2 = (int64_t *)take(nsat * sizeof(int64_t));
     pd.w = w;
@@ -1641,6 +1668,7 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
     td.df = pd.df;
     td.lo = pd.lo;
     td.hi = pd.hi;
+    td.wf = pd.wf;
     td.sat1 = pd.sat1;
     td.sat2 = pd.sat2;
     td.w = w;
@@ -1650,6 +1678,7 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
     td.rows = g.rows;
     td.cols = g.cols;
     td.rng8 = (int8_t *)take((size_t)nr_pad * 8 * kmem);
+    td.rngh = uses_f16(s) ? (uint16_t *)take(sizeof(uint16_t) * (size_t)nr_pad * 8 * kmem) : nullptr;
     td.rngf = nullptr; // the FP32 baseline's operand: not used here
     td.rstat = (int32_t *)take(sizeof(int32_t) * 2 * nr_pad);
     td.rpos = (int32_t *)take(sizeof(int32_t) * nr_pad);

@@ -21,6 +21,13 @@ namespace mars {
 
 template <class X> __device__ __forceinline__ GLB X *glb(X *p) { return (GLB X *)p; }
 
+// binary16 bits of an integer of magnitude <= 2048 (exact: an 11-bit
+// significand holds it).
+__device__ __forceinline__ uint16_t f16_bits(int32_t v)
+{
+    return __builtin_bit_cast(uint16_t, (_Float16)(float)v);
+}
+
 // Class types (uint4, Partial, mars_hip_candidate) cannot be copied to or
 // from address space 1 in C++, so the views hold them as plain vectors and
 // words and these helpers move them.
@@ -89,14 +96,16 @@ struct PlaneG {
     GLB int32_t *d;
     GLB float *df;
     GLB int8_t *lo, *hi;
+    GLB uint16_t *wf;
     GLB int64_t *sat1, *sat2;
     uint32_t w, h, wc, hc, pitch, copy_stride;
 };
 
 __device__ __forceinline__ PlaneG view(const PlaneDesc &p)
 {
-    return PlaneG{glb(p.px), glb(p.d),  glb(p.df), glb(p.lo), glb(p.hi),    glb(p.sat1), glb(p.sat2),
-                  p.w,       p.h,       p.wc,      p.hc,      p.pitch,      p.copy_stride};
+    return PlaneG{glb(p.px),  glb(p.d),    glb(p.df), glb(p.lo), glb(p.hi), glb(p.wf),
+                  glb(p.sat1), glb(p.sat2), p.w,       p.h,       p.wc,      p.hc,
+                  p.pitch,     p.copy_stride};
 }
 
 struct TableG {
@@ -104,10 +113,12 @@ struct TableG {
     const GLB int32_t *d;
     const GLB float *df;
     const GLB int8_t *lo, *hi;
+    const GLB uint16_t *wf;
     const GLB int64_t *sat1, *sat2;
     uint32_t w, wc, pitch, copy_stride;
     const GLB uint32_t *rows, *cols;
     GLB int8_t *rng8;
+    GLB uint16_t *rngh;
     GLB float *rngf;
     GLB int32_t *rstat;
     GLB int32_t *rpos;
@@ -142,6 +153,7 @@ __device__ __forceinline__ TableG view(const TableDesc &t)
     g.df = glb(t.df);
     g.lo = glb(t.lo);
     g.hi = glb(t.hi);
+    g.wf = glb(t.wf);
     g.sat1 = glb(t.sat1);
     g.sat2 = glb(t.sat2);
     g.w = t.w;
@@ -151,6 +163,7 @@ __device__ __forceinline__ TableG view(const TableDesc &t)
     g.rows = glb(t.rows);
     g.cols = glb(t.cols);
     g.rng8 = glb(t.rng8);
+    g.rngh = glb(t.rngh);
     g.rngf = glb(t.rngf);
     g.rstat = glb(t.rstat);
     g.rpos = glb(t.rpos);

@@ -32,6 +32,10 @@ struct PlaneDesc {
     int32_t *d;          // contracted plane (i32)
     float *df;           // contracted plane (f32, FP32 baseline)
     int8_t *lo, *hi;     // split planes D % 4, D / 4 - 128; 16 byte-shifted copies each
+    // D as binary16 (exact: D <= 1020) in the same 16-copy layout, 2-byte
+    // elements (element offset = the split planes' byte offset); null when no
+    // table of the plane has size 4 or 8
+    uint16_t *wf;
     int64_t *sat1, *sat2;
     uint32_t w, h, wc, hc, pitch, copy_stride;
 };
@@ -43,11 +47,13 @@ struct TableDesc {
     const int32_t *d;
     const float *df;
     const int8_t *lo, *hi;
+    const uint16_t *wf;  // binary16 plane (sizes 4 and 8), as PlaneDesc::wf
     const int64_t *sat1, *sat2;
     uint32_t w, wc, pitch, copy_stride;
     // the table
     const uint32_t *rows, *cols;
     int8_t *rng8;        // [nr_pad][8][kmem] r - 128
+    uint16_t *rngh;      // [nr_pad][8][kmem] r - 128 as binary16 (sizes 4 and 8, else null)
     float *rngf;         // [nr_pad/4][K][32] r (FP32 baseline)
     int32_t *rstat;      // [nr_pad][2] t0, t2
     int32_t *rpos;       // [nr_pad] 3 * sum max(r - 128, 0)

@@ -36,6 +36,9 @@ namespace mars {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------- pool
 
@@ -58,12 +61,15 @@ __global__ void k_contract(const PlaneDesc *__restrict__ planes, const Work2 *__
     pl.df[idx] = (float)v;
     const int8_t l = (int8_t)(v & 3);           // Dl = D % 4
     const int8_t hh = (int8_t)((v >> 2) - 128); // Dh' = D / 4 - 128
+    const uint16_t vf = f16_bits(v);
+    GLB uint16_t *wf = pl.wf;
 #pragma unroll
     for (uint32_t c = 0; c < 16; c++) {
         if (j >= c) {
             size_t o = c * (size_t)pl.copy_stride + (size_t)i * pl.pitch + (j - c);
             pl.lo[o] = l;
             pl.hi[o] = hh;
+            if (wf && c < 8) wf[o] = vf; // sizes 4 and 8 read copies 0..7 only
         }
     }
 }
@@ -253,6 +259,7 @@ __global__ __launch_bounds__(256) void k_ranges(const TableDesc *__restrict__ ta
     const uint8_t *src = s_px + (r - rb) * K;
     GLB float *rngf = fp32 ? T.rngf : nullptr;
     uint32_t packed = 0;
+    uint32_t hpk[2] = {0, 0}; // the same four values as binary16
 #pragma unroll
     for (uint32_t b = 0; b < 4; b++) {
         const uint32_t k = k0 + b;
@@ -262,12 +269,14 @@ __global__ __launch_bounds__(256) void k_ranges(const TableDesc *__restrict__ ta
             iso_map(iso, u, v, size, &i, &j);
             const uint32_t pix = src[i * size + j];
             packed |= ((pix - 128u) & 0xffu) << (8 * b);
+            hpk[b >> 1] |= (uint32_t)f16_bits((int32_t)pix) << (16 * (b & 1));
             if (rngf) rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = (float)pix;
         } else if (rngf && k < K) {
             rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = 0.0f;
         }
     }
     *(uint32_t *)(T.rng8 + e) = packed;
+    if (T.rngh) *(GLB u32x2 *)(T.rngh + e) = (u32x2){hpk[0], hpk[1]};
 }
 
 // Range sums t0 = sum r, t2 = sum r^2 and 3 * sum max(r - 128, 0), work
@@ -313,7 +322,48 @@ template <int S> struct Geo {
     static constexpr int KMEM = K < 16 ? 16 : K;           // padded range row length
     static constexpr int NK = (KMEM + KSTEP - 1) / KSTEP;  // K steps
     static constexpr bool PRELOAD_A = NK <= 2;             // keep A in registers
+    // Sizes 4 and 8 correlate in binary16 (v_mfma_f32_32x32x16_f16): r in
+    // [0, 255] and D in [0, 1020] are exact, and t1 = sum r*D < 2^24 even at
+    // s = 8 (64 * 255 * 1020 = 16 646 400), so the binary32 accumulation of
+    // these non-negative integers is exact in any order.  One instruction per
+    // K-step gives t1 itself: no hi/lo planes to combine.
+    static constexpr bool F16 = S == 4 || S == 8;
+    static constexpr int NKH = F16 ? K / 16 : 1;           // binary16 K-steps
 };
+
+// Binary16 B fragment of lane half h for K-step kk (16 values of k) of the
+// window at element offset `boff` of the binary16 plane (the split planes'
+// byte offset: the same copy layout with 2-byte elements).  k = row * S +
+// column, lane half h holding k = 16 kk + 8 h .. + 7, as the A fragment.
+template <int S>
+__device__ __forceinline__ v8h load_bh(const GLB uint16_t *wf, uint32_t boff, uint32_t pitch, int kk,
+                                       int h)
+{
+    const GLB char *base = (const GLB char *)wf;
+    if constexpr (S == 4) {
+        const uint32_t o = 2u * (boff + 2 * h * pitch); // rows 2h, 2h + 1
+        const u32x2 r0 = *(const GLB u32x2 *)(base + o);
+        const u32x2 r1 = *(const GLB u32x2 *)(base + (o + 2 * pitch));
+        const u32x4 t = {r0.x, r0.y, r1.x, r1.y};
+        return __builtin_bit_cast(v8h, t);
+    } else {
+        const uint32_t o = 2u * (boff + (2 * kk + h) * pitch); // row 2 kk + h
+        return __builtin_bit_cast(v8h, *(const GLB u32x4 *)(base + o));
+    }
+}
+
+// Binary16 A fragment: row (range, isometry) of lane & 31 as load_a, k = 16
+// kk + 8 h .. + 7.
+template <int S>
+__device__ __forceinline__ v8h load_ah(const GLB uint16_t *rngh, uint32_t g, int tt, int kk, int lane)
+{
+    constexpr int KMEM = Geo<S>::KMEM;
+    const int m = lane & 31, h = lane >> 5;
+    const int rl = ((m >> 2) & 1) * 4 + (m & 3);
+    const int iso = 4 * tt + (m >> 3);
+    const GLB uint16_t *p = rngh + ((size_t)(g * 8 + rl) * 8 + iso) * KMEM + kk * 16 + h * 8;
+    return __builtin_bit_cast(v8h, *(const GLB u32x4 *)p);
+}
 
 // B fragment (one split plane) of lane half h for K-step kk of the window
 // whose origin is byte `boff` of the plane's copy layout (the copy in which
@@ -341,7 +391,6 @@ __device__ __forceinline__ void load_b(const GLB int8_t *plane, uint32_t boff, u
         const uint32_t r1 = *(const GLB uint32_t *)(plane + (o + pitch));
         *b8 = (long)((uint64_t)r0 | ((uint64_t)r1 << 32));
     } else if constexpr (S == 8) {
-        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
         const uint32_t o = boff + (4 * kk + 2 * h) * pitch;
         const u32x2 r0 = *(const GLB u32x2 *)(plane + o);
         const u32x2 r1 = *(const GLB u32x2 *)(plane + (o + pitch));
@@ -664,6 +713,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     constexpr bool RANKED = KCAP > 0;
     using G = Geo<S>;
     constexpr int NK = G::NK;
+    constexpr bool F16 = G::F16;
+    constexpr int NKH = G::NKH;
     constexpr int64_t n = (int64_t)S * S;
     constexpr int LOG2S = S == 2 ? 1 : S == 4 ? 2 : S == 8 ? 3 : S == 16 ? 4 : 5;
     constexpr int LOG2N = 2 * LOG2S;
@@ -734,7 +785,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // +2^27 while a range cannot prune puts its limit below every v; -2^28
     // for a padding range and, through S1 = -2^21 in the padding position
     // record, -2^28 for a padding position put the limit above every v; no
-    // combination leaves int32.
+    // combination leaves int32.  Binary16 sizes compare t1 itself (0 <= t1
+    // < 2^24, no offset): the same +2^27 / -2^28 range terms, and +2^28 added
+    // for a padding position.
     // noff = -1 - (that offset), so a limit is one three-term add.
     int32_t noff[4];
 #pragma unroll
@@ -748,7 +801,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         Alo[i] = (float)A[i] * (1.0f - 3e-7f); // <= A after both roundings
         rmin[i] = rv ? (int32_t)0x80000000 : 0x7fffffff;
         // no pruning until the first threshold
-        noff[i] = -1 - ((int32_t)(512 * (t0 - 128 * n)) + (rv ? (1 << 27) : -(1 << 28)));
+        noff[i] = -1 - ((F16 ? 0 : (int32_t)(512 * (t0 - 128 * n))) + (rv ? (1 << 27) : -(1 << 28)));
     }
     // t1 = 4*hi + lo + off + 128*S1 with off = 512 * (t0 - 128 n) (< 2^27 in
     // magnitude), recomputed from t0 where needed rather than kept live.
@@ -768,7 +821,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
 
     long a8[2][NK];
     v4i a16[2][NK];
-    if constexpr (G::PRELOAD_A) {
+    v8h ahf[2][NKH];
+    if constexpr (F16) {
+#pragma unroll
+        for (int tt = 0; tt < 2; tt++)
+#pragma unroll
+            for (int kk = 0; kk < NKH; kk++) ahf[tt][kk] = load_ah<S>(T.rngh, g, tt, kk, lane);
+    } else if constexpr (G::PRELOAD_A) {
 #pragma unroll
         for (int tt = 0; tt < 2; tt++)
 #pragma unroll
@@ -892,6 +951,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         uint4 pin; // {window byte offset, S1, sqrt(Di) bits, position index or ~0}
         long l8[NK], h8[NK];
         v4i l16[NK], h16[NK];
+        v8h wh[NKH]; // binary16 sizes
     };
     // The column (lane % 32) is recomputed at every use: kept live across the
     // tile loop it is the value the register allocator spills, and the
@@ -899,7 +959,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     auto load_pin = [&](uint32_t tile) { return ld_u4(T.posinfo, tile * sample * 32 + lane_col()); };
     auto fetch_b = [&](const uint4 pin, Frag &f) {
         f.pin = pin;
-        if constexpr (G::PRELOAD_A) {
+        if constexpr (F16) {
+#pragma unroll
+            for (int kk = 0; kk < NKH; kk++) f.wh[kk] = load_bh<S>(T.wf, pin.x, T.pitch, kk, h);
+        } else if constexpr (G::PRELOAD_A) {
 #pragma unroll
             for (int kk = 0; kk < NK; kk++) {
                 load_b<S>(T.lo, pin.x, T.pitch, kk, h, &f.l8[kk], &f.l16[kk]);
@@ -981,7 +1044,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                     sa[i] = prune ? __builtin_amdgcn_sqrtf(amt) * (1.0f - 2e-5f)
                                   : (INT_LIMIT ? 1.0f : -1.0f);
                     if constexpr (INT_LIMIT)
-                        noff[i] = -1 - (off_of(i) + (rmin[i] == (int32_t)0x80000000
+                        noff[i] = -1 - ((F16 ? 0 : off_of(i)) + (rmin[i] == (int32_t)0x80000000
                                                          ? (prune ? 0 : (1 << 27))
                                                          : -(1 << 28)));
                 }
@@ -996,6 +1059,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // with both, and the (1 - 2e-5) in sa covers the rounding here.
         int32_t lim[4];
         const int32_t o128 = (int32_t)((uint32_t)S1p << 7);
+        // what the limit adds besides noff: -128 S1 (split planes), or 2^28
+        // at a padding position (binary16)
+        const int32_t padj = F16 ? (p < T.P ? 0 : (1 << 28)) : -o128;
         const float usq = CAP_LIMIT ? c.cap_u * sqDi : 0.0f;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -1009,7 +1075,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                 // S1*t0 + nt < 2^31 for sizes <= 8 (nt <= sqrt(A Di) + 8 A),
                 // S1 and t0 < 2^24.
                 const uint32_t X = __umul24((uint32_t)S1p, (uint32_t)T0[i]) + (uint32_t)nt;
-                lim[i] = (int32_t)(X >> LOG2N) + noff[i] - o128;
+                lim[i] = (int32_t)(X >> LOG2N) + noff[i] + padj;
             } else {
                 const double X = ((double)S1p * (double)T0[i] + (double)nt) * inv_n;
                 const int32_t L = (int32_t)floor(X) - 1 - (off_of(i) + o128);
@@ -1026,55 +1092,60 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         constexpr int GW = DRAIN < 8 ? DRAIN : 8; // values per compare group
         static_assert(16 % DRAIN == 0 && DRAIN % GW == 0, "groups tile a row tile's 16 values");
         static_assert(64 * DRAIN + 64 <= Queue<S>::CAP, "a drain interval fits the queue");
+        // Sizes <= 8: one compare per (range, position).  The limit does not
+        // depend on the isometry, so the row tile's 4 values of a range are
+        // tested by their maximum (a v_max3 and a v_max); only a range some
+        // lane passes is tested value by value.  Against a compare per value
+        // this also drops most of the scalar mask ORs and per-value branches
+        // (s = 4 launch 0.367 -> 0.330 ms at 512 x 512).  v: the 16 values as
+        // int32 -- 4 hi + lo, or for binary16 the bits of t1 (non-negative
+        // floats order as their bit patterns); lv: the limits in the same
+        // form.
+        auto epilogue16 = [&](const int32_t (&v)[16], const int32_t (&lv)[4], const int tt) {
+            unsigned long long mr[4], any = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int32_t mx = max(max(max(v[i], v[i + 4]), v[i + 8]), v[i + 12]);
+                mr[i] = __ballot(mx > lv[i]);
+                any |= mr[i];
+            }
+            if (any == 0) return;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (!mr[i]) continue;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int t = 4 * j + i;
+                    const bool sv = v[t] > lv[i];
+                    const unsigned long long m = __ballot(sv);
+                    if (m) {
+                        const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                        if (sv) {
+                            const int32_t t1 = F16 ? (int32_t)__int_as_float(v[t]) : v[t] + off_of(i) + o128;
+                            q[qn + pre] = make_uint2((uint32_t)t1 | ((uint32_t)(4 * h + i) << 28),
+                                                     p * 8 + 4 * tt + j);
+                        }
+                        qn += (uint32_t)__popcll(m);
+                        qtotal += (uint32_t)__popcll(m);
+                    }
+                }
+            }
+            if (qn >= 64) {
+                uint32_t b0 = 0;
+                for (; qn - b0 >= 64; b0 += 64) process(b0, 64);
+                const uint32_t rem = qn - b0;
+                const uint2 tail = q[b0 + ((uint32_t)lane < rem ? lane : 0)];
+                if ((uint32_t)lane < rem) q[lane] = tail;
+                qn = rem;
+            }
+        };
         auto epilogue = [&](const v16i &ah, const v16i &al, const int tt) {
             if constexpr (DRAIN == 16) {
-                // Sizes <= 8: one compare per (range, position).  The limit
-                // does not depend on the isometry, so the row tile's 4 values
-                // of a range are tested by their maximum (a v_max3 and a
-                // v_max); only a range some lane passes is tested value by
-                // value.  Against a compare per value this also drops most
-                // of the scalar mask ORs and per-value branches (s = 4 launch
-                // 0.367 -> 0.330 ms at 512 x 512).
                 int32_t v[16];
 #pragma unroll
                 for (int t = 0; t < 16; t++) v[t] = 4 * ah[t] + al[t];
-                unsigned long long mr[4], any = 0;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int32_t mx = max(max(max(v[i], v[i + 4]), v[i + 8]), v[i + 12]);
-                    mr[i] = __ballot(mx > lim[i]);
-                    any |= mr[i];
-                }
-                if (any != 0) {
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        if (!mr[i]) continue;
-#pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            const int t = 4 * j + i;
-                            const bool sv = v[t] > lim[i];
-                            const unsigned long long m = __ballot(sv);
-                            if (m) {
-                                const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
-                                    (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                                if (sv)
-                                    q[qn + pre] = make_uint2((uint32_t)(v[t] + off_of(i) + o128) |
-                                                                 ((uint32_t)(4 * h + i) << 28),
-                                                             p * 8 + 4 * tt + j);
-                                qn += (uint32_t)__popcll(m);
-                                qtotal += (uint32_t)__popcll(m);
-                            }
-                        }
-                    }
-                    if (qn >= 64) {
-                        uint32_t b0 = 0;
-                        for (; qn - b0 >= 64; b0 += 64) process(b0, 64);
-                        const uint32_t rem = qn - b0;
-                        const uint2 tail = q[b0 + ((uint32_t)lane < rem ? lane : 0)];
-                        if ((uint32_t)lane < rem) q[lane] = tail;
-                        qn = rem;
-                    }
-                }
+                epilogue16(v, lim, tt);
                 return;
             }
 #pragma unroll
@@ -1123,7 +1194,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
             }
         };
 
-        if constexpr (G::PRELOAD_A) {
+        if constexpr (F16) {
+            // As below, one binary16 instruction per K-step; the limits as
+            // binary32 bits (exact below 2^24; a negative limit stays below
+            // every t1 and one of 2^24 or more above, as integers).
+            int32_t lb[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) lb[i] = __float_as_int((float)lim[i]);
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++) {
+                v16f acc = (v16f){};
+#pragma unroll
+                for (int kk = 0; kk < NKH; kk++)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahf[tt][kk], cur.wh[kk], acc, 0, 0, 0);
+                if (tt == 0) prefetch();
+                int32_t v[16];
+#pragma unroll
+                for (int t = 0; t < 16; t++) v[t] = __float_as_int(acc[t]);
+                epilogue16(v, lb, tt);
+            }
+        } else if constexpr (G::PRELOAD_A) {
             // Row tile by row tile: row tile 1's MFMAs run while row tile
             // 0's epilogue uses the VALU, and only one tile's accumulators
             // are live at a time.

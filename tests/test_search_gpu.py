@@ -136,6 +136,23 @@ def test_parity_flat_plane_all_ties():
     _check_tables(img, ctx, "flat")
 
 
+@pytest.mark.parametrize("kind", ["white", "near_white"])
+def test_parity_saturated_planes(kind):
+    """Sizes 4 and 8 correlate in binary16 with binary32 accumulation, exact
+    while t1 = sum r*D < 2^24.  A white plane puts every t1 at its largest
+    value (s = 8: 64 * 255 * 1020 = 16 646 400, 2^24 - 130 816); a plane of
+    random values in 240..255 keeps every t1 within 12% of it with the low
+    bits varying.  Any rounding in the accumulation shows in t1_x4, the rms
+    bits or the winner."""
+    if kind == "white":
+        img = np.full((48, 48), 255, dtype=np.uint8)
+    else:
+        img = np.random.default_rng(31).integers(240, 256, size=(72, 64), dtype=np.uint8)
+    ctx = HipContext(0)
+    ctx.prepare(img, EncodeParams(min_size=4, max_size=8, shift=2), [2])
+    assert _check_tables(img, ctx, kind) > 0
+
+
 def test_parity_fixed_grid_256():
     """256x256 seed-1 plane, fixed 8x8 ranges, stride 8 (the CPU-runnable workload)."""
     img = synth.plane(256, 256, synth.WORKLOAD_SEEDS["fixed256"])
