@@ -107,8 +107,12 @@ uint32_t walk_chunks(uint32_t ntiles) { return std::max<uint32_t>(1, std::min(nt
 // group starts cold once, not once per position chunk; the launch is small
 // and overlaps the other sizes' searches in the step.  (Not for walk-only
 // batches, whose few searched groups the pre-pass would delay more than it
-// saves.)
-constexpr uint32_t kPrepassMinSize = 16;
+// saves.)  Size 32 only: at 512 x 512 without it the s = 32 search has
+// 2.37 M stage-1 survivors and 1.07 M exact fits (126 k and 19 k with it),
+// while at s = 16 the seed's thresholds leave 438 k survivors and the
+// search alone (0.115 ms) is quicker than pre-pass and search (0.141 ms),
+// whose 128-block pre-pass also ran last in the step.
+constexpr uint32_t kPrepassMinSize = 32;
 // The pre-pass samples every s-th tile at size s: at 512 x 512 the sparser
 // sample (against every 8th) leaves the s = 32 survivors at 126 k (from 83 k)
 // but finishes the launch early enough for the full pass to overlap the

@@ -723,7 +723,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // Cauchy-Schwarz survivors sit on low-variance domains that only an
     // alfa above the largest quantised one could fit.
     constexpr bool CAP_LIMIT = S <= 4;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // (wave through readfirstlane: the tile loop and its bounds tests are
+    // then scalar)
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int h = lane >> 5;
     if (c.nwork && blockIdx.x >= *glb(c.nwork)) return; // past a compacted list's end
     const Work3 wk = work[blockIdx.x];
