@@ -9,8 +9,10 @@ The searched node set of the RD walk is every aligned node of every size
   x 8 isometries = 2.77e9 evals.
 A step runs pool build + range prep + search + merge on device-resident data
 (plane in HBM at the start, winners table in HBM at the end), replayed from
-one captured HIP graph.  `value` is evals/s of the split-INT8 MFMA kernel; the
-plain FP32-FMA kernel is timed beside it on the same workload.
+one captured HIP graph.  `value` is evals/s of the MFMA search (exact integer
+correlation: binary16 operands with binary32 accumulation at sizes 4 and 8,
+split-INT8 with INT32 accumulation at 16 and 32); the plain FP32-FMA kernel is
+timed beside it on the same workload.
 
 Also measured (extra keys of the same line): 64-plane batches, the colour
 workload (768x512 RGB -> YCbCr 4:2:0, Fisher classified search with the
@@ -33,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_INT8_OPS = 5.0e15   # MI355X dense INT8 MFMA, ops/s
+PEAK_F16_FLOPS = 2.5e15  # MI355X dense FP16 MFMA, flop/s
 PEAK_HBM = 8.0e12        # bytes/s
 
 W = H = 512
@@ -558,7 +561,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8 MFMA (i32 acc) + f64 fit",
+        "dtype": "exact integer correlation on MFMA (f16 operands / f32 acc at s<=8, "
+                 "split int8 / i32 acc at s>=16) + f64 fit",
         "data": "synthetic (seeded generator, mars_amd/synth.py, seed 2)",
         "config": {"workload": "quadtree512: 512x512 u8 plane, range sizes 32->4, domain stride 4, "
                                "8 isometries, exhaustive, whole RD node set per step",
@@ -580,7 +584,10 @@ def main() -> int:
         "roofline": {"bound": "mfma", "kernel": f"k_search_mfma<{s}>", "achieved": achieved / 1e12,
                      "peak": PEAK_INT8_OPS / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_INT8_OPS,
                      "traffic": _profiled_traffic(f"k_search_mfma<{s}>"),
-                     "op": "int8 ops (2*s^2 per eval)",
+                     "op": "2*s^2 ops per eval (integer multiply-adds)",
+                     "mfma_instruction": ("v_mfma_f32_32x32x16_f16" if s in (4, 8)
+                                          else "v_mfma_i32_32x32x32_i8 (two split planes)"),
+                     "frac_of_f16_peak": (achieved / PEAK_F16_FLOPS if s in (4, 8) else None),
                      "avg_launch_ms": dom_ms,
                      "valu_frac": valu_frac,
                      "valu_frac_basis": "SQ_INSTS_VALU of the launch (profiles/quadtree512_search_counters.md) "

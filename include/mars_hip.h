@@ -214,7 +214,8 @@ int mars_hip_set_walk_tables(mars_hip_ctx *ctx, double t_rms);
  * d_px must stay valid until then.  Every buffer of the step lives in a
  * static device array of MARS_HIP_NOALLOC_SCRATCH_BYTES per GPU; a plane
  * whose step needs more is refused with MARS_HIP_E_UNSUPPORTED before
- * anything runs (a 1024 x 1024 plane fits at any size and stride >= 2).
+ * anything runs (a 1024 x 1024 plane fits at sizes 4..32 with stride >= 2
+ * and at size 2 with stride >= 4; a 2048 x 2048 plane does not).
  * Every call on a GPU shares that array and starts by clearing the part it
  * uses: calls that overlap on one GPU (two streams, two threads, two
  * contexts of one process) race and must be serialised by the caller.

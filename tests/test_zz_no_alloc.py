@@ -91,6 +91,23 @@ def test_noalloc_equals_prepare_on_consecutive_calls():
     ctx.close()
 
 
+@pytest.mark.parametrize("s,shift", [(2, 4), (4, 2), (8, 2), (16, 2), (32, 1)])
+def test_noalloc_fits_the_documented_1024_planes(s, shift):
+    """The header's statement of what fits the scratch: a 1024 x 1024 plane
+    at sizes 4..32 with stride >= 2 and at size 2 with stride >= 4 (stride 1
+    fits at size 32 only); every range finds a winner."""
+    ctx = HipContext(0)
+    img = synth.plane(1024, 1024, 67)
+    print(f"noalloc 1024x1024 size {s} stride {shift}", flush=True)
+    rc, msg, got = _noalloc(ctx, img, EncodeParams(min_size=s, max_size=s, shift=shift))
+    assert rc == N.OK, msg
+    assert len(got) == (1024 // s) ** 2 and all(x is not None for x in got)
+    if s == 2:
+        rc, msg, _ = _noalloc(ctx, img, EncodeParams(min_size=2, max_size=2, shift=2))
+        assert rc == N.E_UNSUPPORTED and "scratch" in msg, msg
+    ctx.close()
+
+
 def test_noalloc_refuses_what_exceeds_the_scratch():
     ctx = HipContext(0)
     img = np.zeros((4096, 4096), np.uint8)
