@@ -150,7 +150,6 @@ struct Table {
     uint32_t nchunks[2] = {1, 1}, tpc[2] = {1, 1}; // current chunking
     uint32_t *d_rows = nullptr, *d_cols = nullptr;
     int8_t *d_rng8 = nullptr;
-    uint16_t *d_rngh = nullptr; // binary16 range operand (sizes 4 and 8)
     float *d_rngf = nullptr;
     int32_t *d_rstat = nullptr;
     int32_t *d_rpos = nullptr;
@@ -473,7 +472,6 @@ int layout(mars_hip_ctx *ctx)
         d.rows = t.d_rows;
         d.cols = t.d_cols;
         d.rng8 = t.d_rng8;
-        d.rngh = t.d_rngh;
         d.rngf = t.d_rngf;
         d.rstat = t.d_rstat;
         d.rpos = t.d_rpos;
@@ -1094,8 +1092,6 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         plan.add(&t.d_rows, 4 * (size_t)t.nr);
         plan.add(&t.d_cols, 4 * (size_t)t.nr);
         plan.add(&t.d_rng8, (size_t)t.nr_pad * 8 * t.kmem);
-        t.d_rngh = nullptr;
-        if (uses_f16(t.size)) plan.add(&t.d_rngh, sizeof(uint16_t) * t.nr_pad * 8 * t.kmem);
         plan.add(&t.d_rngf, (size_t)t.nr_pad / 4 * K * 32 * sizeof(float));
         plan.add(&t.d_rstat, sizeof(int32_t) * 2 * t.nr_pad);
         plan.add(&t.d_rpos, sizeof(int32_t) * t.nr_pad);
@@ -1682,7 +1678,6 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
     td.rows = g.rows;
     td.cols = g.cols;
     td.rng8 = (int8_t *)take((size_t)nr_pad * 8 * kmem);
-    td.rngh = uses_f16(s) ? (uint16_t *)take(sizeof(uint16_t) * (size_t)nr_pad * 8 * kmem) : nullptr;
     td.rngf = nullptr; // the FP32 baseline's operand: not used here
     td.rstat = (int32_t *)take(sizeof(int32_t) * 2 * nr_pad);
     td.rpos = (int32_t *)take(sizeof(int32_t) * nr_pad);

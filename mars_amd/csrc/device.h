@@ -118,7 +118,6 @@ struct TableG {
     uint32_t w, wc, pitch, copy_stride;
     const GLB uint32_t *rows, *cols;
     GLB int8_t *rng8;
-    GLB uint16_t *rngh;
     GLB float *rngf;
     GLB int32_t *rstat;
     GLB int32_t *rpos;
@@ -163,7 +162,6 @@ __device__ __forceinline__ TableG view(const TableDesc &t)
     g.rows = glb(t.rows);
     g.cols = glb(t.cols);
     g.rng8 = glb(t.rng8);
-    g.rngh = glb(t.rngh);
     g.rngf = glb(t.rngf);
     g.rstat = glb(t.rstat);
     g.rpos = glb(t.rpos);

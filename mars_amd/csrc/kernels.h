@@ -53,7 +53,6 @@ struct TableDesc {
     // the table
     const uint32_t *rows, *cols;
     int8_t *rng8;        // [nr_pad][8][kmem] r - 128
-    uint16_t *rngh;      // [nr_pad][8][kmem] r - 128 as binary16 (sizes 4 and 8, else null)
     float *rngf;         // [nr_pad/4][K][32] r (FP32 baseline)
     int32_t *rstat;      // [nr_pad][2] t0, t2
     int32_t *rpos;       // [nr_pad] 3 * sum max(r - 128, 0)

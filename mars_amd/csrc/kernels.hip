@@ -259,7 +259,6 @@ __global__ __launch_bounds__(256) void k_ranges(const TableDesc *__restrict__ ta
     const uint8_t *src = s_px + (r - rb) * K;
     GLB float *rngf = fp32 ? T.rngf : nullptr;
     uint32_t packed = 0;
-    uint32_t hpk[2] = {0, 0}; // the same four values as binary16
 #pragma unroll
     for (uint32_t b = 0; b < 4; b++) {
         const uint32_t k = k0 + b;
@@ -269,14 +268,12 @@ __global__ __launch_bounds__(256) void k_ranges(const TableDesc *__restrict__ ta
             iso_map(iso, u, v, size, &i, &j);
             const uint32_t pix = src[i * size + j];
             packed |= ((pix - 128u) & 0xffu) << (8 * b);
-            hpk[b >> 1] |= (uint32_t)f16_bits((int32_t)pix) << (16 * (b & 1));
             if (rngf) rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = (float)pix;
         } else if (rngf && k < K) {
             rngf[((size_t)(r >> 2) * K + k) * 32 + iso * 4 + (r & 3)] = 0.0f;
         }
     }
     *(uint32_t *)(T.rng8 + e) = packed;
-    if (T.rngh) *(GLB u32x2 *)(T.rngh + e) = (u32x2){hpk[0], hpk[1]};
 }
 
 // Range sums t0 = sum r, t2 = sum r^2 and 3 * sum max(r - 128, 0), work
@@ -353,16 +350,23 @@ __device__ __forceinline__ v8h load_bh(const GLB uint16_t *wf, uint32_t boff, ui
 }
 
 // Binary16 A fragment: row (range, isometry) of lane & 31 as load_a, k = 16
-// kk + 8 h .. + 7.
+// kk + 8 h .. + 7, built from the i8 operand (r - 128) as r: the fragments
+// are loaded once per block, so the conversion costs nothing per tile and
+// the range operand is stored once, in i8.
 template <int S>
-__device__ __forceinline__ v8h load_ah(const GLB uint16_t *rngh, uint32_t g, int tt, int kk, int lane)
+__device__ __forceinline__ v8h load_ah(const GLB int8_t *rng8, uint32_t g, int tt, int kk, int lane)
 {
     constexpr int KMEM = Geo<S>::KMEM;
     const int m = lane & 31, h = lane >> 5;
     const int rl = ((m >> 2) & 1) * 4 + (m & 3);
     const int iso = 4 * tt + (m >> 3);
-    const GLB uint16_t *p = rngh + ((size_t)(g * 8 + rl) * 8 + iso) * KMEM + kk * 16 + h * 8;
-    return __builtin_bit_cast(v8h, *(const GLB u32x4 *)p);
+    const GLB int8_t *p = rng8 + ((size_t)(g * 8 + rl) * 8 + iso) * KMEM + kk * 16 + h * 8;
+    const u32x2 w = *(const GLB u32x2 *)p;
+    v8h a;
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+        a[e] = (_Float16)(float)((int32_t)(int8_t)(w[e >> 2] >> (8 * (e & 3))) + 128);
+    return a;
 }
 
 // B fragment (one split plane) of lane half h for K-step kk of the window
@@ -828,7 +832,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
 #pragma unroll
         for (int tt = 0; tt < 2; tt++)
 #pragma unroll
-            for (int kk = 0; kk < NKH; kk++) ahf[tt][kk] = load_ah<S>(T.rngh, g, tt, kk, lane);
+            for (int kk = 0; kk < NKH; kk++) ahf[tt][kk] = load_ah<S>(T.rng8, g, tt, kk, lane);
     } else if constexpr (G::PRELOAD_A) {
 #pragma unroll
         for (int tt = 0; tt < 2; tt++)
