@@ -550,7 +550,8 @@ int layout(mars_hip_ctx *ctx)
     for (uint32_t i = 0; i < ctx->tables.size(); i++) {
         const Table &t = ctx->tables[i];
         if (t.P == 0) continue;
-        for (uint32_t b = 0; b < cdiv(t.nr, 4); b++) w2.push_back(make_uint2(i, b));
+        const uint32_t per = mars::seed_ranges_per_block(t.size, ctx->topk);
+        for (uint32_t b = 0; b < cdiv(t.nr, per); b++) w2.push_back(make_uint2(i, b));
     }
     end2(ctx->wl_seed);
     // Fisher (tables with an empty pool keep their zeroed results)
@@ -621,7 +622,8 @@ int layout(mars_hip_ctx *ctx)
         for (uint32_t ti : g.tables) {
             const Table &t = ctx->tables[ti];
             if (!walk || t.P == 0) continue;
-            for (uint32_t b = 0; b < cdiv(t.nr, 4); b++) w2.push_back(make_uint2(ti, b));
+            const uint32_t per = mars::seed_ranges_per_block(t.size, ctx->topk);
+            for (uint32_t b = 0; b < cdiv(t.nr, per); b++) w2.push_back(make_uint2(ti, b));
         }
         end2(g.seed);
         begin2(g.activate);
@@ -1622,7 +1624,8 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
     const uint32_t n_contract = std::max<uint32_t>(1, cdiv(nc, 256));
     const uint32_t n_sat_rows = hc + 1, n_sat_cols = cdiv(wc + 1, 16), n_possums = cdiv(P_pad, 256);
     const uint32_t n_ranges = cdiv((uint64_t)nr_pad * 8 * kmem / 4, 256);
-    const uint32_t n_rstats = cdiv(nr_pad, 4), n_seed = cdiv(nr, 4), n_merge = cdiv(nr, 128);
+    const uint32_t n_rstats = cdiv(nr_pad, 4), n_merge = cdiv(nr, 128);
+    const uint32_t n_seed = cdiv(nr, mars::seed_ranges_per_block(s, 1));
     const uint32_t n2 = std::max({n_contract, n_sat_rows, n_sat_cols, n_possums, n_ranges, n_rstats,
                                   n_seed, n_merge});
     const uint64_t nsearch = (uint64_t)ngroups * nchunks;

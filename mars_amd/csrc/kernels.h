@@ -135,6 +135,13 @@ void launch_activate(const TableDesc *t, const Work2 *work, uint32_t nwork, doub
                      uint32_t min_size, Work3 *wq, uint32_t *wq_count, hipStream_t st);
 void launch_zero_u32(uint32_t *p, uint32_t n, hipStream_t st);
 // Work item (table, first of four ranges).
+// Ranges per work item of the threshold seed (work item y covers ranges
+// [y * this, ...)): 32 for the winner-only tables of sizes <= 8, whose
+// waves seed 8 ranges each, else 4 (a wave per range).
+inline __host__ __device__ uint32_t seed_ranges_per_block(uint32_t size, uint32_t topk)
+{
+    return size <= 8 && topk <= 1 ? 32u : 4u;
+}
 void launch_seed(const TableDesc *t, const Work2 *work, uint32_t nwork, const Common &c,
                  hipStream_t st);
 int launch_search(int kernel, uint32_t size, const TableDesc *t, const Work3 *work,

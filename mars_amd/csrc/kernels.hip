@@ -557,7 +557,34 @@ __device__ __forceinline__ void seed_wave(const TableG &T, const uint32_t r, con
     const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
     const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
     const GLB int8_t *rk = T.rng8 + ((size_t)r * 8 + iso) * T.kmem; // 16-byte aligned rows
-    const int32_t t1 = seed_t1<S>(rk, T.d + (size_t)dr * T.wc + dc, T.wc);
+    int32_t t1;
+    if constexpr (S >= 16) {
+        // From the split planes, whose window rows start 16-byte aligned at
+        // these sizes (the search order record's offset): 16 values per
+        // load and per four dot4 per plane, where one i32 load per value of
+        // the contracted plane made the s = 32 waves the seed's slowest.
+        // t1 = 4 sum r' Dh' + sum r' Dl + 512 sum r' + 128 S1, exactly.
+        const uint32_t boff = T.posinfo[T.iperm ? T.iperm[p] : p].x;
+        int32_t ah = 0, al = 0;
+        for (int u = 0; u < S; u++) {
+#pragma unroll
+            for (int c = 0; c < S; c += 16) {
+                const v4i x = *(const GLB v4i *)(rk + u * S + c);
+                const uint32_t o = boff + u * T.pitch + c;
+                const v4i hv = *(const GLB v4i *)(T.hi + o);
+                const v4i lv = *(const GLB v4i *)(T.lo + o);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    ah = __builtin_amdgcn_sdot4(x[q], hv[q], ah, false);
+                    al = __builtin_amdgcn_sdot4(x[q], lv[q], al, false);
+                }
+            }
+        }
+        const int32_t t0 = T.rstat[2 * r];
+        t1 = 4 * ah + al + 512 * (t0 - (int32_t)(128 * n)) + 128 * T.posx[2 * p];
+    } else {
+        t1 = seed_t1<S>(rk, T.d + (size_t)dr * T.wc + dc, T.wc);
+    }
     uint32_t qa, qb;
     double rms, sum;
     fit_f64(n, T.posx[2 * p], T.posx[2 * p + 1], T.rstat[2 * r], t1, T.rstat[2 * r + 1],
@@ -600,12 +627,83 @@ __device__ __forceinline__ void seed_wave(const TableG &T, const uint32_t r, con
     }
 }
 
+// Threshold seed of 8 ranges per wave (winner-only tables, sizes <= 8):
+// lanes 8i..8i+7 take range r0 + i at the 8 spread positions; each lane
+// correlates its window with the range's 8 isometries, fits the one with
+// the largest Ni = n t1 - S1 t0 (the free least-squares favourite), and the
+// range's threshold is the smallest inflated SSE of its 8 fits.  Any
+// candidate's SSE bounds the best one, so the threshold stays valid; the
+// fits, not the correlations, bound the seed, and a wave now does 8 ranges'.
+template <int S>
+__device__ __forceinline__ void seed8_wave(const TableG &T, const uint32_t r0, const Common &cm)
+{
+    constexpr int64_t n = (int64_t)S * S;
+    constexpr int K = S * S;
+    const uint32_t lane = threadIdx.x & 63, P = T.P;
+    const uint32_t r = r0 + (lane >> 3), j = lane & 7;
+    const bool live = r < T.nr && !(T.active && !T.active[r]);
+    uint32_t tb = 0x7f800000u;
+    if (live) {
+        const uint32_t p = P >= 8 ? (uint32_t)(((uint64_t)j * (P - 1)) / 7) : j % P;
+        const uint32_t pi = p / T.npc, pj = p - pi * T.npc;
+        const uint32_t dr = (pi * T.shift) >> 1, dc = (pj * T.shift) >> 1;
+        const GLB int32_t *dw = T.d + (size_t)dr * T.wc + dc;
+        int32_t dv[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) dv[k] = dw[(size_t)(k / S) * T.wc + k % S];
+        const int32_t S1 = T.posx[2 * p];
+        const int64_t t0 = T.rstat[2 * r];
+        int32_t best = 0;
+        int64_t best_ni = INT64_MIN;
+        for (uint32_t iso = 0; iso < 8; iso++) {
+            const GLB int8_t *rk = T.rng8 + ((size_t)r * 8 + iso) * T.kmem; // 16-byte aligned
+            int32_t t1 = 0;
+            if constexpr (S == 2) {
+                const uint32_t w = *(const GLB uint32_t *)rk;
+#pragma unroll
+                for (int k = 0; k < 4; k++) t1 += ((int32_t)(int8_t)(w >> (8 * k)) + 128) * dv[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; k += 16) {
+                    const v4i w = *(const GLB v4i *)(rk + k);
+#pragma unroll
+                    for (int e = 0; e < 16; e++)
+                        t1 += ((int32_t)(int8_t)((uint32_t)w[e >> 2] >> (8 * (e & 3))) + 128) * dv[k + e];
+                }
+            }
+            const int64_t ni = n * (int64_t)t1 - (int64_t)S1 * t0;
+            if (ni > best_ni) {
+                best_ni = ni;
+                best = t1;
+            }
+        }
+        uint32_t qa, qb;
+        double rms, sum;
+        fit_f64(n, S1, T.posx[2 * p + 1], t0, best, T.rstat[2 * r + 1], cm.max_alfa, cm.bits_alfa,
+                cm.bits_beta, &qa, &qb, &rms, &sum);
+        tb = tau_bits(sum, n);
+    }
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) tb = min(tb, (uint32_t)__shfl_xor((int)tb, off));
+    if (live && j == 0) T.gthr[r] = tb;
+}
+
 __global__ __launch_bounds__(256) void k_seed(const TableDesc *__restrict__ tabs,
                                               const Work2 *__restrict__ work, Common cm)
 {
     __shared__ uint32_t s_tau[4][64];
     const Work2 wk = work[blockIdx.x];
     const TableG T = view(tabs[wk.x]);
+    if (seed_ranges_per_block(T.size, cm.topk) == 32) {
+        const uint32_t r0 = wk.y * 32 + (threadIdx.x >> 6) * 8;
+        if (r0 >= T.nr) return; // the whole wave
+        switch (T.size) {
+        case 2: seed8_wave<2>(T, r0, cm); break;
+        case 4: seed8_wave<4>(T, r0, cm); break;
+        default: seed8_wave<8>(T, r0, cm); break;
+        }
+        return;
+    }
     const uint32_t wave = threadIdx.x >> 6, r = wk.y * 4 + wave;
     if (r >= T.nr) return; // the whole wave: no barrier below is block-wide
     if (T.active && !T.active[r]) return; // walk-only: not visited this step
