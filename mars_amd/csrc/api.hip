@@ -235,7 +235,7 @@ struct mars_hip_ctx {
     Work2 *d_work2 = nullptr;
     Work3 *d_work3 = nullptr;
     size_t work2_cap = 0, work3_cap = 0;
-    Slice wl_contract, wl_sat_rows, wl_sat_cols, wl_possums, wl_ranges, wl_rstats, wl_seed;
+    Slice wl_contract, wl_sat_rows, wl_sat_cols, wl_possums, wl_ranges, wl_seed;
     Slice wl_fdom, wl_frng, wl_ftab, wl_frange; // Fisher: positions, range blocks, tables, ranges
     unsigned long long *d_counters = nullptr;
     double *d_yscale = nullptr;
@@ -539,13 +539,6 @@ int layout(mars_hip_ctx *ctx)
             w2.push_back(make_uint2(i, b));
     }
     end2(ctx->wl_ranges);
-    begin2(ctx->wl_rstats);
-    for (uint32_t i = 0; i < ctx->tables.size(); i++) {
-        const Table &t = ctx->tables[i];
-        if (t.P == 0) continue;
-        for (uint32_t b = 0; b < cdiv(t.nr_pad, 4); b++) w2.push_back(make_uint2(i, b));
-    }
-    end2(ctx->wl_rstats);
     begin2(ctx->wl_seed);
     for (uint32_t i = 0; i < ctx->tables.size(); i++) {
         const Table &t = ctx->tables[i];
@@ -673,7 +666,6 @@ int enqueue_prep(mars_hip_ctx *ctx, int kernel, hipStream_t st, bool fork,
     }
     mars::launch_ranges(ctx->d_tabs, w2 + ctx->wl_ranges.off, ctx->wl_ranges.n,
                         kernel == MARS_HIP_KERNEL_FP32, rs);
-    mars::launch_rstats(ctx->d_tabs, w2 + ctx->wl_rstats.off, ctx->wl_rstats.n, rs);
     if (fork) HIP_TRY(ctx, hipEventRecord(ctx->prep_join_ev, rs));
     if (pool_ev) HIP_TRY(ctx, hipEventRecord(pool_ev[0], st));
     mars::launch_contract(ctx->d_planes, w2 + ctx->wl_contract.off, ctx->wl_contract.n,
@@ -1624,10 +1616,10 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
     const uint32_t n_contract = std::max<uint32_t>(1, cdiv(nc, 256));
     const uint32_t n_sat_rows = hc + 1, n_sat_cols = cdiv(wc + 1, 16), n_possums = cdiv(P_pad, 256);
     const uint32_t n_ranges = cdiv((uint64_t)nr_pad * 8 * kmem / 4, 256);
-    const uint32_t n_rstats = cdiv(nr_pad, 4), n_merge = cdiv(nr, 128);
+    const uint32_t n_merge = cdiv(nr, 128);
     const uint32_t n_seed = cdiv(nr, mars::seed_ranges_per_block(s, 1));
-    const uint32_t n2 = std::max({n_contract, n_sat_rows, n_sat_cols, n_possums, n_ranges, n_rstats,
-                                  n_seed, n_merge});
+    const uint32_t n2 = std::max({n_contract, n_sat_rows, n_sat_cols, n_possums, n_ranges, n_seed,
+                                  n_merge});
     const uint64_t nsearch = (uint64_t)ngroups * nchunks;
     if (n2 >= (1u << 24) || nsearch >= (1u << 24))
         return fail(ctx, MARS_HIP_E_UNSUPPORTED, "plane too large for one step");
@@ -1744,7 +1736,6 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
     mars::launch_grid_setup(g, st);
     if (P > 0) {
         mars::launch_ranges(tabs, w2, n_ranges, false, st);
-        mars::launch_rstats(tabs, w2, n_rstats, st);
     }
     mars::launch_contract(planes, w2, n_contract, counters, kCounters, st);
     mars::launch_sat(planes, w2, n_sat_rows, w2, n_sat_cols, st);

@@ -1,8 +1,8 @@
 // Fisher classified search (crates/mars-search/src/fisher.rs) on the GPU.
 //
 // Per (plane, range size, stride) table, after the pool (contracted plane,
-// summed-area tables), k_possums (window sums S1, S2), k_ranges (each range
-// under the 8 isometries) and k_rstats (t0, t2):
+// summed-area tables), k_possums (window sums S1, S2) and k_ranges (each
+// range under the 8 isometries, and its sums t0, t2):
 //   k_fisher_domains  every domain position classified from its quadrant
 //                     sums and sums of squares, read off the summed-area
 //                     tables (fisher.rs:33-38);

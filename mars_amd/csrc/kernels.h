@@ -124,7 +124,6 @@ void launch_sat(const PlaneDesc *planes, const Work2 *rows_work, uint32_t n_rows
 void launch_possums(const TableDesc *t, const Work2 *work, uint32_t nwork, hipStream_t st);
 void launch_ranges(const TableDesc *t, const Work2 *work, uint32_t nwork, bool fp32,
                    hipStream_t st);
-void launch_rstats(const TableDesc *t, const Work2 *work, uint32_t nwork, hipStream_t st);
 // Walk-only tables, work item (table, block of 256 ranges): active[r] from
 // the parent's merged result (split when its rms > t_rms and its size >
 // min_size, the threshold walk's rule).
@@ -155,7 +154,7 @@ void launch_merge_ranked(const TableDesc *t, const Work2 *work, uint32_t nwork, 
 // (dom_work: table, block of 256 positions) and range (rng_work: table,
 // block of 4 ranges), the buckets (tab_work: table, 0), then every range's
 // bucket scan (range_work: table, range).  Needs the pool, k_possums,
-// k_ranges and k_rstats of those tables.
+// and k_ranges of those tables.
 void launch_fisher(const TableDesc *t, const Work2 *dom_work, uint32_t n_dom,
                    const Work2 *rng_work, uint32_t n_rng, const Work2 *tab_work, uint32_t n_tab,
                    const Work2 *range_work, uint32_t n_range, const Common &c, hipStream_t st);

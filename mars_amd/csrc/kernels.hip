@@ -46,11 +46,8 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 // (i32), D as f32, and the i8 planes lo = D % 4, hi = D / 4 - 128 into
 // copies c = 0..15 at x = j - c (copy c holds the row shifted left by c
 // bytes).  Block 0 of the launch also clears the step's counters.
-__global__ void k_contract(const PlaneDesc *__restrict__ planes, const Work2 *__restrict__ work,
-                           unsigned long long *__restrict__ counters, uint32_t n_counters)
+__device__ __forceinline__ void contract_item(const PlaneDesc *__restrict__ planes, const Work2 wk)
 {
-    if (blockIdx.x == 0 && threadIdx.x < n_counters) glb(counters)[threadIdx.x] = 0;
-    const Work2 wk = work[blockIdx.x];
     const PlaneG pl = view(planes[wk.x]);
     const uint32_t idx = wk.y * blockDim.x + threadIdx.x;
     if (idx >= pl.wc * pl.hc) return;
@@ -229,15 +226,17 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
 
 // Range operands, work item (table, block): one thread per 4 bytes of
 // rng8, layout [range][iso][kmem] (i8 r-128, zero padded to kmem); with
-// `fp32` also rngf, layout [range/4][k][iso*4 + range%4] (f32 r).
-__global__ __launch_bounds__(256) void k_ranges(const TableDesc *__restrict__ tabs,
-                                                const Work2 *__restrict__ work, int fp32)
+// `fp32` also rngf, layout [range/4][k][iso*4 + range%4] (f32 r).  The block
+// that starts a range also writes its sums t0 = sum r, t2 = sum r^2 and
+// 3 * sum max(r - 128, 0) (zero for a padding range) and resets its pruning
+// threshold: its pixels are in LDS here anyway.
+__device__ __forceinline__ void ranges_item(const TableDesc *__restrict__ tabs, const Work2 wk,
+                                            int fp32)
 {
     // The block's 1024 output bytes cover whole ranges (sizes <= 8) or part
     // of one range; its ranges' pixels are staged in LDS first (row-wise,
     // coalesced), so each output byte is an LDS read rather than a gather.
     __shared__ uint8_t s_px[1024];
-    const Work2 wk = work[blockIdx.x];
     const TableG T = view(tabs[wk.x]);
     const uint32_t lg_size = __builtin_ctz(T.size), lg_kmem = __builtin_ctz(T.kmem);
     const uint32_t size = T.size, K = size * size, kmem = T.kmem;
@@ -252,6 +251,32 @@ __global__ __launch_bounds__(256) void k_ranges(const TableDesc *__restrict__ ta
                         : (uint8_t)128;
     }
     __syncthreads();
+    if ((e0 & (8 * kmem - 1)) == 0) { // the block starts its ranges (a wave per range)
+        const uint32_t lane = threadIdx.x & 63;
+        for (uint32_t i = threadIdx.x >> 6; i < nrb; i += 4) {
+            const uint32_t rr = rb + i;
+            int32_t s1 = 0, s2 = 0, sp = 0;
+            if (rr < T.nr)
+                for (uint32_t k = lane; k < K; k += 64) {
+                    const int32_t px = s_px[i * K + k];
+                    s1 += px;
+                    s2 += px * px;
+                    sp += px > 128 ? px - 128 : 0;
+                }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                s1 += __shfl_xor(s1, off);
+                s2 += __shfl_xor(s2, off);
+                sp += __shfl_xor(sp, off);
+            }
+            if (lane == 0) {
+                T.rstat[2 * rr] = s1;
+                T.rstat[2 * rr + 1] = s2;
+                T.rpos[rr] = 3 * sp; // largest sum of r'*Dl over Dl in [0, 3]
+                T.gthr[rr] = 0x7f800000u; // +inf: no threshold yet
+            }
+        }
+    }
     const uint32_t e = e0 + threadIdx.x * 4;
     if (e >= total) return;
     const uint32_t r = e >> (lg_kmem + 3), iso = (e >> lg_kmem) & 7, k0 = e & (kmem - 1);
@@ -276,38 +301,17 @@ __global__ __launch_bounds__(256) void k_ranges(const TableDesc *__restrict__ ta
     *(uint32_t *)(T.rng8 + e) = packed;
 }
 
-// Range sums t0 = sum r, t2 = sum r^2 and 3 * sum max(r - 128, 0), work
-// item (table, block of 4 ranges): one wave per (padded) range.  Also resets
-// the range's pruning threshold.
-__global__ void k_rstats(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work)
+__global__ void k_contract(const PlaneDesc *__restrict__ planes, const Work2 *__restrict__ work,
+                           unsigned long long *__restrict__ counters, uint32_t n_counters)
 {
-    const Work2 wk = work[blockIdx.x];
-    const TableG T = view(tabs[wk.x]);
-    const uint32_t r = wk.y * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (r >= T.nr_pad) return;
-    const uint32_t lg_size = __builtin_ctz(T.size), size = T.size, K = size * size;
-    int32_t s1 = 0, s2 = 0, sp = 0;
-    if (r < T.nr) {
-        const uint32_t row = T.rows[r], col = T.cols[r];
-        for (uint32_t e = lane; e < K; e += 64) {
-            const int32_t pix = T.px[(row + (e >> lg_size)) * T.w + col + (e & (size - 1))];
-            s1 += pix;
-            s2 += pix * pix;
-            sp += pix > 128 ? pix - 128 : 0;
-        }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        s1 += __shfl_xor(s1, off);
-        s2 += __shfl_xor(s2, off);
-        sp += __shfl_xor(sp, off);
-    }
-    if (lane == 0) {
-        T.rstat[2 * r] = s1;
-        T.rstat[2 * r + 1] = s2;
-        T.rpos[r] = 3 * sp; // largest sum of r'*Dl over Dl in [0, 3]
-        T.gthr[r] = 0x7f800000u; // +inf: no threshold yet
-    }
+    if (blockIdx.x == 0 && threadIdx.x < n_counters) glb(counters)[threadIdx.x] = 0;
+    contract_item(planes, work[blockIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_ranges(const TableDesc *__restrict__ tabs,
+                                                const Work2 *__restrict__ work, int fp32)
+{
+    ranges_item(tabs, work[blockIdx.x], fp32);
 }
 
 // ---------------------------------------------------------------- MFMA search
@@ -1927,10 +1931,7 @@ void launch_ranges(const TableDesc *t, const Work2 *work, uint32_t n, bool fp32,
     if (n) hipLaunchKernelGGL(k_ranges, dim3(n), dim3(256), 0, st, t, work, fp32 ? 1 : 0);
 }
 
-void launch_rstats(const TableDesc *t, const Work2 *work, uint32_t n, hipStream_t st)
-{
-    if (n) hipLaunchKernelGGL(k_rstats, dim3(n), dim3(256), 0, st, t, work);
-}
+
 
 void launch_merge_ranked(const TableDesc *t, const Work2 *work, uint32_t n, const Common &c,
                          hipStream_t st)
