@@ -373,10 +373,10 @@ __device__ __forceinline__ v8h load_ah(const GLB int8_t *rng8, uint32_t g, int t
     return a;
 }
 
-// B fragment (one split plane) of lane half h for K-step kk of the window
-// whose origin is byte `boff` of the plane's copy layout (the copy in which
-// the window's rows start 4/8/16-byte aligned; all copies of a plane span
-// < 4 GiB).  The k order inside a fragment is the same one used for the A
+// B fragment (one split plane; sizes 2, 16 and 32) of lane half h for
+// K-step kk of the window whose origin is byte `boff` of the plane's copy
+// layout (at sizes >= 16 the copy in which the window's rows start 16-byte
+// aligned; all copies of a plane span < 4 GiB).  The k order inside a fragment is the same one used for the A
 // fragment, so the product does not depend on how the instruction orders K
 // internally.
 template <int S>
@@ -392,23 +392,8 @@ __device__ __forceinline__ void load_b(const GLB int8_t *plane, uint32_t boff, u
                 ((uint32_t)p[pitch + 1] << 24);
         }
         *b8 = (long)x;
-    } else if constexpr (S == 4) {
-        // (one 32-bit offset per load: the plane base stays a scalar operand)
-        const uint32_t o = boff + 2 * h * pitch;
-        const uint32_t r0 = *(const GLB uint32_t *)(plane + o);
-        const uint32_t r1 = *(const GLB uint32_t *)(plane + (o + pitch));
-        *b8 = (long)((uint64_t)r0 | ((uint64_t)r1 << 32));
-    } else if constexpr (S == 8) {
-        const uint32_t o = boff + (4 * kk + 2 * h) * pitch;
-        const u32x2 r0 = *(const GLB u32x2 *)(plane + o);
-        const u32x2 r1 = *(const GLB u32x2 *)(plane + (o + pitch));
-        v4i t;
-        t.x = (int)r0.x;
-        t.y = (int)r0.y;
-        t.z = (int)r1.x;
-        t.w = (int)r1.y;
-        *b16 = t;
     } else {
+        static_assert(S >= 16, "sizes 4 and 8 search in binary16 (load_bh)");
         const uint32_t k0 = 32 * kk + 16 * h;
         *b16 = *(const GLB v4i *)(plane + boff + (k0 / S) * pitch + (k0 % S));
     }
