@@ -155,6 +155,7 @@ struct Table {
     int32_t *d_rpos = nullptr;
     uint32_t *d_gthr = nullptr;
     uint4 *d_posinfo = nullptr;
+    uint4 *d_posbias = nullptr; // sizes 4 and 8
     uint32_t *d_iperm = nullptr;
     int32_t *d_posx = nullptr;
     double *d_posd = nullptr;
@@ -477,6 +478,7 @@ int layout(mars_hip_ctx *ctx)
         d.rpos = t.d_rpos;
         d.gthr = t.d_gthr;
         d.posinfo = t.d_posinfo;
+        d.posbias = t.d_posbias;
         d.iperm = t.iperm.empty() ? nullptr : t.d_iperm;
         d.posx = t.d_posx;
         d.posd = t.d_posd;
@@ -1091,6 +1093,8 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
         plan.add(&t.d_rpos, sizeof(int32_t) * t.nr_pad);
         plan.add(&t.d_gthr, sizeof(uint32_t) * t.nr_pad);
         plan.add(&t.d_posinfo, sizeof(uint4) * (size_t)t.P_pad);
+        t.d_posbias = nullptr;
+        if (uses_f16(t.size)) plan.add(&t.d_posbias, 2 * sizeof(uint4) * (size_t)t.P_pad);
         plan.add(&t.d_iperm, sizeof(uint32_t) * t.iperm.size());
         plan.add(&t.d_posx, 2 * sizeof(int32_t) * (size_t)t.P);
         plan.add(&t.d_posd, 2 * sizeof(double) * (size_t)t.P);
@@ -1678,6 +1682,7 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
     td.rpos = (int32_t *)take(sizeof(int32_t) * nr_pad);
     td.gthr = (uint32_t *)take(sizeof(uint32_t) * nr_pad);
     td.posinfo = (uint4 *)take(sizeof(uint4) * (size_t)P_pad);
+    td.posbias = uses_f16(s) ? (uint4 *)take(2 * sizeof(uint4) * (size_t)P_pad) : nullptr;
     g.iperm = order_m ? (uint32_t *)take(sizeof(uint32_t) * (size_t)P) : nullptr;
     td.iperm = g.iperm;
     td.posx = (int32_t *)take(2 * sizeof(int32_t) * (size_t)P);

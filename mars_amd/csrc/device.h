@@ -28,6 +28,29 @@ __device__ __forceinline__ uint16_t f16_bits(int32_t v)
     return __builtin_bit_cast(uint16_t, (_Float16)(float)v);
 }
 
+// binary16 bits of a float that binary16 holds exactly.
+__device__ __forceinline__ uint16_t f16_bits_of(float x)
+{
+    return __builtin_bit_cast(uint16_t, (_Float16)x);
+}
+
+// binary16 bits of x >= 0 rounded toward zero (the largest binary16 <= x;
+// 65504 above the range).
+__device__ __forceinline__ uint16_t f16_rd_bits(float x)
+{
+    const _Float16 h = (_Float16)x;
+    const uint16_t b = __builtin_bit_cast(uint16_t, h);
+    return (float)h > x ? (uint16_t)(b - 1) : b;
+}
+
+// binary16 bits of x >= 0 rounded up (the smallest binary16 >= x, x <= 65504).
+__device__ __forceinline__ uint16_t f16_ru_bits(float x)
+{
+    const _Float16 h = (_Float16)x;
+    const uint16_t b = __builtin_bit_cast(uint16_t, h);
+    return (float)h < x ? (uint16_t)(b + 1) : b;
+}
+
 // Class types (uint4, Partial, mars_hip_candidate) cannot be copied to or
 // from address space 1 in C++, so the views hold them as plain vectors and
 // words and these helpers move them.
@@ -123,6 +146,7 @@ struct TableG {
     GLB int32_t *rpos;
     GLB uint32_t *gthr;
     GLB u32x4 *posinfo;
+    GLB u32x4 *posbias;
     GLB int32_t *posx;
     const GLB uint32_t *iperm;
     GLB double *posd;
@@ -167,6 +191,7 @@ __device__ __forceinline__ TableG view(const TableDesc &t)
     g.rpos = glb(t.rpos);
     g.gthr = glb(t.gthr);
     g.posinfo = (GLB u32x4 *)t.posinfo;
+    g.posbias = (GLB u32x4 *)t.posbias;
     g.posx = glb(t.posx);
     g.iperm = glb(t.iperm);
     g.posd = glb(t.posd);

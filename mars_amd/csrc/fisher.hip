@@ -240,11 +240,11 @@ __device__ __forceinline__ void fisher_range(const TableG &T, uint32_t r, uint32
     auto gather = [&](uint32_t c) {
         const uint32_t e = T.flist[start + c];
         const uint32_t k = fisher::kMapping[isom][e & 7];
-        const uint4 rec = ld_u4(T.posinfo, e >> 3); // {window offset, S1, -, p}
+        const uint3 rec = ld_u3(T.posinfo, e >> 3); // {window offset, S1, p}
         int32_t ah = 0, al = 0;
         window_dots<S>(T.hi + rec.x, T.lo + rec.x, T.pitch, rng + (size_t)k * T.kmem, ah, al);
         const int32_t s1 = (int32_t)rec.y;
-        return Cand{rec.w, k, 4 * ah + al + off_r + 128 * s1, s1, T.posx[2 * rec.w + 1]};
+        return Cand{rec.z, k, 4 * ah + al + off_r + 128 * s1, s1, T.posx[2 * rec.z + 1]};
     };
     auto consider = [&](const Cand &x) {
         // Free least-squares bound with alfa2 >= 0 (encode.rs:261-267):

@@ -58,10 +58,15 @@ struct TableDesc {
     int32_t *rpos;       // [nr_pad] 3 * sum max(r - 128, 0)
     uint32_t *gthr;      // [nr_pad] pruning threshold (float bits, SSE units)
     // Search-order records, P_pad = 32 * ceil(P / 32) slots:
-    // {window byte offset, S1, sqrt(n*S2 - S1^2) as f32 rounded down, p}
+    // {window byte offset, S1, p, sqrt(n*S2 - S1^2) as f32 rounded down}
     // (p = ~0 in padding slots).  Slot of position p: iperm[p], or p when
     // iperm is null.
     uint4 *posinfo;
+    // Sizes 4 and 8 (null otherwise), per search slot, two records: the
+    // position's factors of the stage-1 bound as sixteen binary16 values (the
+    // B operand of one more MFMA K-step), {S1h 256/n, S1l/n, dh, dl, 2^11, 1,
+    // 1, 1 | dh, 0, ...} with S1 = 256 S1h + S1l and dh + dl <= sqrt(Di)/n.
+    uint4 *posbias;
     const uint32_t *iperm;
     int32_t *posx;       // [P][2] S1, S2
     double *posd;        // [P][2] Di = n*S2 - S1^2, 1/Di (binary64, stage 2)

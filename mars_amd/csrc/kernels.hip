@@ -187,7 +187,7 @@ __global__ void k_sat_cols(const PlaneDesc *__restrict__ planes, const Work2 *__
 
 // Work item (table, block of 256 domain positions): window sums from the
 // SATs.  Per position p: {S1, S2} and {Di, 1/Di} (binary64); the search's
-// hot record {window byte offset, S1, sqrt(Di) as f32, p} goes to slot
+// hot record {window byte offset, S1, p, sqrt(Di) as f32} goes to slot
 // iperm[p] of the search order (identity when iperm is null), and slots
 // P..P_pad-1 get a padding record (p = ~0).
 __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__restrict__ work)
@@ -197,8 +197,13 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
     const uint32_t p = wk.y * blockDim.x + threadIdx.x;
     if (p >= T.P_pad) return;
     if (p >= T.P) { // padding slot of the last search tile: no position
-        // S1 = -2^21: the MFMA search's stage-1 limit excludes the slot.
-        T.posinfo[p] = (u32x4){0u, (uint32_t)-(1 << 21), 0u, 0xffffffffu};
+        // S1 = -2^21: the MFMA search's stage-1 limit excludes the slot
+        // (binary16 sizes: the 2^12 in k 9 of its bias record).
+        T.posinfo[p] = (u32x4){0u, (uint32_t)-(1 << 21), 0xffffffffu, 0u};
+        if (T.posbias) {
+            T.posbias[2 * p] = (u32x4){0u, 0u, 0u, 0u};
+            T.posbias[2 * p + 1] = (u32x4){0x6c000000u, 0u, 0u, 0u}; // k 9 = 2^12
+        }
         return;
     }
     const uint32_t W1 = T.wc + 1, size = T.size;
@@ -217,7 +222,26 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
     const uint32_t cp = dc & cmask;
     const uint32_t boff = cp * T.copy_stride + dr * T.pitch + (dc - cp);
     const uint32_t q = T.iperm ? T.iperm[p] : p; // slot in search order
-    T.posinfo[q] = (u32x4){boff, (uint32_t)s1, __float_as_uint(sq), p};
+    T.posinfo[q] = (u32x4){boff, (uint32_t)s1, p, __float_as_uint(sq)};
+    if (T.posbias) {
+        // Binary16 sizes: the position's factors of the stage-1 bound (the
+        // B operand of the bias K-step of k_search_mfma, 32 bytes).  Every value is
+        // exact or rounded down: S1 = 256 S1h + S1l with S1h <= 63 (s = 4) or
+        // 255 (s = 8), so S1h 256/n and S1l/n are binary16 integers times
+        // powers of two; dh + dl <= sqrt(Di)/n, the (1 - 2^-20) covering the
+        // binary64 square root's rounding before sq was rounded down.
+        const uint32_t S1 = (uint32_t)s1, s1h = S1 >> 8, s1l = S1 & 255;
+        const float inv_n = 1.0f / (float)n; // a power of two
+        const float dn = sq * (1.0f - 0x1p-20f) * inv_n;
+        const uint32_t dh = f16_rd_bits(dn);
+        const uint32_t dl = f16_rd_bits(dn - (float)__builtin_bit_cast(_Float16, (uint16_t)dh));
+        const uint32_t b0 = f16_bits_of((float)s1h * (256.0f * inv_n));
+        const uint32_t b1 = f16_bits_of((float)s1l * inv_n);
+        // k 0..7 (lane half 0), then k 8..15 (half 1); k 4..7 are the
+        // constants {2^11, 1, 1, 1} of the intercept and margin slots
+        T.posbias[2 * q] = (u32x4){b0 | (b1 << 16), dh | (dl << 16), 0x3c006800u, 0x3c003c00u};
+        T.posbias[2 * q + 1] = (u32x4){dh, 0u, 0u, 0u};
+    }
     T.posx[2 * p] = (int32_t)s1;
     T.posx[2 * p + 1] = (int32_t)s2;
     T.posd[2 * p] = (double)di;
@@ -354,11 +378,13 @@ __device__ __forceinline__ v8h load_bh(const GLB uint16_t *wf, uint32_t boff, ui
 }
 
 // Binary16 A fragment: row (range, isometry) of lane & 31 as load_a, k = 16
-// kk + 8 h .. + 7, built from the i8 operand (r - 128) as r: the fragments
-// are loaded once per block, so the conversion costs nothing per tile and
-// the range operand is stored once, in i8.
+// kk + 8 h .. + 7, built from the i8 operand (r - 128) as r - 128 + add
+// (the search centres r: add = 128 - round(t0/n), |r - m| <= 255 exact):
+// the fragments are loaded once per block, so the conversion costs nothing
+// per tile and the range operand is stored once, in i8.
 template <int S>
-__device__ __forceinline__ v8h load_ah(const GLB int8_t *rng8, uint32_t g, int tt, int kk, int lane)
+__device__ __forceinline__ v8h load_ah(const GLB int8_t *rng8, uint32_t g, int tt, int kk, int lane,
+                                       int32_t add)
 {
     constexpr int KMEM = Geo<S>::KMEM;
     const int m = lane & 31, h = lane >> 5;
@@ -369,7 +395,7 @@ __device__ __forceinline__ v8h load_ah(const GLB int8_t *rng8, uint32_t g, int t
     v8h a;
 #pragma unroll
     for (int e = 0; e < 8; e++)
-        a[e] = (_Float16)(float)((int32_t)(int8_t)(w[e >> 2] >> (8 * (e & 3))) + 128);
+        a[e] = (_Float16)(float)((int32_t)(int8_t)(w[e >> 2] >> (8 * (e & 3))) + add);
     return a;
 }
 
@@ -741,6 +767,10 @@ __device__ __forceinline__ bool stage2_keep(const Common &a, double nd, double i
     return boundary || sse <= (double)thr;
 }
 
+__device__ __forceinline__ int32_t vmax(int32_t a, int32_t b) { return max(a, b); }
+// IEEE maximum (v_maximum3_f32): no NaN quieting of the MFMA results.
+__device__ __forceinline__ float vmax(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+
 // Survivor queue: the wave drains full batches of 64 after every DRAIN
 // values (one value per lane each), so DRAIN*64 + 63 entries always fit.
 template <int S> struct Queue {
@@ -907,6 +937,100 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
 #pragma unroll
     for (int i = 0; i < 4; i++) sa[i] = INT_LIMIT ? 1.0f : -1.0f;
     uint32_t seen = 0xffffffffu; // s_epoch the limits were computed at (none yet)
+    uint32_t ep_pre = 0;
+    bool dirty = false;
+    // Binary16 sizes: the stage-1 limit rides on the matrix pipe.  The
+    // range operand is centred, r - m with m = round(t0/n), so the K-steps
+    // give the exact integer X0 = t1 - m S1 (|X0| <= 1020 |r - m|_1 < 2^24).
+    // With a = A - n tau, x = sqrt(a), d = sqrt(Di), a candidate can win
+    // only if n t1 - S1 t0 >= L(d) = x d + max(x - U d, 0)^2 / (2U) (the
+    // slope cap U bounds alfa/4), i.e. X0 >= S1 e/n + L(d)/n, e = t0 - n m.
+    // L is convex in d, so every tangent T_t(d) = (x - t) d + t (2x - t)/(2U)
+    // (t >= 0) lies below it.  One more K-step per tile gives, per (range,
+    // position) and per isometry row j of the range, a right-hand side
+    //   nb_j = S1 e/n + ((x - t_j) d + t_j (2x - t_j)/(2U))/n - M
+    // with t = 0 (the plain bound, L's line beyond d = x/U) and the tangents
+    // at d = 0, x/(3U), 2x/(3U) (t = x, 2x/3, x/3); a candidate passes when
+    // X0 exceeds the largest.  As a rank-10 product (A per row, B per
+    // position; k 0..7 in lane half 0, k 8, 9 in half 1):
+    //   A  {e, e, sh, sh, k1, k2, k3, -M | sl, 2^15, 0..}
+    //   B  {S1h 256/n, S1l/n, dh, dl, 2^11, 1, 1, 1 | dh, 0 or 2^12, 0..}
+    // S1 = 256 S1h + S1l and e exact binary16 integers (|e| <= n/2), sh + sl
+    // <= x - t and dh + dl <= d/n, 2^11 k1 + k2 + k3 <= t (2x - t)/(2U n),
+    // all rounded down, sl dl dropped: the exact nb_j never exceeds the
+    // bound less M.  M = 2^-19 E with E the terms' largest possible sum of
+    // magnitudes (tau = 0: 1020 (|e| + sqrt(A)) + A/(2U n) at most; 1020 >=
+    // D and d/n) exceeds the binary32 rounding of the K-step (at most 9
+    // additions of an ulp of E each, whatever the instruction's order).
+    // While A - n tau <= 0 the 2^11 slot is -2^15 (nb < 2^15 - 2^26 < -2^24
+    // <= X0: everything passes), and 2^15 for a padding range (nb > 2^26 -
+    // 2^15 > X0); a padding position's 2^12 in k 9 adds 2^27 (nb > 2^26).
+    const int brl = ((lane & 31) >> 2 & 1) * 4 + (lane & 3); // the A row's range
+    const int bj = (lane & 31) >> 3;                          // its isometry row: the bound
+    uint32_t bconst = 0;   // {e, e} (binary16) of a real range
+    float balo = 0.0f;     // A of the row's range rounded down
+    float be0 = 0.0f;      // 1020 |e|
+    bool bvalid = false;   // the row's range is real
+    int32_t acentre = 128; // r - 128 + acentre = r - m
+    v8h abias = (v8h){};
+    if constexpr (F16) {
+        const uint32_t r = g * 8 + brl;
+        bvalid = r < T.nr;
+        const int64_t t0 = bvalid ? T.rstat[2 * r] : 0, t2 = bvalid ? T.rstat[2 * r + 1] : 0;
+        const int64_t m = (t0 + n / 2) >> LOG2N, e = t0 - n * m;
+        acentre = 128 - (int32_t)m;
+        balo = (float)(double)(n * t2 - t0 * t0) * (1.0f - 3e-7f);
+        be0 = 1020.0f * (float)(e < 0 ? -e : e);
+        const uint32_t eh = f16_bits_of((float)e);
+        bconst = eh | (eh << 16);
+    }
+    // Per lane: the row's tangent t = tf x gives slope (1 - tf) x = bsl x
+    // and intercept tf (2 - tf) x^2 cap_k / n = (A - n tau) bkn (cap_k <=
+    // 1/(2U)), both rounded down: 2^-19 covers the roundings of the
+    // products, of A - n tau and of the square root.
+    const float btf = bj == 0 ? 0.0f : bj == 1 ? 1.0f : bj == 2 ? 2.0f / 3.0f : 1.0f / 3.0f;
+    const float bsl = (1.0f - btf) * (1.0f - 0x1p-19f);
+    const float bkn = btf * (2.0f - btf) * c.cap_k * (1.0f / (float)n) * (1.0f - 0x1p-19f);
+    // -M in the high half: M = 2^-19 (1020 (|e| + sqrt(A) bsl) + A bkn),
+    // the terms' sum at tau = 0 (the largest), kept above that value through
+    // the rounding toward zero by the 2^-8; at least 2^-14 (binary16 normal)
+    uint32_t bmh = 0;
+    if constexpr (F16) {
+        const float sa = sqrtf(fmaxf(balo, 0.0f)) * (1.0f + 0x1p-10f);
+        const float mm = fmaxf((be0 + 1020.0f * sa * bsl + fmaxf(balo, 0.0f) * (1.0f + 0x1p-10f) * bkn) * 0x1p-19f * (1.0f + 0x1p-8f), 0x1p-14f);
+        bmh = ((uint32_t)__builtin_bit_cast(uint16_t, __builtin_amdgcn_cvt_pkrtz(mm, 0.0f)[0]) | 0x8000u) << 16;
+    }
+    // The A operand of the bias K-step for threshold bits `tb` of the row's
+    // range; v_cvt_pkrtz_f16_f32 rounds two values toward zero, i.e. down
+    // for these non-negative ones.  Rebuilt whenever a threshold of the
+    // block drops, so it is kept short.
+    auto make_abias = [&](uint32_t tb) -> v8h {
+        const float amt = balo - (float)n * __uint_as_float(tb);
+        const float x = __builtin_amdgcn_sqrtf(fmaxf(amt, 0.0f));
+        const float sg = x * bsl;
+        const float kn = fmaxf(amt, 0.0f) * bkn;
+        const auto p1 = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(sg, kn * 0x1p-11f));
+        const float sh = (float)__builtin_bit_cast(_Float16, (uint16_t)p1);
+        const float k1 = (float)__builtin_bit_cast(_Float16, (uint16_t)(p1 >> 16)) * 2048.0f;
+        const float r1 = kn - k1;
+        const auto p2 = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(sg - sh, r1));
+        const float r2 = r1 - (float)__builtin_bit_cast(_Float16, (uint16_t)(p2 >> 16));
+        const auto p3 = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(r2, 0.0f)) | bmh;
+        const uint32_t shb = p1 & 0xffffu;
+        uint32_t w0 = bconst, w1 = shb | (shb << 16);
+        uint32_t w2 = (p1 >> 16) | (p2 & 0xffff0000u);  // {k1, k2}
+        uint32_t w3 = p3;                               // {k3, -M}
+        uint32_t w4 = p2 & 0xffffu;
+        if (!(amt > 0.0f) || !bvalid) {
+            // no pruning yet: -2^15 in the 2^11 slot; a padding range: 2^15
+            w0 = bvalid ? bconst : 0u;
+            w1 = w3 = w4 = 0u;
+            w2 = bvalid ? 0xf800u : 0x7800u;
+        }
+        // half 1: k 8 = sl, k 9 = 2^15 (times 2^12 for a padding position)
+        const u32x4 w = h == 0 ? (u32x4){w0, w1, w2, w3} : (u32x4){w4 | 0x78000000u, 0u, 0u, 0u};
+        return __builtin_bit_cast(v8h, w);
+    };
     // Lower a threshold, noting the drop for every wave.
     auto lower = [&](uint32_t slot, uint32_t v) {
         if (atomicMin(&s_thr[slot], v) > v) atomicAdd(&s_epoch, 1u);
@@ -919,7 +1043,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
 #pragma unroll
         for (int tt = 0; tt < 2; tt++)
 #pragma unroll
-            for (int kk = 0; kk < NKH; kk++) ahf[tt][kk] = load_ah<S>(T.rng8, g, tt, kk, lane);
+            for (int kk = 0; kk < NKH; kk++) ahf[tt][kk] = load_ah<S>(T.rng8, g, tt, kk, lane, acentre);
     } else if constexpr (G::PRELOAD_A) {
 #pragma unroll
         for (int tt = 0; tt < 2; tt++)
@@ -996,6 +1120,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         stage3 += keep ? 1 : 0;
         // Rare: serialise the updates of the wave's best records.
         unsigned long long m = __ballot(win);
+        if (m) dirty = true; // this wave lowered a threshold: its own limits are stale now
         while (m) {
             const int l = __builtin_ctzll(m);
             m &= m - 1;
@@ -1041,20 +1166,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // (for sizes whose A fragments live in registers) the B fragments one
     // tile ahead, so neither load's latency sits in front of the MFMAs.
     struct Frag {
-        uint4 pin; // {window byte offset, S1, sqrt(Di) bits, position index or ~0}
+        uint4 pin; // {window byte offset, S1, position index or ~0, sqrt(Di) bits}
         long l8[NK], h8[NK];
         v4i l16[NK], h16[NK];
         v8h wh[NKH]; // binary16 sizes
+        v8h wb;      // binary16 sizes: the bias K-step's B operand (zero in lanes 32..63)
     };
     // The column (lane % 32) is recomputed at every use: kept live across the
     // tile loop it is the value the register allocator spills, and the
     // reload's wait would also wait for the prefetches issued before it.
-    auto load_pin = [&](uint32_t tile) { return ld_u4(T.posinfo, tile * sample * 32 + lane_col()); };
-    auto fetch_b = [&](const uint4 pin, Frag &f) {
+    // Binary16 sizes read only {offset, S1, p}: a loaded word the tile never
+    // uses would be dead at once, and the register allocator would reuse its
+    // register while the load is in flight, waiting for the prefetch.
+    auto load_pin = [&](uint32_t tile) {
+        if constexpr (F16) {
+            const uint3 w = ld_u3(T.posinfo, tile * sample * 32 + lane_col());
+            return make_uint4(w.x, w.y, w.z, 0u);
+        } else {
+            return ld_u4(T.posinfo, tile * sample * 32 + lane_col());
+        }
+    };
+    auto fetch_b = [&](const uint4 pin, const uint32_t tile, Frag &f) {
         f.pin = pin;
         if constexpr (F16) {
 #pragma unroll
             for (int kk = 0; kk < NKH; kk++) f.wh[kk] = load_bh<S>(T.wf, pin.x, T.pitch, kk, h);
+            // lane half h: record half h = k 8h..8h+7, one load straight into
+            // the operand (a vector assembled from a load and constants
+            // would wait for the load where it is assembled)
+            f.wb = __builtin_bit_cast(v8h, T.posbias[(size_t)(tile * sample * 32 + lane_col()) * 2 + h]);
         } else if constexpr (G::PRELOAD_A) {
 #pragma unroll
             for (int kk = 0; kk < NK; kk++) {
@@ -1066,7 +1206,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     Frag nxt = {};
     uint4 pin_ahead = make_uint4(0, 0, 0, 0);
     const bool has_tiles = t_begin + wave < t_end;
-    if (has_tiles) fetch_b(load_pin(t_begin + wave), nxt);
+    if (has_tiles) fetch_b(load_pin(t_begin + wave), t_begin + wave, nxt);
     if (t_begin + wave + 4 < t_end) pin_ahead = load_pin(t_begin + wave + 4);
     // Sizes >= 16: window fragments in flight PD K-steps ahead, across tiles.
     constexpr int PD = S == 16 ? 4 : NK < 8 ? NK : 8;
@@ -1097,20 +1237,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         }
         // Whether any threshold dropped since the limits were computed
         // (read before the MFMAs so the LDS latency overlaps them).
-        const uint32_t ep = __builtin_amdgcn_readfirstlane(lds_load(&s_epoch));
+        // (Binary16 sizes read the epoch during the previous tile, hiding
+        // the LDS latency; this wave's own drops set `dirty` at once.)
+        const uint32_t ep = F16 ? ep_pre : __builtin_amdgcn_readfirstlane(lds_load(&s_epoch));
         // (cur: this tile; nxt: filled for tile + 4).  Sizes with register
         // A fragments issue these loads after the first MFMAs: every load in
         // flight at an MFMA is then one the MFMA needs, so the compiler's
         // in-order wait count cannot make it wait for the prefetch too.
         auto prefetch = [&]() __attribute__((always_inline)) {
-            if (tile + 4 < t_end) fetch_b(pin_ahead, nxt);
+            if (tile + 4 < t_end) fetch_b(pin_ahead, tile + 4, nxt);
             if (tile + 8 < t_end) pin_ahead = load_pin(tile + 8);
         };
         if constexpr (!G::PRELOAD_A) prefetch();
-        const uint32_t p = cur.pin.w; // position index, ~0 for padding
+        const uint32_t p = cur.pin.z; // position index, ~0 for padding
         const uint32_t boff = cur.pin.x;
         const int32_t S1p = (int32_t)cur.pin.y;
-        const float sqDi = __uint_as_float(cur.pin.z);
+        const float sqDi = __uint_as_float(cur.pin.w);
         // INT_MIN, or INT_MAX for a padding slot
         const int32_t pmin = p < T.P ? (int32_t)0x80000000 : 0x7fffffff;
 
@@ -1118,10 +1260,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // since the last time (a scalar branch).  Thresholds only drop, so
         // one read a moment stale is a looser, still valid, limit.
         {
-            if (ep != seen) {
+            if (ep != seen || dirty) {
                 seen = ep;
+                dirty = false;
+                if constexpr (F16) abias = make_abias(lds_load(&s_thr[brl]));
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
+                for (int i = 0; i < (F16 ? 0 : 4); i++) {
                     const float thr = __uint_as_float(lds_load(&s_thr[4 * h + i]));
                     float amt;
                     if constexpr (INT_LIMIT) {
@@ -1150,14 +1294,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // (a + U^2 Di) / (2U) = sqrt(a Di) + (sqrt(a) - U sqrt(Di))^2 / (2U).
         // nt is that bound from sa <= sqrt(a) and sqDi <= sqrt(Di); it rises
         // with both, and the (1 - 2e-5) in sa covers the rounding here.
-        int32_t lim[4];
+        int32_t lim[4] = {0, 0, 0, 0};
         const int32_t o128 = (int32_t)((uint32_t)S1p << 7);
         // what the limit adds besides noff: -128 S1 (split planes), or 2^28
         // at a padding position (binary16)
         const int32_t padj = F16 ? (p < T.P ? 0 : (1 << 28)) : -o128;
         const float usq = CAP_LIMIT ? c.cap_u * sqDi : 0.0f;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < (F16 ? 0 : 4); i++) {
             const float sab = INT_LIMIT ? sa[i] : fabsf(sa[i]);
             float nt = sab * sqDi; // <= sqrt((A - n tau) Di) when pruning
             if constexpr (CAP_LIMIT) {
@@ -1194,11 +1338,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // int32 -- 4 hi + lo, or for binary16 the bits of t1 (non-negative
         // floats order as their bit patterns); lv: the limits in the same
         // form.
-        auto epilogue16 = [&](const int32_t (&v)[16], const int32_t (&lv)[4], const int tt) {
+        // v: the row tile's values, lv: the range's limits in the same form
+        // (binary16: X0 = t1 - m S1 and max_j nb_j, binary32); t1of(t, i,
+        // m) gives a survivor's exact t1.
+        auto epilogue16 = [&](const auto (&v)[16], const auto (&lv)[4], const int tt,
+                              auto &&t1of) {
             unsigned long long mr[4], any = 0;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int32_t mx = max(max(max(v[i], v[i + 4]), v[i + 8]), v[i + 12]);
+                const auto mx = vmax(vmax(vmax(v[i], v[i + 4]), v[i + 8]), v[i + 12]);
                 mr[i] = __ballot(mx > lv[i]);
                 any |= mr[i];
             }
@@ -1206,6 +1354,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 if (!mr[i]) continue;
+                // binary16: the range's centre m (t1 = X0 + m S1)
+                const int32_t mc = F16 ? (int32_t)((s_T0[4 * h + i] + n / 2) >> LOG2N) : 0;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const int t = 4 * j + i;
@@ -1215,7 +1365,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                         const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
                             (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
                         if (sv) {
-                            const int32_t t1 = F16 ? (int32_t)__int_as_float(v[t]) : v[t] + off_of(i) + o128;
+                            const int32_t t1 = t1of(t, i, mc);
                             q[qn + pre] = make_uint2((uint32_t)t1 | ((uint32_t)(4 * h + i) << 28),
                                                      p * 8 + 4 * tt + j);
                         }
@@ -1238,7 +1388,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                 int32_t v[16];
 #pragma unroll
                 for (int t = 0; t < 16; t++) v[t] = 4 * ah[t] + al[t];
-                epilogue16(v, lim, tt);
+                epilogue16(v, lim, tt, [&](int t, int i, int32_t) { return v[t] + off_of(i) + o128; });
                 return;
             }
 #pragma unroll
@@ -1288,23 +1438,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         };
 
         if constexpr (F16) {
-            // As below, one binary16 instruction per K-step; the limits as
-            // binary32 bits (exact below 2^24; a negative limit stays below
-            // every t1 and one of 2^24 or more above, as integers).
-            int32_t lb[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) lb[i] = __float_as_int((float)lim[i]);
-#pragma unroll
-            for (int tt = 0; tt < 2; tt++) {
-                v16f acc = (v16f){};
+            // The bias K-step gives, per (range, position), the four
+            // right-hand sides nb_j (items i + 4j of the lane's range i), the
+            // same for both row tiles: one instruction per tile.  Then, row
+            // tile by row tile, one binary16 instruction per K-step gives X0
+            // = t1 - m S1 exactly, and a candidate passes stage 1 when X0 >
+            // max_j nb_j, tested on the maximum over isometries.  (Issuing
+            // row tile 1's chain before row tile 0's epilogue measured
+            // slower: its accumulators stay live across the queue drain.)
+            auto chain = [&](int tt) __attribute__((always_inline)) {
+                v16f z = (v16f){};
 #pragma unroll
                 for (int kk = 0; kk < NKH; kk++)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahf[tt][kk], cur.wh[kk], acc, 0, 0, 0);
-                if (tt == 0) prefetch();
-                int32_t v[16];
+                    z = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahf[tt][kk], cur.wh[kk], z, 0, 0, 0);
+                return z;
+            };
+            const v16f nbt = __builtin_amdgcn_mfma_f32_32x32x16_f16(abias, cur.wb, (v16f){}, 0, 0, 0);
+            const v16f acc0 = chain(0);
+            prefetch();
+            // the epoch for the next tile, read now so its latency is hidden
+            ep_pre = __builtin_amdgcn_readfirstlane(lds_load(&s_epoch));
+            float nb[4];
 #pragma unroll
-                for (int t = 0; t < 16; t++) v[t] = __float_as_int(acc[t]);
-                epilogue16(v, lb, tt);
+            for (int i = 0; i < 4; i++) nb[i] = vmax(vmax(vmax(nbt[i], nbt[i + 4]), nbt[i + 8]), nbt[i + 12]);
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++) {
+                const v16f acc = tt == 0 ? acc0 : chain(1);
+                float v[16];
+#pragma unroll
+                for (int t = 0; t < 16; t++) v[t] = acc[t];
+                epilogue16(v, nb, tt, [&](int t, int, int32_t mc) { return (int32_t)v[t] + mc * S1p; });
             }
         } else if constexpr (G::PRELOAD_A) {
             // Row tile by row tile: row tile 1's MFMAs run while row tile
