@@ -1244,9 +1244,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // A fragments issue these loads after the first MFMAs: every load in
         // flight at an MFMA is then one the MFMA needs, so the compiler's
         // in-order wait count cannot make it wait for the prefetch too.
+        // Unconditional loads (a tile past the end reads the last tile
+        // again, unused): the wait before the next tile's MFMAs then counts
+        // only the loads it needs, where loads behind a branch made it wait
+        // for every load in flight.
         auto prefetch = [&]() __attribute__((always_inline)) {
-            if (tile + 4 < t_end) fetch_b(pin_ahead, tile + 4, nxt);
-            if (tile + 8 < t_end) pin_ahead = load_pin(tile + 8);
+            fetch_b(pin_ahead, min(tile + 4, t_end - 1), nxt);
+            pin_ahead = load_pin(min(tile + 8, t_end - 1));
         };
         if constexpr (!G::PRELOAD_A) prefetch();
         const uint32_t p = cur.pin.z; // position index, ~0 for padding
@@ -1511,15 +1515,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                     acc_hi[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbh[d], acc_hi[1], 0, 0, 0);
                     acc_lo[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, pbl[d], acc_lo[0], 0, 0, 0);
                     acc_lo[1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, pbl[d], acc_lo[1], 0, 0, 0);
-                    // Refill the slot with the fragment PD steps ahead.
+                    // Refill the slot with the fragment PD steps ahead: of
+                    // this tile, of the next one, or (no next tile) a
+                    // harmless re-read of this tile's first step.  The
+                    // loads are unconditional, so the wait before each
+                    // step's MFMAs counts only the loads it needs and the
+                    // later ones stay in flight.
                     long dummy8 = 0;
-                    if (kk + PD < NK) {
-                        load_b<S>(T.lo, boff, T.pitch, kk + PD, h, &dummy8, &pbl[d]);
-                        load_b<S>(T.hi, boff, T.pitch, kk + PD, h, &dummy8, &pbh[d]);
-                    } else if (has_next) {
-                        load_b<S>(T.lo, nboff, T.pitch, kk + PD - NK, h, &dummy8, &pbl[d]);
-                        load_b<S>(T.hi, nboff, T.pitch, kk + PD - NK, h, &dummy8, &pbh[d]);
-                    }
+                    const bool here = kk + PD < NK;
+                    const uint32_t sboff = here || !has_next ? boff : nboff;
+                    const int skk = here ? kk + PD : has_next ? kk + PD - NK : 0;
+                    load_b<S>(T.lo, sboff, T.pitch, skk, h, &dummy8, &pbl[d]);
+                    load_b<S>(T.hi, sboff, T.pitch, skk, h, &dummy8, &pbh[d]);
                 }
             }
             epilogue(acc_hi[0], acc_lo[0], 0);
