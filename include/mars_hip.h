@@ -38,7 +38,8 @@ extern "C" {
 #define MARS_HIP_E_UNSUPPORTED (-5)
 
 /* Search kernels (mars_hip_set_kernel). */
-#define MARS_HIP_KERNEL_MFMA_I8 0 /* split-INT8 MFMA + exact certified epilogue (default) */
+#define MARS_HIP_KERNEL_MFMA_I8 0 /* exact MFMA correlation (binary16 at sizes 4, 8; split INT8
+                                     at 2, 16, 32) + certified epilogue (default) */
 #define MARS_HIP_KERNEL_FP32 1    /* plain FP32-FMA correlation + binary32 fit (baseline) */
 
 /* Candidate retrieval methods (crates/mars-search/src/lib.rs:357-410 MethodName). */
@@ -207,7 +208,7 @@ int mars_hip_set_walk_tables(mars_hip_ctx *ctx, double t_rms);
  * reference Candidate (8-byte aligned), d_found[i] = 1, or 0 with a zeroed
  * record when no domain position is legal.  Same results as
  * mars_hip_prepare with min_size == max_size and p->shift as the one stride,
- * by the split-INT8 MFMA search (whatever mars_hip_set_kernel chose).
+ * by the exact MFMA search (whatever mars_hip_set_kernel chose).
  *
  * Enqueues its work on `stream` (a hipStream_t; null: the context's stream)
  * and returns; the results are there once the stream reaches that point, and

@@ -2,23 +2,31 @@
 //
 // Data flow for one plane (all device resident):
 //   k_contract   u8 plane -> contracted D (i32, the 2x2 box sums of
-//                encode.rs:473-501) + the split planes D = 256*Dh + Dl, stored
-//                as i8 (Dl-128, Dh) in 16 byte-shifted copies so that any
-//                4/8/16-byte window row segment is one aligned load.
+//                encode.rs:473-501) + the split planes Dl = D % 4 and
+//                Dh' = D/4 - 128 (i8) in 16 byte-shifted copies, so that any
+//                4/8/16-byte window row segment is one aligned load, and D as
+//                binary16 in the same layout when a table has size 4 or 8.
+//   k_sat_rows, k_sat_cols  summed-area tables of D and D^2.
 //   k_possums    per (size, shift) table: per domain position the window sums
 //                S1 = sum D, S2 = sum D^2 (moments.rs:21-44) and
-//                Di = s0*S2 - S1^2.
+//                Di = s0*S2 - S1^2; the search-order record; at sizes 4 and
+//                8 the position's factors of the stage-1 bound.
 //   k_ranges     per table: each range block permuted under the 8 isometries
 //                (encode.rs:833-842) as i8 r-128 (MFMA operand) and as f32
 //                (FP32 baseline operand), plus t0 = sum r, t2 = sum r^2.
+//   k_seed       a starting threshold per range from a few exact fits.
 //   k_search_mfma<S>  range x position x isometry correlation as a GEMM
 //                [8 ranges x 8 isometries] x [32 positions] per wave tile,
-//                K = S*S, on v_mfma_i32_32x32x{16,32}_i8 with hi/lo split of
-//                D, then the certified epilogue described at the kernel.
+//                K = S*S, exact: v_mfma_f32_32x32x16_f16 on the centred range
+//                and D at sizes 4 and 8 (with one more K-step giving the
+//                stage-1 limits), v_mfma_i32_32x32x{16,32}_i8 on the split
+//                planes at 2, 16 and 32; then the certified epilogue
+//                described at the kernel.
 //   k_search_fp32<S>  the plain baseline: FP32 FMA correlation + binary32
 //                fit (encode.rs:295-337) for every candidate.
-//   k_merge      per range: min over position chunks by (rms, key), the
-//                winner's moments and (qalfa, qbeta) recomputed exactly.
+//   k_merge_exact, k_merge  per range: min over position chunks by (rms,
+//                key), the winner's moments and (qalfa, qbeta) recomputed
+//                exactly.
 //
 // Every parity-bearing floating-point expression is compiled with
 // -ffp-contract=off (and `#pragma clang fp contract(off)` below) so the
@@ -778,18 +786,20 @@ template <int S> struct Queue {
     static constexpr int CAP = 64 + 64 * DRAIN;
 };
 
-// Certified split-INT8 search for one range size S.
+// Certified exhaustive search for one range size S.
 //
 // Work item (table, position chunk, group of 8 ranges); block = 4 waves,
 // each wave takes the chunk's 32-position tiles round robin.  A wave tile is
 // 64 GEMM rows (8 ranges x 8 isometries, two 32-row MFMA tiles) x 32
-// positions.  Output layout of v_mfma_i32_32x32: item t of lane l is
+// positions.  Output layout of the 32x32 MFMAs: item t of lane l is
 // row 8*(t/4) + 4*(l/32) + t%4, column l%32; rows are arranged so that lane
 // half h owns ranges 4h..4h+3 and item t of row tile tt is range t%4,
 // isometry 4*tt + t/4.
 //
-// Correlation.  r' = r - 128 (i8), D = 4*(Dh' + 128) + Dl with Dh' = D/4 - 128
-// and Dl = D%4 (both i8):
+// Correlation.  Sizes 4 and 8: binary16 operands r - m (m = round(t0/n))
+// and D, binary32 accumulation of integers below 2^24: X0 = t1 - m*S1
+// exactly (Geo<S>::F16).  Sizes 2, 16, 32: r' = r - 128 (i8),
+// D = 4*(Dh' + 128) + Dl with Dh' = D/4 - 128 and Dl = D%4 (both i8):
 //   t1_x4 = sum r*D = 4*hi + lo + 512*sum r' + 128*S1,
 //   hi = sum r'Dh',  lo = sum r'Dl  (exact).
 //
@@ -805,10 +815,14 @@ template <int S> struct Queue {
 // so with u = alfa2/4 <= U, n*SSE >= A - 2u*Ni + u^2*Di and, where
 // A - n tau > U^2 Di, the bound rises to the tangent (A - n tau + U^2 Di)/(2U)
 // (about a fifth of the plain bound's survivors at s = 4 remain).
-// The limit is computed once per (range, position) from the tile's four
-// thresholds (one LDS read issued before the MFMAs) and the position's
-// sqrt(Di), rounded so that it never excludes more than the exact bound;
-// then one integer compare per candidate.  Candidates with Ni < 0 are
+// Sizes 2, 16, 32: the limit is computed once per (range, position) from
+// the lane's cached per-range factors and the position's sqrt(Di), rounded
+// so that it never excludes more than the exact bound; then one integer
+// compare per candidate.  Sizes 4 and 8: one more binary16 K-step per tile
+// gives per (range, position) the plain bound and three tangents of the
+// capped one (the construction is described where its A operand is built);
+// one compare per (range, position) on the maximum over isometries.
+// Candidates with Ni < 0 are
 // covered too (their qalfa is 0, SSE >= A/n).  Survivors are appended to a
 // per-wave LDS queue (wave ballot + mbcnt).
 //
