@@ -1223,7 +1223,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     if (has_tiles) fetch_b(load_pin(t_begin + wave), t_begin + wave, nxt);
     if (t_begin + wave + 4 < t_end) pin_ahead = load_pin(t_begin + wave + 4);
     // Sizes >= 16: window fragments in flight PD K-steps ahead, across tiles.
-    constexpr int PD = S == 16 ? 4 : NK < 8 ? NK : 8;
+    constexpr int PD = NK < 8 ? NK : 8;
     v4i pbl[PD], pbh[PD];
     if constexpr (!G::PRELOAD_A) {
         long dummy8 = 0;
