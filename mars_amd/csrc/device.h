@@ -263,7 +263,9 @@ __device__ __forceinline__ void fit_f64(int64_t ms0, int64_t ms1_x4, int64_t ms2
 
     uint32_t max_qalfa = (1u << bits_alfa) - 1;
     uint32_t qalfa = quantise_f64(alfa / max_alfa * (double)(1u << bits_alfa), max_qalfa);
-    double alfa2 = (double)qalfa / (double)(1u << bits_alfa) * max_alfa;
+    // qalfa / 2^bits_alfa: a division by a power of two is exact, so the
+    // scaling by ldexp gives the same binary64 bits without a divide.
+    double alfa2 = ldexp((double)qalfa, -(int)bits_alfa) * max_alfa;
 
     double beta = (t0 - alfa2 * s1) / s0;
     if (alfa2 > 0.0) beta += alfa2 * 255.0;
