@@ -55,6 +55,7 @@ __device__ __forceinline__ uint16_t f16_ru_bits(float x)
 // from address space 1 in C++, so the views hold them as plain vectors and
 // words and these helpers move them.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint4 ld_u4(const GLB u32x4 *p, size_t i)
 {
     const u32x4 v = p[i];

@@ -236,11 +236,56 @@ __device__ __forceinline__ void fisher_range(const TableG &T, uint32_t r, uint32
         uint32_t p, k;
         int32_t t1, s1, s2;
     };
-    // Exact t1 of candidate c of the bucket from the split planes.
+    // Sizes 4 and 8 with the binary16 plane: the range's 8 isometries as
+    // binary16 r (exact) in LDS, so a candidate reads one load per window
+    // row (not one per split plane) and its record (S2 in the last word):
+    // the gathers' L2 requests are what bounds this kernel.
+    constexpr bool F16 = S == 4 || S == 8;
+    __shared__ __align__(16) _Float16 s_r[F16 ? 8 * K : 2];
+    const bool f16 = F16 && T.wf != nullptr;
+    if constexpr (F16) {
+        if (f16) {
+            for (uint32_t i = lane; i < 8 * K; i += 64)
+                s_r[i] = (_Float16)(float)((int32_t)rng[(i / K) * T.kmem + i % K] + 128);
+            __syncthreads();
+        }
+    }
+    // Exact t1 of candidate c of the bucket: binary16 dot products
+    // accumulated in binary32 (every partial sum an integer below
+    // 64 * 255 * 1020 < 2^24), or i8 dot products on the split planes.  The
+    // search-order record is {window offset, S1, p, (S2 at sizes 4 and 8)}.
     auto gather = [&](uint32_t c) {
         const uint32_t e = T.flist[start + c];
         const uint32_t k = fisher::kMapping[isom][e & 7];
-        const uint3 rec = ld_u3(T.posinfo, e >> 3); // {window offset, S1, p}
+        if constexpr (F16) {
+            if (f16) {
+                typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+                const uint4 rec = ld_u4(T.posinfo, e >> 3);
+                float acc = 0.0f;
+                const GLB uint16_t *w = T.wf + rec.x;
+                const h2 *rr = (const h2 *)(s_r + k * K);
+#pragma unroll
+                for (uint32_t u = 0; u < S; u++) {
+                    if constexpr (S == 4) {
+                        const u32x2 d = *(const GLB u32x2 *)(w + u * T.pitch);
+                        const uint32_t d0 = d.x, d1 = d.y;
+                        acc = __builtin_amdgcn_fdot2(rr[2 * u], __builtin_bit_cast(h2, d0), acc, false);
+                        acc = __builtin_amdgcn_fdot2(rr[2 * u + 1], __builtin_bit_cast(h2, d1), acc, false);
+                    } else {
+                        const u32x4 d = *(const GLB u32x4 *)(w + u * T.pitch);
+                        // A vector element goes through a scalar: bit_cast of the
+                        // element lvalue itself reads element 0 with this hipcc.
+#pragma unroll
+                        for (uint32_t q = 0; q < 4; q++) {
+                            const uint32_t dq = d[q];
+                            acc = __builtin_amdgcn_fdot2(rr[4 * u + q], __builtin_bit_cast(h2, dq), acc, false);
+                        }
+                    }
+                }
+                return Cand{rec.z, k, (int32_t)acc, (int32_t)rec.y, (int32_t)rec.w};
+            }
+        }
+        const uint3 rec = ld_u3(T.posinfo, e >> 3);
         int32_t ah = 0, al = 0;
         window_dots<S>(T.hi + rec.x, T.lo + rec.x, T.pitch, rng + (size_t)k * T.kmem, ah, al);
         const int32_t s1 = (int32_t)rec.y;

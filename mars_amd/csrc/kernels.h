@@ -59,6 +59,7 @@ struct TableDesc {
     uint32_t *gthr;      // [nr_pad] pruning threshold (float bits, SSE units)
     // Search-order records, P_pad = 32 * ceil(P / 32) slots:
     // {window byte offset, S1, p, sqrt(n*S2 - S1^2) as f32 rounded down}
+    // (sizes 4 and 8: S2 in the last word)
     // (p = ~0 in padding slots).  Slot of position p: iperm[p], or p when
     // iperm is null.
     uint4 *posinfo;

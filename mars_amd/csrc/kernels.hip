@@ -46,7 +46,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------- pool
 
@@ -230,7 +229,10 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
     const uint32_t cp = dc & cmask;
     const uint32_t boff = cp * T.copy_stride + dr * T.pitch + (dc - cp);
     const uint32_t q = T.iperm ? T.iperm[p] : p; // slot in search order
-    T.posinfo[q] = (u32x4){boff, (uint32_t)s1, p, __float_as_uint(sq)};
+    // The last word: sqrt(Di) for the search's limits at sizes 2, 16, 32;
+    // S2 at sizes 4 and 8, whose search does not read it (their limits come
+    // from posbias) and whose Fisher gather then needs one record per candidate.
+    T.posinfo[q] = (u32x4){boff, (uint32_t)s1, p, T.posbias ? (uint32_t)s2 : __float_as_uint(sq)};
     if (T.posbias) {
         // Binary16 sizes: the position's factors of the stage-1 bound (the
         // B operand of the bias K-step of k_search_mfma, 32 bytes).  Every value is
