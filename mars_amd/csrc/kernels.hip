@@ -890,6 +890,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // (rms, key), and how many are filled.
     __shared__ TopRec s_top[RANKED ? 4 : 1][RANKED ? 8 : 1][RANKED ? KCAP : 1];
     __shared__ uint32_t s_topn[4][8];
+    // Winner-only: per wave and range, the least high word of a drain's
+    // winning binary64 rms (a non-negative double orders as its bits).
+    __shared__ uint32_t s_hmin[RANKED ? 1 : 4][8];
     // Range operand of the block (sizes >= 16): 64 rows of KMEM bytes,
     // padded rows so the lanes of a fragment read spread over the banks.
     constexpr int ASTRIDE = G::KMEM + 16;
@@ -1134,9 +1137,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
             }
         }
         stage3 += keep ? 1 : 0;
-        // Rare: serialise the updates of the wave's best records.
         unsigned long long m = __ballot(win);
         if (m) dirty = true; // this wave lowered a threshold: its own limits are stale now
+        if constexpr (!RANKED) {
+            // Several lanes of one range often beat the wave's best of
+            // before the drain, and each would take a trip of the serial
+            // loop below.  Only those whose rms has the least high word
+            // among the range's winners can be its new best: the others
+            // drop here (one LDS min and one read, in the wave's order).
+            if (m) {
+                if ((uint32_t)lane < 8) s_hmin[wave][lane] = 0xffffffffu;
+                const uint32_t hw = (uint32_t)((uint64_t)__double_as_longlong(rms) >> 32);
+                if (win) {
+                    atomicMin(&s_hmin[wave][slot], hw);
+                    win = lds_load(&s_hmin[wave][slot]) == hw;
+                }
+                m = __ballot(win);
+            }
+        }
+        // Serialise the updates of the wave's best records.
         while (m) {
             const int l = __builtin_ctzll(m);
             m &= m - 1;
