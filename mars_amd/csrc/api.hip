@@ -121,6 +121,9 @@ uint32_t prepass_stride(uint32_t s) { return s; }
 
 // Sizes whose search correlates in binary16 (the kernel's Geo<S>::F16).
 bool uses_f16(uint32_t s) { return s == 4 || s == 8; }
+// Positions of one table: the kernels address its 16-byte search-order
+// records (two per slot at sizes 4 and 8) with 32-bit byte offsets.
+constexpr uint32_t kMaxPositions = (1u << 27) - 32;
 // The binary16 plane: copies 0..7 of the split planes' layout (all that
 // sizes 4 and 8 address), 2-byte elements.
 size_t f16_plane_bytes(size_t copy_stride) { return 8 * copy_stride * sizeof(uint16_t); }
@@ -989,7 +992,7 @@ int prepare_batch(mars_hip_ctx *ctx, const mars_hip_plane *in, uint32_t n_planes
                     t.npc = (pl.w - 2 * s) / shift + 1;
                 }
                 t.P = t.npr * t.npc;
-                if (t.P > (0xffffffffu / 8)) return fail(ctx, MARS_HIP_E_UNSUPPORTED, "too many positions");
+                if (t.P > kMaxPositions) return fail(ctx, MARS_HIP_E_UNSUPPORTED, "too many positions");
                 t.P_pad = (t.P + 31) / 32 * 32;
                 t.fisher = p->method == MARS_HIP_METHOD_FISHER;
                 t.topk = ctx->topk;
@@ -1600,7 +1603,7 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
         npc = (w - 2 * s) / shift + 1;
     }
     const uint64_t P64 = (uint64_t)npr * npc;
-    if (P64 > 0xffffffffu / 8) return fail(ctx, MARS_HIP_E_UNSUPPORTED, "too many positions");
+    if (P64 > kMaxPositions) return fail(ctx, MARS_HIP_E_UNSUPPORTED, "too many positions");
     const uint32_t P = (uint32_t)P64, P_pad = (P + 31) / 32 * 32;
     const uint32_t wc = w / 2, hc = h / 2, pitch = ((wc + 15) / 16) * 16 + 16;
     const size_t copy_stride = (size_t)pitch * std::max<uint32_t>(hc, 1);

@@ -84,6 +84,14 @@ __device__ __forceinline__ uint32_t lane_col()
 // The first three words of a record: a 12-byte load, so no destination
 // register of an in-flight load is dead (the compiler would reuse it and
 // wait for the load).
+// Record i of an array of 16-byte records as the array's base plus a 32-bit
+// byte offset, so a load takes the base in SGPRs and one VGPR offset (no
+// 64-bit address arithmetic per lane).  Callers keep 16 i below 2^32: a
+// table holds at most 2^27 - 32 positions, two records each at most.
+__device__ __forceinline__ const GLB u32x4 *rec16(const GLB u32x4 *p, uint32_t i)
+{
+    return (const GLB u32x4 *)((const GLB char *)p + 16u * i);
+}
 __device__ __forceinline__ uint3 ld_u3(const GLB u32x4 *p, size_t i)
 {
     typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));

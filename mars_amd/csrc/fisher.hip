@@ -253,7 +253,8 @@ __device__ __forceinline__ void fisher_range(const TableG &T, uint32_t r, uint32
     // Exact t1 of candidate c of the bucket: binary16 dot products
     // accumulated in binary32 (every partial sum an integer below
     // 64 * 255 * 1020 < 2^24), or i8 dot products on the split planes.  The
-    // search-order record is {window offset, S1, p, (S2 at sizes 4 and 8)}.
+    // search-order record is {window offset, S1, p, -}, at sizes 4 and 8
+    // {window offset, p, S1, S2}.
     auto gather = [&](uint32_t c) {
         const uint32_t e = T.flist[start + c];
         const uint32_t k = fisher::kMapping[isom][e & 7];
@@ -282,7 +283,7 @@ __device__ __forceinline__ void fisher_range(const TableG &T, uint32_t r, uint32
                         }
                     }
                 }
-                return Cand{rec.z, k, (int32_t)acc, (int32_t)rec.y, (int32_t)rec.w};
+                return Cand{rec.y, k, (int32_t)acc, (int32_t)rec.z, (int32_t)rec.w};
             }
         }
         const uint3 rec = ld_u3(T.posinfo, e >> 3);

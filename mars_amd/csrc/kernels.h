@@ -58,8 +58,8 @@ struct TableDesc {
     int32_t *rpos;       // [nr_pad] 3 * sum max(r - 128, 0)
     uint32_t *gthr;      // [nr_pad] pruning threshold (float bits, SSE units)
     // Search-order records, P_pad = 32 * ceil(P / 32) slots:
-    // {window byte offset, S1, p, sqrt(n*S2 - S1^2) as f32 rounded down}
-    // (sizes 4 and 8: S2 in the last word)
+    // {window byte offset, S1, p, sqrt(n*S2 - S1^2) as f32 rounded down},
+    // at sizes 4 and 8 {window byte offset, p, S1, S2}
     // (p = ~0 in padding slots).  Slot of position p: iperm[p], or p when
     // iperm is null.
     uint4 *posinfo;

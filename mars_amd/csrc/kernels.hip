@@ -206,7 +206,8 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
     if (p >= T.P) { // padding slot of the last search tile: no position
         // S1 = -2^21: the MFMA search's stage-1 limit excludes the slot
         // (binary16 sizes: the 2^12 in k 9 of its bias record).
-        T.posinfo[p] = (u32x4){0u, (uint32_t)-(1 << 21), 0xffffffffu, 0u};
+        T.posinfo[p] = T.posbias ? (u32x4){0u, 0xffffffffu, (uint32_t)-(1 << 21), 0u}
+                                 : (u32x4){0u, (uint32_t)-(1 << 21), 0xffffffffu, 0u};
         if (T.posbias) {
             T.posbias[2 * p] = (u32x4){0u, 0u, 0u, 0u};
             T.posbias[2 * p + 1] = (u32x4){0x6c000000u, 0u, 0u, 0u}; // k 9 = 2^12
@@ -229,10 +230,11 @@ __global__ void k_possums(const TableDesc *__restrict__ tabs, const Work2 *__res
     const uint32_t cp = dc & cmask;
     const uint32_t boff = cp * T.copy_stride + dr * T.pitch + (dc - cp);
     const uint32_t q = T.iperm ? T.iperm[p] : p; // slot in search order
-    // The last word: sqrt(Di) for the search's limits at sizes 2, 16, 32;
-    // S2 at sizes 4 and 8, whose search does not read it (their limits come
-    // from posbias) and whose Fisher gather then needs one record per candidate.
-    T.posinfo[q] = (u32x4){boff, (uint32_t)s1, p, T.posbias ? (uint32_t)s2 : __float_as_uint(sq)};
+    // Sizes 2, 16, 32: {offset, S1, p, sqrt(Di)} for the search's limits.
+    // Sizes 4 and 8 (limits from posbias): {offset, p, S1, S2}, the search
+    // reading the first two words and the Fisher gather all four.
+    T.posinfo[q] = T.posbias ? (u32x4){boff, p, (uint32_t)s1, (uint32_t)s2}
+                             : (u32x4){boff, (uint32_t)s1, p, __float_as_uint(sq)};
     if (T.posbias) {
         // Binary16 sizes: the position's factors of the stage-1 bound (the
         // B operand of the bias K-step of k_search_mfma, 32 bytes).  Every value is
@@ -1113,14 +1115,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     auto process = [&](uint32_t b0, uint32_t cnt) {
         const bool act = (uint32_t)lane < cnt;
         const uint2 e = q[b0 + (act ? lane : 0)];
-        const int32_t t1 = (int32_t)(e.x & 0x0fffffffu);
-        const uint32_t slot = e.x >> 28, key = e.y, p = key >> 3;
+        // The entry: {t1 | range << 28, position * 8 + isometry}; at the
+        // binary16 sizes {X0 + 2^24 | range << 28, search slot * 8 +
+        // isometry} (X0 = t1 - m S1, |X0| < 2^24), completed below from the
+        // slot's record {offset, p, S1, S2} and the range's centre m.
+        int32_t t1 = (int32_t)(e.x & 0x0fffffffu);
+        const uint32_t slot = e.x >> 28;
+        uint32_t key = e.y, p = key >> 3;
         bool keep = false, win = false;
         double rms = 0.0, sum = 0.0;
         if (act) {
-            const int32_t S1 = T.posx[2 * p], S2 = T.posx[2 * p + 1];
+            int32_t S1, S2;
+            if constexpr (F16) {
+                const uint4 rec = ld_u4(rec16(T.posinfo, p), 0);
+                p = rec.y;
+                key = p * 8 + (key & 7);
+                S1 = (int32_t)rec.z;
+                S2 = (int32_t)rec.w;
+            } else {
+                S1 = T.posx[2 * p];
+                S2 = T.posx[2 * p + 1];
+            }
             const double Di = T.posd[2 * p], invDi = T.posd[2 * p + 1];
             const int64_t t0 = s_T0[slot], t2 = s_T2[slot];
+            if constexpr (F16) t1 += (int32_t)((t0 + n / 2) >> LOG2N) * S1 - (1 << 24);
             keep = !c.stage2 || stage2_keep(c, nd, inv_n, t1, S1, t0, Di, invDi, s_A[slot],
                                             __uint_as_float(s_thr[slot]));
             if (keep) {
@@ -1201,7 +1219,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // (for sizes whose A fragments live in registers) the B fragments one
     // tile ahead, so neither load's latency sits in front of the MFMAs.
     struct Frag {
-        uint4 pin; // {window byte offset, S1, position index or ~0, sqrt(Di) bits}
+        uint4 pin; // {window byte offset, S1, position index or ~0, sqrt(Di) bits}; binary16: {offset}
         long l8[NK], h8[NK];
         v4i l16[NK], h16[NK];
         v8h wh[NKH]; // binary16 sizes
@@ -1210,18 +1228,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // The column (lane % 32) is recomputed at every use: kept live across the
     // tile loop it is the value the register allocator spills, and the
     // reload's wait would also wait for the prefetches issued before it.
-    // Binary16 sizes read only {offset, S1, p}: a loaded word the tile never
-    // uses would be dead at once, and the register allocator would reuse its
-    // register while the load is in flight, waiting for the prefetch.
-    auto load_pin = [&](uint32_t tile) {
+    // Binary16 sizes read only the offset (their queue entries name the
+    // search slot, not the position): a loaded word the tile never uses
+    // would be dead at once, and the register allocator would reuse its
+    // register while the load is in flight, waiting for the prefetch; and
+    // with one word, the record of tile + 8 can load into the register of
+    // tile + 4's once that one's fragment loads are issued, where a record
+    // also read in its own tile would rotate through three registers in a
+    // loop unrolled by two, a copy (and a wait for the load) per trip.
+    // lc: lane_col(), computed once for the loads of one prefetch.
+    auto load_pin = [&](uint32_t tile, uint32_t lc) {
+        const GLB u32x4 *rec = rec16(T.posinfo, tile * sample * 32 + lc);
         if constexpr (F16) {
-            const uint3 w = ld_u3(T.posinfo, tile * sample * 32 + lane_col());
-            return make_uint4(w.x, w.y, w.z, 0u);
+            return make_uint4(*(const GLB uint32_t *)rec, 0u, 0u, 0u); // the offset
         } else {
-            return ld_u4(T.posinfo, tile * sample * 32 + lane_col());
+            return ld_u4(rec, 0);
         }
     };
-    auto fetch_b = [&](const uint4 pin, const uint32_t tile, Frag &f) {
+    auto fetch_b = [&](const uint4 pin, const uint32_t tile, uint32_t lc, Frag &f) {
         f.pin = pin;
         if constexpr (F16) {
 #pragma unroll
@@ -1229,7 +1253,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
             // lane half h: record half h = k 8h..8h+7, one load straight into
             // the operand (a vector assembled from a load and constants
             // would wait for the load where it is assembled)
-            f.wb = __builtin_bit_cast(v8h, T.posbias[(size_t)(tile * sample * 32 + lane_col()) * 2 + h]);
+            f.wb = __builtin_bit_cast(v8h, *rec16(T.posbias, (tile * sample * 32 + lc) * 2 + h));
         } else if constexpr (G::PRELOAD_A) {
 #pragma unroll
             for (int kk = 0; kk < NK; kk++) {
@@ -1241,8 +1265,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     Frag nxt = {};
     uint4 pin_ahead = make_uint4(0, 0, 0, 0);
     const bool has_tiles = t_begin + wave < t_end;
-    if (has_tiles) fetch_b(load_pin(t_begin + wave), t_begin + wave, nxt);
-    if (t_begin + wave + 4 < t_end) pin_ahead = load_pin(t_begin + wave + 4);
+    if (has_tiles) fetch_b(load_pin(t_begin + wave, lane_col()), t_begin + wave, lane_col(), nxt);
+    if (t_begin + wave + 4 < t_end) pin_ahead = load_pin(t_begin + wave + 4, lane_col());
     // Sizes >= 16: window fragments in flight PD K-steps ahead, across tiles.
     constexpr int PD = NK < 8 ? NK : 8;
     v4i pbl[PD], pbh[PD];
@@ -1284,8 +1308,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // only the loads it needs, where loads behind a branch made it wait
         // for every load in flight.
         auto prefetch = [&]() __attribute__((always_inline)) {
-            fetch_b(pin_ahead, min(tile + 4, t_end - 1), nxt);
-            pin_ahead = load_pin(min(tile + 8, t_end - 1));
+            const uint32_t lc = lane_col();
+            fetch_b(pin_ahead, min(tile + 4, t_end - 1), lc, nxt);
+            pin_ahead = load_pin(min(tile + 8, t_end - 1), lc);
         };
         if constexpr (!G::PRELOAD_A) prefetch();
         const uint32_t p = cur.pin.z; // position index, ~0 for padding
@@ -1373,15 +1398,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // tested by their maximum (a v_max3 and a v_max); only a range some
         // lane passes is tested value by value.  Against a compare per value
         // this also drops most of the scalar mask ORs and per-value branches
-        // (s = 4 launch 0.367 -> 0.330 ms at 512 x 512).  v: the 16 values as
-        // int32 -- 4 hi + lo, or for binary16 the bits of t1 (non-negative
-        // floats order as their bit patterns); lv: the limits in the same
-        // form.
+        // (s = 4 launch 0.367 -> 0.330 ms at 512 x 512).
         // v: the row tile's values, lv: the range's limits in the same form
-        // (binary16: X0 = t1 - m S1 and max_j nb_j, binary32); t1of(t, i,
-        // m) gives a survivor's exact t1.
+        // (int32 4 hi + lo and its limit; binary16: X0 = t1 - m S1 and
+        // max_j nb_j, binary32); t1of(t, i) gives a survivor's queue word:
+        // its exact t1, or X0 + 2^24 at the binary16 sizes; key0() the key
+        // of the lane's column without the isometry (position or, binary16,
+        // search slot times 8).
         auto epilogue16 = [&](const auto (&v)[16], const auto (&lv)[4], const int tt,
-                              auto &&t1of) {
+                              auto &&t1of, auto &&key0) {
             unsigned long long mr[4], any = 0;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -1393,8 +1418,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 if (!mr[i]) continue;
-                // binary16: the range's centre m (t1 = X0 + m S1)
-                const int32_t mc = F16 ? (int32_t)((s_T0[4 * h + i] + n / 2) >> LOG2N) : 0;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const int t = 4 * j + i;
@@ -1404,9 +1427,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                         const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
                             (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
                         if (sv) {
-                            const int32_t t1 = t1of(t, i, mc);
-                            q[qn + pre] = make_uint2((uint32_t)t1 | ((uint32_t)(4 * h + i) << 28),
-                                                     p * 8 + 4 * tt + j);
+                            q[qn + pre] = make_uint2((uint32_t)t1of(t, i) | ((uint32_t)(4 * h + i) << 28),
+                                                     key0() + 4 * tt + j);
                         }
                         qn += (uint32_t)__popcll(m);
                         qtotal += (uint32_t)__popcll(m);
@@ -1427,7 +1449,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                 int32_t v[16];
 #pragma unroll
                 for (int t = 0; t < 16; t++) v[t] = 4 * ah[t] + al[t];
-                epilogue16(v, lim, tt, [&](int t, int i, int32_t) { return v[t] + off_of(i) + o128; });
+                epilogue16(v, lim, tt, [&](int t, int i) { return v[t] + off_of(i) + o128; },
+                           [&]() { return p * 8; });
                 return;
             }
 #pragma unroll
@@ -1506,7 +1529,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                 float v[16];
 #pragma unroll
                 for (int t = 0; t < 16; t++) v[t] = acc[t];
-                epilogue16(v, nb, tt, [&](int t, int, int32_t mc) { return (int32_t)v[t] + mc * S1p; });
+                epilogue16(
+                    v, nb, tt, [&](int t, int) { return (int32_t)v[t] + (1 << 24); },
+                    [&]() { return (tile * sample * 32 + lane_col()) * 8; });
             }
         } else if constexpr (G::PRELOAD_A) {
             // Row tile by row tile: row tile 1's MFMAs run while row tile
