@@ -1298,7 +1298,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // (read before the MFMAs so the LDS latency overlaps them).
         // (Binary16 sizes read the epoch during the previous tile, hiding
         // the LDS latency; this wave's own drops set `dirty` at once.)
-        const uint32_t ep = F16 ? ep_pre : __builtin_amdgcn_readfirstlane(lds_load(&s_epoch));
+        const uint32_t ep = __builtin_amdgcn_readfirstlane(F16 ? ep_pre : lds_load(&s_epoch));
         // (cur: this tile; nxt: filled for tile + 4).  Sizes with register
         // A fragments issue these loads after the first MFMAs: every load in
         // flight at an MFMA is then one the MFMA needs, so the compiler's
@@ -1415,6 +1415,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                 any |= mr[i];
             }
             if (any == 0) return;
+            const uint32_t k0 = key0();
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 if (!mr[i]) continue;
@@ -1427,8 +1428,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                         const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
                             (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
                         if (sv) {
-                            q[qn + pre] = make_uint2((uint32_t)t1of(t, i) | ((uint32_t)(4 * h + i) << 28),
-                                                     key0() + 4 * tt + j);
+                            // (the word's low bits are below 2^28: + is |)
+                            q[qn + pre] = make_uint2((uint32_t)t1of(t, i) + ((uint32_t)(4 * h + i) << 28),
+                                                     k0 + 4 * tt + j);
                         }
                         qn += (uint32_t)__popcll(m);
                         qtotal += (uint32_t)__popcll(m);
@@ -1519,7 +1521,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
             const v16f acc0 = chain(0);
             prefetch();
             // the epoch for the next tile, read now so its latency is hidden
-            ep_pre = __builtin_amdgcn_readfirstlane(lds_load(&s_epoch));
+            // (kept in a VGPR: a readfirstlane here would wait for the read)
+            ep_pre = lds_load(&s_epoch);
             float nb[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) nb[i] = vmax(vmax(vmax(nbt[i], nbt[i + 4]), nbt[i + 8]), nbt[i + 12]);
