@@ -111,8 +111,9 @@ uint32_t walk_chunks(uint32_t ntiles) { return std::max<uint32_t>(1, std::min(nt
 // 2.37 M stage-1 survivors and 1.07 M exact fits (126 k and 19 k with it),
 // while at s = 16 the seed's thresholds leave 438 k survivors and the
 // search alone (0.115 ms) is quicker than pre-pass and search (0.141 ms),
-// whose 128-block pre-pass also ran last in the step.
-constexpr uint32_t kPrepassMinSize = 32;
+// whose 128-block pre-pass also ran last in the step.  (kPrepassMinSize,
+// 32, is in kernels.h: the search kernel fixes its sample stride to 1 below
+// that size.)
 // The pre-pass samples every s-th tile at size s: at 512 x 512 the sparser
 // sample (against every 8th) leaves the s = 32 survivors at 126 k (from 83 k)
 // but finishes the launch early enough for the full pass to overlap the
@@ -593,7 +594,7 @@ int layout(mars_hip_ctx *ctx)
             g.search[k].n = (uint32_t)w3.size() - g.search[k].off;
             if (k == 0) {
                 g.prepass.off = (uint32_t)w3.size();
-                if (g.size >= kPrepassMinSize && ctx->walk_prepared <= 0.0)
+                if (g.size >= mars::kPrepassMinSize && ctx->walk_prepared <= 0.0)
                     for (uint32_t ti : g.tables) {
                         const Table &t = ctx->tables[ti];
                         if (t.P == 0) continue;
@@ -1619,7 +1620,7 @@ int mars_hip_search_grid_noalloc(mars_hip_ctx *ctx, const uint8_t *d_px, uint32_
     const Shape &sh = ctx->shape[ceil_log2(s) - 1];
     uint32_t nchunks = 1, tpc = 1;
     chunking(cdiv(P, 32), ngroups, sh.target_blocks, sh.min_tiles, &nchunks, &tpc);
-    const bool prepass = s >= kPrepassMinSize && P > 0;
+    const bool prepass = s >= mars::kPrepassMinSize && P > 0;
     const uint32_t n_contract = std::max<uint32_t>(1, cdiv(nc, 256));
     const uint32_t n_sat_rows = hc + 1, n_sat_cols = cdiv(wc + 1, 16), n_possums = cdiv(P_pad, 256);
     const uint32_t n_ranges = cdiv((uint64_t)nr_pad * 8 * kmem / 4, 256);

@@ -18,6 +18,10 @@ namespace mars {
 
 // Most candidates a ranked table keeps per range (search_top_k's k).
 constexpr uint32_t kMaxRanked = 32;
+// Smallest range size whose MFMA search runs a threshold pre-pass (every
+// sample-th tile) before the full pass; smaller sizes always search with
+// sample 1.
+constexpr uint32_t kPrepassMinSize = 32;
 
 // Per-(range, position chunk) partial winner.
 struct Partial {

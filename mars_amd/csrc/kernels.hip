@@ -957,9 +957,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     float sa[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) sa[i] = INT_LIMIT ? 1.0f : -1.0f;
-    uint32_t seen = 0xffffffffu; // s_epoch the limits were computed at (none yet)
+    // s_epoch the limits were computed at; ~0 (none yet, or this wave just
+    // lowered a threshold itself) never equals the epoch, a count of drops.
+    uint32_t seen = 0xffffffffu;
     uint32_t ep_pre = 0;
-    bool dirty = false;
     // Binary16 sizes: the stage-1 limit rides on the matrix pipe.  The
     // range operand is centred, r - m with m = round(t0/n), so the K-steps
     // give the exact integer X0 = t1 - m S1 (|X0| <= 1020 |r - m|_1 < 2^24).
@@ -1098,7 +1099,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // With c.sample > 1 (threshold pre-pass) the block covers every
     // sample-th tile of all of its group's positions (whatever its chunk):
     // tile index i below stands for tile i * sample.
-    const uint32_t sample = c.sample;
+    const uint32_t sample = S >= kPrepassMinSize ? c.sample : 1u;
     const bool whole = sample > 1;
     const uint32_t ntiles = (T.P_pad / 32 + sample - 1) / sample;
     const uint32_t tpc = whole ? ntiles : tiles_per_chunk;
@@ -1113,6 +1114,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
 
     // Stage 2 + 3 over the queue entries [b0, b0 + cnt), cnt <= 64.
     auto process = [&](uint32_t b0, uint32_t cnt) {
+        qtotal += cnt; // every queued survivor is drained exactly once
         const bool act = (uint32_t)lane < cnt;
         const uint2 e = q[b0 + (act ? lane : 0)];
         // The entry: {t1 | range << 28, position * 8 + isometry}; at the
@@ -1156,7 +1158,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         }
         stage3 += keep ? 1 : 0;
         unsigned long long m = __ballot(win);
-        if (m) dirty = true; // this wave lowered a threshold: its own limits are stale now
+        if (m) seen = 0xffffffffu; // this wave lowered a threshold: its own limits are stale now
         if constexpr (!RANKED) {
             // Several lanes of one range often beat the wave's best of
             // before the drain, and each would take a trip of the serial
@@ -1297,7 +1299,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // Whether any threshold dropped since the limits were computed
         // (read before the MFMAs so the LDS latency overlaps them).
         // (Binary16 sizes read the epoch during the previous tile, hiding
-        // the LDS latency; this wave's own drops set `dirty` at once.)
+        // the LDS latency; this wave's own drops reset `seen` at once.)
         const uint32_t ep = __builtin_amdgcn_readfirstlane(F16 ? ep_pre : lds_load(&s_epoch));
         // (cur: this tile; nxt: filled for tile + 4).  Sizes with register
         // A fragments issue these loads after the first MFMAs: every load in
@@ -1324,9 +1326,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
         // since the last time (a scalar branch).  Thresholds only drop, so
         // one read a moment stale is a looser, still valid, limit.
         {
-            if (ep != seen || dirty) {
+            if (ep != seen) {
                 seen = ep;
-                dirty = false;
                 if constexpr (F16) abias = make_abias(lds_load(&s_thr[brl]));
 #pragma unroll
                 for (int i = 0; i < (F16 ? 0 : 4); i++) {
@@ -1433,7 +1434,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                                                      k0 + 4 * tt + j);
                         }
                         qn += (uint32_t)__popcll(m);
-                        qtotal += (uint32_t)__popcll(m);
                     }
                 }
             }
@@ -1485,8 +1485,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                                                          p * 8 + 4 * tt + (t >> 2));
                             }
                             qn += (uint32_t)__popcll(m);
-                            qtotal += (uint32_t)__popcll(m);
-                        }
+                            }
                     }
                 }
                 // Drain full batches; keep the tail (< 64) for later.
