@@ -431,10 +431,24 @@ __device__ __forceinline__ void load_b(const GLB int8_t *plane, uint32_t boff, u
         }
         *b8 = (long)x;
     } else {
-        static_assert(S >= 16, "sizes 4 and 8 search in binary16 (load_bh)");
-        const uint32_t k0 = 32 * kk + 16 * h;
-        *b16 = *(const GLB v4i *)(plane + boff + (k0 / S) * pitch + (k0 % S));
+        static_assert(S == 2, "sizes 4 and 8 search in binary16 (load_bh), sizes >= 16 by load_b16");
     }
+}
+
+// B fragment at sizes >= 16 (16 bytes of one split plane) of K-step kk:
+// k = 32 kk + 16 h .. + 15, row 2 kk + h at S = 16, row kk and column 16 h
+// at S = 32.  lane_off = the window's offset plus the lane's part (h pitch,
+// or 16 h); the K-step's part, a multiple of the pitch, is the same for the
+// whole wave, and the two add as 32-bit offsets from the plane: one add per
+// load (a row index times the pitch had cost a v_mul_lo_u32 and 64-bit
+// address arithmetic per load).
+template <int S>
+__device__ __forceinline__ v4i load_b16(const GLB int8_t *plane, uint32_t lane_off, uint32_t pitch,
+                                        int kk)
+{
+    static_assert(S >= 16, "split-plane K-steps of 32 values");
+    const uint32_t step = (S == 16 ? 2u * (uint32_t)kk : (uint32_t)kk) * pitch;
+    return *(const GLB v4i *)(plane + (lane_off + step));
 }
 
 template <int S>
@@ -1272,15 +1286,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
     // Sizes >= 16: window fragments in flight PD K-steps ahead, across tiles.
     constexpr int PD = NK < 8 ? NK : 8;
     v4i pbl[PD], pbh[PD];
+    // The lane's part of its fragments' offsets (sizes >= 16), opaque to
+    // the compiler so that it is not folded back into a row index times
+    // the pitch per load.
+    uint32_t hoff = S == 16 ? (uint32_t)h * T.pitch : 16u * (uint32_t)h;
+    asm volatile("" : "+v"(hoff));
     if constexpr (!G::PRELOAD_A) {
-        long dummy8 = 0;
 #pragma unroll
         for (int d = 0; d < PD; d++) {
             pbl[d] = (v4i){};
             pbh[d] = (v4i){};
             if (has_tiles) {
-                load_b<S>(T.lo, nxt.pin.x, T.pitch, d, h, &dummy8, &pbl[d]);
-                load_b<S>(T.hi, nxt.pin.x, T.pitch, d, h, &dummy8, &pbh[d]);
+                pbl[d] = load_b16<S>(T.lo, nxt.pin.x + hoff, T.pitch, d);
+                pbh[d] = load_b16<S>(T.hi, nxt.pin.x + hoff, T.pitch, d);
             }
         }
     }
@@ -1583,12 +1601,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                     // loads are unconditional, so the wait before each
                     // step's MFMAs counts only the loads it needs and the
                     // later ones stay in flight.
-                    long dummy8 = 0;
                     const bool here = kk + PD < NK;
-                    const uint32_t sboff = here || !has_next ? boff : nboff;
+                    const uint32_t sboff = (here || !has_next ? boff : nboff) + hoff;
                     const int skk = here ? kk + PD : has_next ? kk + PD - NK : 0;
-                    load_b<S>(T.lo, sboff, T.pitch, skk, h, &dummy8, &pbl[d]);
-                    load_b<S>(T.hi, sboff, T.pitch, skk, h, &dummy8, &pbh[d]);
+                    pbl[d] = load_b16<S>(T.lo, sboff, T.pitch, skk);
+                    pbh[d] = load_b16<S>(T.hi, sboff, T.pitch, skk);
                 }
             }
             epilogue(acc_hi[0], acc_lo[0], 0);
