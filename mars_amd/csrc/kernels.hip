@@ -1403,11 +1403,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
             }
         }
 
-        // Epilogue of one row tile.  Sizes >= 16: one compare per value,
-        // issued 8 at a time into SGPR masks that the scalar unit ORs, so a
-        // group costs one branch; only a group where some lane passes goes
-        // value by value and appends (wave ballot + mbcnt).  The queue is
-        // drained after every DRAIN values.  Sizes <= 8: see below.
+        // Epilogue of one row tile.  Sizes >= 16: the queue is drained after
+        // every DRAIN = 4 values per lane (size 16: one range's four
+        // isometries, compared by their maximum; size 32: four ranges of one
+        // isometry, a compare each); only a group some lane passes goes
+        // value by value and appends (wave ballot + mbcnt).  Sizes <= 8: see
+        // below.
         constexpr int DRAIN = Queue<S>::DRAIN;
         constexpr int GW = DRAIN < 8 ? DRAIN : 8; // values per compare group
         static_assert(16 % DRAIN == 0 && DRAIN % GW == 0, "groups tile a row tile's 16 values");
@@ -1473,37 +1474,70 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(S <= 4 ? 4 
                            [&]() { return p * 8; });
                 return;
             }
+            // Size 16: drain interval grp = range grp of the lane half, its
+            // four isometries' values tested by their maximum against the
+            // range's limit (which does not depend on the isometry), one
+            // compare and one scalar test; only a range some lane passes
+            // goes value by value.  Size 32 keeps groups of four ranges of
+            // one isometry, a compare per value: the range grouping raised
+            // its register use to the limit and measured slower there
+            // (0.122 -> 0.127 ms, where size 16 went 0.088 -> 0.085 ms).
 #pragma unroll
             for (int grp = 0; grp < 16 / DRAIN; grp++) {
+                if constexpr (S == 16) {
+                    static_assert(DRAIN == 4, "one range (four isometries) per drain interval");
+                    const int i = grp;
+                    int32_t v[4];
 #pragma unroll
-                for (int g0 = 0; g0 < DRAIN; g0 += GW) {
-                    int32_t v[GW];
-                    unsigned long long msk[GW], any = 0;
+                    for (int e = 0; e < 4; e++) v[e] = 4 * ah[i + 4 * e] + al[i + 4 * e];
+                    const int32_t mx = max(max(v[0], v[1]), max(v[2], v[3]));
+                    if (__ballot(mx > lim[i]) == 0) continue;
 #pragma unroll
-                    for (int e = 0; e < GW; e++) {
-                        const int t = grp * DRAIN + g0 + e;
-                        v[e] = 4 * ah[t] + al[t];
-                        msk[e] = __ballot(v[e] > lim[t & 3]);
-                        any |= msk[e];
-                    }
-                    if (any == 0) continue;
-                    // The compare masks stay in scalar registers: a value
-                    // without survivors costs one scalar test here.
-#pragma unroll
-                    for (int e = 0; e < GW; e++) {
-                        const int t = grp * DRAIN + g0 + e, i = t & 3;
-                        const unsigned long long m = msk[e];
+                    for (int e = 0; e < 4; e++) {
+                        const bool sv = v[e] > lim[i];
+                        const unsigned long long m = __ballot(sv);
                         if (m) {
-                            const bool sv = v[e] > lim[i]; // the compare behind m
                             const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
                                 (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
                             if (sv) {
                                 const uint32_t slot = 4 * h + i;
                                 q[qn + pre] = make_uint2((uint32_t)(v[e] + off_of(i) + o128) | (slot << 28),
-                                                         p * 8 + 4 * tt + (t >> 2));
+                                                         p * 8 + 4 * tt + e);
                             }
                             qn += (uint32_t)__popcll(m);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int g0 = 0; g0 < DRAIN; g0 += GW) {
+                        int32_t v[GW];
+                        unsigned long long msk[GW], any = 0;
+#pragma unroll
+                        for (int e = 0; e < GW; e++) {
+                            const int t = grp * DRAIN + g0 + e;
+                            v[e] = 4 * ah[t] + al[t];
+                            msk[e] = __ballot(v[e] > lim[t & 3]);
+                            any |= msk[e];
+                        }
+                        if (any == 0) continue;
+                        // The compare masks stay in scalar registers: a value
+                        // without survivors costs one scalar test here.
+#pragma unroll
+                        for (int e = 0; e < GW; e++) {
+                            const int t = grp * DRAIN + g0 + e, i = t & 3;
+                            const unsigned long long m = msk[e];
+                            if (m) {
+                                const bool sv = v[e] > lim[i]; // the compare behind m
+                                const uint32_t pre = __builtin_amdgcn_mbcnt_hi(
+                                    (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                                if (sv) {
+                                    const uint32_t slot = 4 * h + i;
+                                    q[qn + pre] = make_uint2((uint32_t)(v[e] + off_of(i) + o128) | (slot << 28),
+                                                             p * 8 + 4 * tt + (t >> 2));
+                                }
+                                qn += (uint32_t)__popcll(m);
                             }
+                        }
                     }
                 }
                 // Drain full batches; keep the tail (< 64) for later.
