@@ -117,7 +117,9 @@ void mars_hip_close(mars_hip_ctx *ctx);
  * -- every aligned non-forced node from max_size down to min_size, plus the
  * border nodes down to size 2 (encode.rs:1330-1362, 1865-1919) -- once per
  * stride in shifts[0..n_shifts).  Single-threaded; copies its inputs.
- * Supported sizes: 2..32.
+ * Supported sizes: 2..32.  MARS_HIP_E_UNSUPPORTED for a plane whose window
+ * copies exceed 32-bit offsets or a (size, stride) table of more than
+ * 2^27 - 32 domain positions (32-bit byte offsets of its records).
  */
 int mars_hip_prepare(mars_hip_ctx *ctx, const uint8_t *px, uint32_t w, uint32_t h,
                      const mars_hip_params *p, const uint32_t *shifts, uint32_t n_shifts);
